@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""ISF step throughput on one MI355X (the 256x128x128 jet) + roofline + CPU baseline.
+
+One step = the reference simloop body (test/runtests.jl:45-64): Schroedinger flow,
+normalize, pressure projection, particle spawn, jet constraint, second projection,
+hbar-scaled velocity one-form with 1/d, RK4 staggered advection of every particle slot.
+
+Workload: grid 256x128x128, box 8x4x4, hbar 0.1, dt 1/48, psi = (1, 0.01) normalized,
+jet velocity (1,0,0), nozzle radius 0.5 / length 0.5 at (0.3, 1.966, 2.066),
+10 constraint iterations, 1,000,000 particle slots filled uniformly in the nozzle
+cylinder (Philox4x32-10, seed 20261203), 1000 ring-spawned particles per step.
+Inputs are resident in device memory before the timed region starts.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "schroedingerssmoke.jl_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "ISF steps/s at 256×128×128 (fwd, fwd+adjoint, batched) + % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOAD = dict(vol_size=(8, 4, 4), dims=(256, 128, 128), hbar=0.1, dt=1.0 / 48.0, jet_velocity=(1.0, 0.0, 0.0),
+                nozzle_cen=(0.3, 2 - 0.034, 2 + 0.066), nozzle_len=0.5, nozzle_rad=0.5, n_spawn=1000,
+                n_slots=1_000_000, n_init=1_000_000, constraint_iters=10, seed=20261203, phase_mode=0)
+
+
+def stage_bytes(nx, ny, nz, nslots, n_spawn):
+    """Algorithmic (compulsory) HBM bytes of one launch of every stage of a step: each
+    array the stage must read or write, counted once (tables and index math are free)."""
+    N = nx * ny * nz
+    half = 8 * (nx // 2 + 1) * ny * nz  # Poisson half spectrum, complex64
+    b = {
+        "flow_x_fwd": 16 * N + 16 * N,
+        "flow_plane": 16 * N + 16 * N,
+        "flow_x_inv_normalize": 16 * N + 16 * N,
+        "spawn": 12 * n_spawn,
+        "velocity_sharp": 16 * N + 12 * N,
+        "advect": 24 * nslots + 12 * N,
+    }
+    for p in ("p1", "p2"):
+        b[f"{p}_div"] = 16 * N + 4 * N
+        b[f"{p}_r2c_x"] = 4 * N + half
+        b[f"{p}_plane"] = 2 * half
+    b["p1_c2r_x_gauge_restrict"] = half + 32 * N
+    b["p2_c2r_x_gauge"] = half + 32 * N
+    return b
+
+
+def b_alg_step(N, nslots):
+    """Compulsory bytes of one step: 248 B/cell + 24 B/particle slot."""
+    return 248 * N + 24 * nslots
+
+
+def setup_gpu(w):
+    from isf_amd import ISF, DeviceArray
+    f32 = np.float32
+    isf = ISF(w["vol_size"], w["dims"], w["hbar"], w["dt"])
+    nx, ny, nz = w["dims"]
+    psi_h = np.zeros((nz, ny, nx, 4), f32)
+    psi_h[..., 0] = 1.0
+    psi_h[..., 2] = f32(0.01)
+    psi = isf.psi_array(psi_h)
+    isf.normalize_(psi)
+    v = np.array(w["jet_velocity"], f32)
+    hb = f32(w["hbar"])
+    kvec = v / hb
+    omega = f32(np.sum(v ** f32(2)) / (f32(2) * hb))
+    cen = np.array(w["nozzle_cen"], np.float64).astype(f32)
+    isf.jet_configure(kvec, omega, cen, f32(w["nozzle_len"]), f32(w["nozzle_rad"]), w["n_spawn"], w["seed"],
+                      w["phase_mode"])
+    isf.constrain_(psi, w["constraint_iters"])
+    parts = DeviceArray.from_host(np.full((w["n_slots"], 3), -1.0, f32))
+    isf.fill_nozzle_(parts, 0, w["n_init"], w["seed"])
+    isf.synchronize()
+    return isf, psi, parts
+
+
+def cpu_baseline(w, budget_s=12.0):
+    """The FP32 oracle (project-written C/OpenMP restatement of the reference's Array
+    path) on the same workload, timed on this host: steps after the constraint setup,
+    until budget_s seconds of stepping have elapsed (at least 2 steps)."""
+    from oracle.oracle import JetConfig, OracleRun
+    cfg = JetConfig(**{k: w[k] for k in JetConfig.__dataclass_fields__ if k in w})
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    run = OracleRun(cfg, np.float32)
+    run.step(1)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        run.step(1)
+        n += 1
+        el = time.perf_counter() - t0
+        if (el >= budget_s and n >= 2) or n >= 50:
+            break
+    return {"value": n / el, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 oracle (oracle/isf_oracle.c, OpenMP {threads} threads, radix-2 FFT) on the same "
+                      f"256x128x128 / 1M-particle workload after 10 constraint iterations + 1 untimed step: "
+                      f"{n} steps in {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=90)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU slab decomposition is a design, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+
+    w = WORKLOAD
+    nx, ny, nz = w["dims"]
+    N = nx * ny * nz
+    print(f"[bench] setup {nx}x{ny}x{nz}, {w['n_slots']} particle slots", file=sys.stderr, flush=True)
+    isf, psi, parts = setup_gpu(w)
+    it = 0
+    for _ in range(args.warmup):
+        it += 1
+        isf.step_(psi, parts, it)
+    isf.synchronize()
+    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    isf.timer_start()
+    for _ in range(args.steps):
+        it += 1
+        isf.step_(psi, parts, it)
+    ms = isf.timer_stop()
+    isf.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = ms / args.steps
+    steps_s = 1000.0 / ms_per_step
+    norm = isf.step_norm2()
+    print(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step (events), wall {wall:.3f} s, "
+          f"norm2/N = {norm / N:.9f}", file=sys.stderr, flush=True)
+
+    # per-stage HIP-event timing of the same step on the same stream
+    acc = {}
+    for _ in range(args.profile_steps):
+        it += 1
+        for k, v in isf.step_profile(psi, parts, it).items():
+            acc.setdefault(k, []).append(v)
+    stage_ms = {k: float(np.mean(v)) for k, v in acc.items()}
+    sb = stage_bytes(nx, ny, nz, w["n_slots"], w["n_spawn"])
+    dom = max(stage_ms, key=stage_ms.get)
+    achieved = sb[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    stages = {k: {"ms": round(stage_ms[k], 5), "GB": round(sb[k] / 1e9, 4),
+                  "GB_s": round(sb[k] / (stage_ms[k] * 1e-3) / 1e9, 1)} for k in stage_ms}
+    balg = b_alg_step(N, w["n_slots"])
+
+    out = {
+        "metric": METRIC,
+        "value": round(steps_s, 2),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "plain single jet 256x128x128 (box 8x4x4, hbar 0.1, dt 1/48), 1M particles "
+                               "seeded in the nozzle, 1000 spawned per step, forward step",
+                   "grid": [nx, ny, nz], "particles": w["n_slots"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "step_roofline": {"B_alg_per_step": balg, "achieved_GB_s": round(balg * steps_s / 1e9, 1),
+                          "frac": round(balg * steps_s / 1e9 / HBM_PEAK_GBS, 4)},
+        "stages": stages,
+    }
+    if not args.no_cpu:
+        print("[bench] CPU baseline leg", file=sys.stderr, flush=True)
+        cb = cpu_baseline(w)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = round(steps_s / cb["value"], 1)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,133 @@
+/*
+ * isf.h -- C ABI of libisf, the MI355X (gfx950) implementation of the incompressible
+ * Schroedinger flow (ISF) step of SchroedingersSmoke.jl.
+ *
+ * Every entry point replaces one function of the reference's generic/GPU path; the
+ * reference file:line is given beside it (paths relative to the SchroedingersSmoke.jl
+ * repository root).  Conventions:
+ *   - Return value: 0 on success, a negative ISF_E* code on failure; the message is
+ *     available from isf_last_error() (thread-local).  Nothing aborts.
+ *   - Array arguments are DEVICE pointers (from isf_malloc or any HIP allocation on the
+ *     current device) in the reference's layouts, x fastest (column-major):
+ *       psi       4 floats per cell  {re1, im1, re2, im2}   (ComplexF32, ComplexF32)
+ *       velocity  3 floats per cell  {vx, vy, vz}           NTuple{3,Float32}
+ *       f, q      1 float per cell
+ *       isjet     1 byte per cell
+ *       particles 3 floats per slot  {x, y, z}              NTuple{3,Float32}
+ *   - All work is enqueued asynchronously on the handle's HIP stream; isf_synchronize
+ *     waits for it (GPUArrays.synchronize, test/runtests.jl:65).
+ *   - One host thread per handle.
+ */
+#ifndef ISF_H
+#define ISF_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISF_OK 0
+#define ISF_EINVAL (-1)      /* bad argument (null pointer, unsupported size, ...) */
+#define ISF_EHIP (-2)        /* HIP runtime error */
+#define ISF_ENOMEM (-3)      /* device allocation failed */
+#define ISF_EHANDLE (-4)     /* null or destroyed handle */
+
+/* phase_mode values of the jet constraint */
+#define ISF_PHASE_DOT 0      /* phase = k.x - omega t (src/MatlabPort driver, examples/matlabport/simple.jl:47,107) */
+#define ISF_PHASE_LEGACY 1   /* phase = sum(k .+ x) - omega t (reference tuple_dot, src/particles.jl:1,15) */
+
+typedef struct isf_handle* isf_t;
+
+/* ---- library / memory -------------------------------------------------------- */
+const char* isf_last_error(void);
+const char* isf_version(void);
+int isf_device_count(int* out);
+int isf_malloc(void** ptr, size_t bytes);
+int isf_free(void* ptr);
+int isf_memcpy_h2d(void* dst, const void* src, size_t bytes);  /* ArrayType(hostarray), test/runtests.jl:19 */
+int isf_memcpy_d2h(void* dst, const void* src, size_t bytes);  /* Array(devarray), examples/jet.jl:83 */
+int isf_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int isf_memset(void* dst, int value, size_t bytes);
+
+/* ---- ISF instance ------------------------------------------------------------- */
+/* ISF{ArrayType,UInt32,Float32}(physical_size, dims, hbar, dt), src/parallel.jl:41-98.
+ * res: grid_res (powers of two; x in 16..512, y and z in 16..128 with ny*nz <= 16384);
+ * box: physical_size (integers, src/parallel.jl:12,45).  Builds the Schroedinger mask
+ * and Poisson factor (:47-63) as per-axis FP64-derived tables, the FFT twiddles and
+ * the scratch arrays. */
+int isf_create(isf_t* out, const int res[3], const int box[3], float hbar, float dt);
+int isf_destroy(isf_t h);
+int isf_synchronize(isf_t h);                                         /* test/runtests.jl:65 */
+/* Fields the drivers read (test/runtests.jl:25,44,59-63): d, dt, hbar, grid_res. */
+int isf_get_grid(isf_t h, int res[3], float d[3], float box[3], float* hbar, float* dt);
+/* isf.velocity (src/parallel.jl:14): device pointer of the handle's velocity field
+ * (3 floats per cell), written by isf_step and by isf_velocity_one_form(..., NULL). */
+int isf_velocity_ptr(isf_t h, float** out);
+
+/* ---- reference operators -------------------------------------------------------- */
+/* schroedinger_flow!(isf, psi), src/parallel.jl:176-187 (in place). */
+int isf_schroedinger_flow(isf_t h, float* psi);
+/* psi .= normalize_psi.(psi), src/parallel.jl:162-165, test/runtests.jl:49. */
+int isf_normalize(isf_t h, float* psi);
+/* velocity_one_form!(isf, psi, hbar), src/parallel.jl:100-122; sharp != 0 also applies
+ * the driver's in-place StaggeredSharp v .*= 1 ./ d (test/runtests.jl:61-62).
+ * vel == NULL writes the handle's own velocity field. */
+int isf_velocity_one_form(isf_t h, const float* psi, float hbar, int sharp, float* vel);
+/* div!(isf), src/parallel.jl:131-152: f from a velocity field (vel == NULL: handle's). */
+int isf_div(isf_t h, const float* vel, float* f);
+/* poisson_solve(isf), src/parallel.jl:168-174: q = ifft(fac .* fft(f)).  q is returned
+ * real (1 float per cell); the reference's imaginary part is FFT rounding noise. */
+int isf_poisson_solve(isf_t h, const float* f, float* q);
+/* psi .= gauge_transform.(psi, q), src/parallel.jl:125-129,159 with real q. */
+int isf_gauge_transform(isf_t h, float* psi, const float* q);
+/* pressure_project!(isf, psi), src/parallel.jl:155-160. */
+int isf_pressure_project(isf_t h, float* psi);
+
+/* isjet.(isf.positions, cen, len, rad), src/particles.jl:6-10, test/runtests.jl:25. */
+int isf_isjet(isf_t h, const float cen[3], float len, float rad, unsigned char* out);
+/* restrict_velocity!(isf, psi, kvec, isjet, omgterm), src/particles.jl:12-27.
+ * isjet == NULL evaluates the nozzle predicate of isf_jet_configure analytically. */
+int isf_restrict_velocity(isf_t h, float* psi, const unsigned char* isjet, const float kvec[3], float omgterm,
+                          int phase_mode);
+/* add_particles!(particles, first+1:first+n, cen, rad), src/particles.jl:34-94.
+ * 0-based slots first..first+n-1 (wrapped modulo nslots); the draws come from
+ * Philox4x32-10 with counter (slot, step, 0, 0) and key = seed. */
+int isf_add_particles(isf_t h, float* parts, long long nslots, long long first, long long n, const float cen[3],
+                      float rad, unsigned long long seed, unsigned step);
+/* staggered_advect!(particles, isf), src/parallel.jl:190-261 (vel == NULL: handle's). */
+int isf_staggered_advect(isf_t h, float* parts, long long nslots, const float* vel);
+
+/* ---- fused step (the simloop body, test/runtests.jl:45-64) -------------------------- */
+/* Jet parameters of the driver (test/runtests.jl:7-12,23-25): kvec = v / hbar,
+ * omega = |v|^2 / (2 hbar), nozzle centre/length/radius, particles spawned per step,
+ * spawn seed and phase mode. */
+int isf_jet_configure(isf_t h, const float kvec[3], float omega, const float cen[3], float len, float rad,
+                      long long n_spawn, unsigned long long seed, int phase_mode);
+/* Fill slots first..first+n-1 uniformly inside the configured nozzle cylinder (initial
+ * particle cloud of the at-scale workloads; no reference counterpart). */
+int isf_fill_nozzle(isf_t h, float* parts, long long nslots, long long first, long long n, unsigned long long seed);
+/* The 10 constraint iterations of the setup, test/runtests.jl:27-30. */
+int isf_constrain(isf_t h, float* psi, int iterations);
+/* One simulation step number iter (1-based, t = iter*dt):
+ * flow, normalize, project, spawn, restrict, project, one-form(hbar) * 1/d, RK4 advect. */
+int isf_step(isf_t h, float* psi, float* parts, long long nslots, int iter);
+/* FP64 sum over cells of |psi1|^2 + |psi2|^2 (no reference counterpart; health check). */
+int isf_norm2(isf_t h, const float* psi, double* out);
+/* The same sum accumulated by the last isf_step (read after isf_synchronize). */
+int isf_step_norm2(isf_t h, double* out);
+
+/* ---- timing helpers (HIP events on the handle's stream) ----------------------------- */
+/* Number and names of the kernel stages of isf_step, in launch order. */
+int isf_step_stage_count(void);
+const char* isf_step_stage_name(int i);
+/* isf_step with a HIP event between consecutive stages; waits for the step and writes
+ * the duration of every stage (ms) to stage_ms[0 .. isf_step_stage_count()-1]. */
+int isf_step_profile(isf_t h, float* psi, float* parts, long long nslots, int iter, float* stage_ms);
+int isf_timer_start(isf_t h);
+int isf_timer_stop(isf_t h, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ISF_H */

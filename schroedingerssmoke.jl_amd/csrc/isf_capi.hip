@@ -1,0 +1,606 @@
+// isf_capi.hip -- host side of libisf: the C ABI declared in include/isf.h.
+//
+// The handle plays the role of the reference's ISF struct (src/parallel.jl:4-35): grid
+// constants, the spectral tables (here per-axis, derived in FP64), the scratch arrays
+// and the velocity field, plus the HIP stream every call is enqueued on.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/isf.h"
+#include "isf_internal.hpp"
+
+using namespace isf;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+static int hip_fail(hipError_t e, const char* what) {
+    return fail(ISF_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+#define HIP_CHECK(expr, what)                          \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+#define LAUNCH_CHECK(expr, what)                                      \
+    do {                                                              \
+        int e_ = (expr);                                              \
+        if (e_ != 0) return hip_fail((hipError_t)e_, what);           \
+    } while (0)
+
+static constexpr uint32_t kMagic = 0x15F0CAFEu;
+
+struct isf_handle {
+    uint32_t magic;
+    int device;
+    GridInfo g;
+    float hbar, dt;
+    size_t N;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    void* table_mem;
+    DevTables tb;
+    float2* S;          // Schroedinger spectrum, 2N complex; f and P alias it in projections
+    float* vel;         // velocity field, 3N floats
+    double* norm_parts; // per-block FP64 partial sums
+    size_t norm_nparts;
+    double* norm_out;
+    // jet (isf_jet_configure)
+    bool jet_set;
+    JetParams jet;
+    float omega;
+    long long n_spawn;
+    unsigned long long seed;
+};
+
+static float* f_of(isf_t h) { return reinterpret_cast<float*>(h->S); }
+static float2* P_of(isf_t h) { return reinterpret_cast<float2*>(reinterpret_cast<char*>(h->S) + 4 * h->N); }
+
+static int check_handle(isf_t h) {
+    if (!h || h->magic != kMagic) return fail(ISF_EHANDLE, "null or destroyed isf handle");
+    return ISF_OK;
+}
+#define CHECK_H(h)                              \
+    do {                                        \
+        int r_ = check_handle(h);               \
+        if (r_) return r_;                      \
+    } while (0)
+#define CHECK_PTR(p, name)                                                   \
+    do {                                                                     \
+        if (!(p)) return fail(ISF_EINVAL, "%s must not be NULL", name);      \
+    } while (0)
+
+extern "C" {
+
+const char* isf_last_error(void) { return g_err.c_str(); }
+const char* isf_version(void) { return "libisf 0.1 (gfx950)"; }
+
+int isf_device_count(int* out) {
+    CHECK_PTR(out, "out");
+    HIP_CHECK(hipGetDeviceCount(out), "hipGetDeviceCount");
+    return ISF_OK;
+}
+int isf_malloc(void** ptr, size_t bytes) {
+    CHECK_PTR(ptr, "ptr");
+    *ptr = nullptr;
+    if (bytes == 0) return ISF_OK;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) return fail(ISF_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return ISF_OK;
+}
+int isf_free(void* ptr) {
+    if (ptr) HIP_CHECK(hipFree(ptr), "hipFree");
+    return ISF_OK;
+}
+int isf_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return ISF_OK;
+}
+int isf_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return ISF_OK;
+}
+int isf_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+    HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+    return ISF_OK;
+}
+int isf_memset(void* dst, int value, size_t bytes) {
+    HIP_CHECK(hipMemset(dst, value, bytes), "hipMemset");
+    return ISF_OK;
+}
+
+static bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int isf_create(isf_t* out, const int res[3], const int box[3], float hbar, float dt) {
+    CHECK_PTR(out, "out");
+    CHECK_PTR(res, "res");
+    CHECK_PTR(box, "box");
+    *out = nullptr;
+    for (int a = 0; a < 3; a++) {
+        if (!pow2(res[a])) return fail(ISF_EINVAL, "grid_res[%d] = %d is not a power of two", a, res[a]);
+        if (box[a] <= 0) return fail(ISF_EINVAL, "physical_size[%d] = %d must be positive", a, box[a]);
+    }
+    if (!x_len_supported(res[0])) return fail(ISF_EINVAL, "grid_res[0] = %d unsupported (16..512)", res[0]);
+    if (!plane_supported(res[1], res[2]))
+        return fail(ISF_EINVAL, "grid_res[1:2] = (%d, %d) unsupported (16..128 each, ny*nz <= 16384)", res[1], res[2]);
+    if (!(hbar > 0.0f) || !(dt > 0.0f)) return fail(ISF_EINVAL, "hbar and dt must be positive");
+
+    isf_handle* h = new isf_handle();
+    std::memset(h, 0, sizeof *h);
+    h->magic = kMagic;
+    hipError_t e = hipGetDevice(&h->device);
+    if (e != hipSuccess) {
+        delete h;
+        return hip_fail(e, "hipGetDevice");
+    }
+    GridInfo& g = h->g;
+    g.nx = res[0];
+    g.ny = res[1];
+    g.nz = res[2];
+    // d = VF(physical_size ./ grid_res), src/parallel.jl:46
+    g.dx = (float)((double)box[0] / res[0]);
+    g.dy = (float)((double)box[1] / res[1]);
+    g.dz = (float)((double)box[2] / res[2]);
+    g.Lx = (float)box[0];
+    g.Ly = (float)box[1];
+    g.Lz = (float)box[2];
+    h->hbar = hbar;
+    h->dt = dt;
+    h->N = (size_t)g.nx * g.ny * g.nz;
+
+    // ---- per-axis tables (host, FP64)
+    const int nmax = 512;
+    std::vector<float2> tw(2 * nmax);
+    for (int n = 2; n <= nmax; n *= 2)
+        for (int m = 0; m < n; m++) {
+            const double a = -2.0 * M_PI * (double)m / (double)n;
+            tw[n + m] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+    // Schroedinger mask, src/parallel.jl:49,58-62 in FFT order: exp(i c k^2) per axis,
+    // c = -4 pi^2 hbar dt / 2, k = j/L (j < n/2) or (j-n)/L; 1/N folded into the x factor.
+    const double c = -4.0 * M_PI * M_PI * (double)hbar * (double)dt / 2.0;
+    const int n3[3] = {g.nx, g.ny, g.nz};
+    const double dd[3] = {(double)g.dx, (double)g.dy, (double)g.dz};
+    std::vector<float2> mask[3];
+    std::vector<double> s2[3];
+    for (int a = 0; a < 3; a++) {
+        mask[a].resize(n3[a]);
+        s2[a].resize(n3[a]);
+        const double scale = a == 0 ? 1.0 / (double)h->N : 1.0;
+        for (int j = 0; j < n3[a]; j++) {
+            const double k = (j < n3[a] / 2 ? j : j - n3[a]) / (double)box[a];
+            const double ph = c * k * k;
+            mask[a][j] = make_float2((float)(std::cos(ph) * scale), (float)(std::sin(ph) * scale));
+            const double s = std::sin(M_PI * j / n3[a]) / dd[a];  // src/parallel.jl:53
+            s2[a][j] = s * s;
+        }
+    }
+    size_t bytes = tw.size() * sizeof(float2);
+    size_t off_mask[3], off_s2[3];
+    for (int a = 0; a < 3; a++) {
+        off_mask[a] = bytes;
+        bytes += ((n3[a] * sizeof(float2) + 255) / 256) * 256;
+    }
+    for (int a = 0; a < 3; a++) {
+        off_s2[a] = bytes;
+        bytes += ((n3[a] * sizeof(double) + 255) / 256) * 256;
+    }
+    auto cleanup = [&](int code) {
+        isf_destroy(h);
+        return code;
+    };
+    e = hipMalloc(&h->table_mem, bytes);
+    if (e != hipSuccess) return cleanup(fail(ISF_ENOMEM, "tables: %s", hipGetErrorString(e)));
+    char* base = (char*)h->table_mem;
+    std::vector<char> host(bytes, 0);
+    std::memcpy(host.data(), tw.data(), tw.size() * sizeof(float2));
+    for (int a = 0; a < 3; a++) {
+        std::memcpy(host.data() + off_mask[a], mask[a].data(), n3[a] * sizeof(float2));
+        std::memcpy(host.data() + off_s2[a], s2[a].data(), n3[a] * sizeof(double));
+    }
+    e = hipMemcpy(base, host.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "tables upload"));
+    h->tb.tw = (float2*)base;
+    h->tb.mask_x = (float2*)(base + off_mask[0]);
+    h->tb.mask_y = (float2*)(base + off_mask[1]);
+    h->tb.mask_z = (float2*)(base + off_mask[2]);
+    h->tb.s2x = (double*)(base + off_s2[0]);
+    h->tb.s2y = (double*)(base + off_s2[1]);
+    h->tb.s2z = (double*)(base + off_s2[2]);
+
+    // ---- scratch
+    if ((e = hipMalloc(&h->S, 16 * h->N)) != hipSuccess) return cleanup(fail(ISF_ENOMEM, "scratch: %s", hipGetErrorString(e)));
+    if ((e = hipMalloc(&h->vel, 12 * h->N)) != hipSuccess) return cleanup(fail(ISF_ENOMEM, "velocity: %s", hipGetErrorString(e)));
+    h->norm_nparts = c2r_x_blocks(g) > 1024 ? c2r_x_blocks(g) : 1024;
+    if ((e = hipMalloc(&h->norm_parts, sizeof(double) * (h->norm_nparts + 1))) != hipSuccess)
+        return cleanup(fail(ISF_ENOMEM, "norm partials: %s", hipGetErrorString(e)));
+    h->norm_out = h->norm_parts + h->norm_nparts;
+    if ((e = hipMemset(h->vel, 0, 12 * h->N)) != hipSuccess) return cleanup(hip_fail(e, "hipMemset"));
+    if ((e = hipMemset(h->norm_parts, 0, sizeof(double) * (h->norm_nparts + 1))) != hipSuccess)
+        return cleanup(hip_fail(e, "hipMemset"));
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipStreamCreate"));
+    if ((e = hipEventCreate(&h->ev0)) != hipSuccess) return cleanup(hip_fail(e, "hipEventCreate"));
+    if ((e = hipEventCreate(&h->ev1)) != hipSuccess) return cleanup(hip_fail(e, "hipEventCreate"));
+    *out = h;
+    return ISF_OK;
+}
+
+int isf_destroy(isf_t h) {
+    CHECK_H(h);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->S) (void)hipFree(h->S);
+    if (h->vel) (void)hipFree(h->vel);
+    if (h->norm_parts) (void)hipFree(h->norm_parts);
+    if (h->table_mem) (void)hipFree(h->table_mem);
+    h->magic = 0;
+    delete h;
+    return ISF_OK;
+}
+
+int isf_synchronize(isf_t h) {
+    CHECK_H(h);
+    HIP_CHECK(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+    return ISF_OK;
+}
+
+int isf_get_grid(isf_t h, int res[3], float d[3], float box[3], float* hbar, float* dt) {
+    CHECK_H(h);
+    if (res) {
+        res[0] = h->g.nx;
+        res[1] = h->g.ny;
+        res[2] = h->g.nz;
+    }
+    if (d) {
+        d[0] = h->g.dx;
+        d[1] = h->g.dy;
+        d[2] = h->g.dz;
+    }
+    if (box) {
+        box[0] = h->g.Lx;
+        box[1] = h->g.Ly;
+        box[2] = h->g.Lz;
+    }
+    if (hbar) *hbar = h->hbar;
+    if (dt) *dt = h->dt;
+    return ISF_OK;
+}
+
+int isf_velocity_ptr(isf_t h, float** out) {
+    CHECK_H(h);
+    CHECK_PTR(out, "out");
+    *out = h->vel;
+    return ISF_OK;
+}
+
+// ------------------------------------------------------------------ operators
+
+static int flow(isf_t h, float* psi, bool normalize) {
+    float4* p = reinterpret_cast<float4*>(psi);
+    LAUNCH_CHECK(launch_flow_x_fwd(h->g, p, h->S, h->tb, h->stream), "flow x forward");
+    LAUNCH_CHECK(launch_flow_plane(h->g, h->S, h->tb, h->stream), "flow plane");
+    LAUNCH_CHECK(launch_flow_x_inv(h->g, h->S, p, normalize, h->tb, h->stream), "flow x inverse");
+    return ISF_OK;
+}
+
+static double poisson_scale(isf_t h) { return -0.25 * 2.0 / (double)h->N; }
+
+// velocity one-form (hbar=1) + div + Poisson + gauge; mode C2R_GAUGE(_RESTRICT)
+static int project(isf_t h, float* psi, int c2r_mode, const JetParams* jet, double* parts) {
+    float4* p = reinterpret_cast<float4*>(psi);
+    LAUNCH_CHECK(launch_div_psi(h->g, p, f_of(h), h->stream), "div(one_form(psi))");
+    LAUNCH_CHECK(launch_r2c_x(h->g, f_of(h), P_of(h), h->tb, h->stream), "poisson x forward");
+    LAUNCH_CHECK(launch_poisson_plane(h->g, P_of(h), poisson_scale(h), h->tb, h->stream), "poisson plane");
+    LAUNCH_CHECK(launch_c2r_x(h->g, P_of(h), c2r_mode, nullptr, p, jet, parts, h->tb, h->stream), "poisson x inverse + gauge");
+    return ISF_OK;
+}
+
+int isf_schroedinger_flow(isf_t h, float* psi) {
+    CHECK_H(h);
+    CHECK_PTR(psi, "psi");
+    return flow(h, psi, false);
+}
+
+int isf_normalize(isf_t h, float* psi) {
+    CHECK_H(h);
+    CHECK_PTR(psi, "psi");
+    LAUNCH_CHECK(launch_normalize(h->g, reinterpret_cast<float4*>(psi), h->stream), "normalize");
+    return ISF_OK;
+}
+
+int isf_velocity_one_form(isf_t h, const float* psi, float hbar, int sharp, float* vel) {
+    CHECK_H(h);
+    CHECK_PTR(psi, "psi");
+    LAUNCH_CHECK(launch_velocity(h->g, reinterpret_cast<const float4*>(psi), hbar, sharp, vel ? vel : h->vel, h->stream),
+                 "velocity_one_form");
+    return ISF_OK;
+}
+
+int isf_div(isf_t h, const float* vel, float* f) {
+    CHECK_H(h);
+    CHECK_PTR(f, "f");
+    LAUNCH_CHECK(launch_div_vel(h->g, vel ? vel : h->vel, f, h->stream), "div");
+    return ISF_OK;
+}
+
+int isf_poisson_solve(isf_t h, const float* f, float* q) {
+    CHECK_H(h);
+    CHECK_PTR(f, "f");
+    CHECK_PTR(q, "q");
+    LAUNCH_CHECK(launch_r2c_x(h->g, f, P_of(h), h->tb, h->stream), "poisson x forward");
+    LAUNCH_CHECK(launch_poisson_plane(h->g, P_of(h), poisson_scale(h), h->tb, h->stream), "poisson plane");
+    LAUNCH_CHECK(launch_c2r_x(h->g, P_of(h), 0, q, nullptr, nullptr, nullptr, h->tb, h->stream), "poisson x inverse");
+    return ISF_OK;
+}
+
+int isf_gauge_transform(isf_t h, float* psi, const float* q) {
+    CHECK_H(h);
+    CHECK_PTR(psi, "psi");
+    CHECK_PTR(q, "q");
+    LAUNCH_CHECK(launch_gauge(h->g, reinterpret_cast<float4*>(psi), q, h->stream), "gauge_transform");
+    return ISF_OK;
+}
+
+int isf_pressure_project(isf_t h, float* psi) {
+    CHECK_H(h);
+    CHECK_PTR(psi, "psi");
+    return project(h, psi, 1, nullptr, nullptr);
+}
+
+static JetParams jet_from(const float cen[3], float len, float rad) {
+    JetParams j{};
+    j.cx = cen[0];
+    j.cy = cen[1];
+    j.cz = cen[2];
+    j.len = len;
+    j.rad = rad;
+    return j;
+}
+
+int isf_isjet(isf_t h, const float cen[3], float len, float rad, unsigned char* out) {
+    CHECK_H(h);
+    CHECK_PTR(cen, "cen");
+    CHECK_PTR(out, "out");
+    LAUNCH_CHECK(launch_isjet(h->g, jet_from(cen, len, rad), out, h->stream), "isjet");
+    return ISF_OK;
+}
+
+int isf_restrict_velocity(isf_t h, float* psi, const unsigned char* isjet, const float kvec[3], float omgterm,
+                          int phase_mode) {
+    CHECK_H(h);
+    CHECK_PTR(psi, "psi");
+    CHECK_PTR(kvec, "kvec");
+    if (phase_mode != ISF_PHASE_DOT && phase_mode != ISF_PHASE_LEGACY) return fail(ISF_EINVAL, "bad phase_mode %d", phase_mode);
+    if (!isjet && !h->jet_set) return fail(ISF_EINVAL, "isjet is NULL and no nozzle was configured");
+    JetParams j = h->jet_set ? h->jet : JetParams{};
+    j.kx = kvec[0];
+    j.ky = kvec[1];
+    j.kz = kvec[2];
+    j.omgterm = omgterm;
+    j.phase_mode = phase_mode;
+    LAUNCH_CHECK(launch_restrict(h->g, reinterpret_cast<float4*>(psi), isjet, j, h->stream), "restrict_velocity");
+    return ISF_OK;
+}
+
+int isf_add_particles(isf_t h, float* parts, long long nslots, long long first, long long n, const float cen[3],
+                      float rad, unsigned long long seed, unsigned step) {
+    CHECK_H(h);
+    CHECK_PTR(parts, "parts");
+    CHECK_PTR(cen, "cen");
+    if (nslots <= 0 || first < 0 || n < 0) return fail(ISF_EINVAL, "bad particle range");
+    LAUNCH_CHECK(launch_spawn(parts, nslots, first % nslots, n, jet_from(cen, 0.0f, rad), seed, step, h->stream),
+                 "add_particles");
+    return ISF_OK;
+}
+
+int isf_staggered_advect(isf_t h, float* parts, long long nslots, const float* vel) {
+    CHECK_H(h);
+    CHECK_PTR(parts, "parts");
+    if (nslots < 0) return fail(ISF_EINVAL, "bad nslots");
+    LAUNCH_CHECK(launch_advect(h->g, parts, nslots, vel ? vel : h->vel, h->dt, h->stream), "staggered_advect");
+    return ISF_OK;
+}
+
+int isf_jet_configure(isf_t h, const float kvec[3], float omega, const float cen[3], float len, float rad,
+                      long long n_spawn, unsigned long long seed, int phase_mode) {
+    CHECK_H(h);
+    CHECK_PTR(kvec, "kvec");
+    CHECK_PTR(cen, "cen");
+    if (phase_mode != ISF_PHASE_DOT && phase_mode != ISF_PHASE_LEGACY) return fail(ISF_EINVAL, "bad phase_mode %d", phase_mode);
+    if (n_spawn < 0) return fail(ISF_EINVAL, "n_spawn must be >= 0");
+    JetParams j = jet_from(cen, len, rad);
+    j.kx = kvec[0];
+    j.ky = kvec[1];
+    j.kz = kvec[2];
+    j.phase_mode = phase_mode;
+    h->jet = j;
+    h->omega = omega;
+    h->n_spawn = n_spawn;
+    h->seed = seed;
+    h->jet_set = true;
+    return ISF_OK;
+}
+
+int isf_fill_nozzle(isf_t h, float* parts, long long nslots, long long first, long long n, unsigned long long seed) {
+    CHECK_H(h);
+    CHECK_PTR(parts, "parts");
+    if (!h->jet_set) return fail(ISF_EINVAL, "isf_jet_configure must be called first");
+    if (nslots <= 0 || first < 0 || n < 0) return fail(ISF_EINVAL, "bad particle range");
+    LAUNCH_CHECK(launch_fill_cylinder(parts, nslots, first % nslots, n, h->jet, seed, h->stream), "fill_nozzle");
+    return ISF_OK;
+}
+
+int isf_constrain(isf_t h, float* psi, int iterations) {
+    CHECK_H(h);
+    CHECK_PTR(psi, "psi");
+    if (!h->jet_set) return fail(ISF_EINVAL, "isf_jet_configure must be called first");
+    JetParams j = h->jet;
+    j.omgterm = 0.0f;  // test/runtests.jl:28
+    for (int it = 0; it < iterations; it++) {
+        LAUNCH_CHECK(launch_restrict(h->g, reinterpret_cast<float4*>(psi), nullptr, j, h->stream), "restrict_velocity");
+        int r = project(h, psi, 1, nullptr, nullptr);
+        if (r) return r;
+    }
+    return ISF_OK;
+}
+
+// Stage names of one step, in launch order (isf_step_profile reports one time each).
+static const char* kStageNames[] = {
+    "flow_x_fwd", "flow_plane", "flow_x_inv_normalize",                          // :48-49
+    "p1_div", "p1_r2c_x", "p1_plane", "p1_c2r_x_gauge_restrict",                 // :50, :57
+    "spawn",                                                                     // :52-54
+    "p2_div", "p2_r2c_x", "p2_plane", "p2_c2r_x_gauge",                          // :58
+    "velocity_sharp", "advect"};                                                 // :59-63
+static constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
+
+// One simloop body (test/runtests.jl:45-64).  If ev is non-null (kNumStages + 1 events),
+// ev[0] is recorded on the handle's stream before the first stage and ev[k + 1] after
+// stage k.
+static int step_impl(isf_t h, float* psi, float* parts, long long nslots, int iter, hipEvent_t* ev) {
+    const float t = (float)iter * h->dt;  // test/runtests.jl:46
+    JetParams j = h->jet;
+    j.omgterm = h->omega * t;             // test/runtests.jl:57
+    float4* p = reinterpret_cast<float4*>(psi);
+    hipStream_t s = h->stream;
+    int k = 0;
+    auto mark = [&]() -> int {
+        if (ev) {
+            hipError_t e = hipEventRecord(ev[++k], s);
+            if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+        }
+        return ISF_OK;
+    };
+#define STAGE(expr, what)              \
+    do {                               \
+        LAUNCH_CHECK(expr, what);      \
+        int m_ = mark();               \
+        if (m_) return m_;             \
+    } while (0)
+    if (ev) HIP_CHECK(hipEventRecord(ev[0], s), "hipEventRecord");
+    // :48-49 schroedinger_flow! + normalize
+    STAGE(launch_flow_x_fwd(h->g, p, h->S, h->tb, s), "flow x forward");
+    STAGE(launch_flow_plane(h->g, h->S, h->tb, s), "flow plane");
+    STAGE(launch_flow_x_inv(h->g, h->S, p, true, h->tb, s), "flow x inverse + normalize");
+    // :50 pressure_project! with the restrict of :57 fused into its gauge pass
+    const double sc = poisson_scale(h);
+    STAGE(launch_div_psi(h->g, p, f_of(h), s), "div(one_form(psi))");
+    STAGE(launch_r2c_x(h->g, f_of(h), P_of(h), h->tb, s), "poisson x forward");
+    STAGE(launch_poisson_plane(h->g, P_of(h), sc, h->tb, s), "poisson plane");
+    STAGE(launch_c2r_x(h->g, P_of(h), 2, nullptr, p, &j, nullptr, h->tb, s), "poisson x inverse + gauge + restrict");
+    // :52-54 add_particles! into the ring buffer
+    long long first0 = 0, nsp = 0;
+    if (nslots > 0 && h->n_spawn > 0) {
+        const long long start1 = ((long long)(iter - 1) * h->n_spawn + 1) % nslots;
+        first0 = start1 == 0 ? nslots - 1 : start1 - 1;
+        nsp = h->n_spawn;
+    }
+    STAGE(launch_spawn(parts, nslots, first0, nsp, h->jet, h->seed, (unsigned)iter, s), "add_particles");
+    // :58 pressure_project! (FP64 norm partials accumulated in its gauge pass)
+    STAGE(launch_div_psi(h->g, p, f_of(h), s), "div(one_form(psi))");
+    STAGE(launch_r2c_x(h->g, f_of(h), P_of(h), h->tb, s), "poisson x forward");
+    STAGE(launch_poisson_plane(h->g, P_of(h), sc, h->tb, s), "poisson plane");
+    STAGE(launch_c2r_x(h->g, P_of(h), 1, nullptr, p, nullptr, h->norm_parts, h->tb, s), "poisson x inverse + gauge");
+    // :59-62 velocity_one_form!(isf, psi, hbar) and the in-place StaggeredSharp
+    STAGE(launch_velocity(h->g, p, h->hbar, 1, h->vel, s), "velocity_one_form");
+    // :63 staggered_advect!
+    STAGE(launch_advect(h->g, parts, nslots, h->vel, h->dt, s), "staggered_advect");
+#undef STAGE
+    return ISF_OK;
+}
+
+static int step_args_ok(isf_t h, float* psi, float* parts, long long nslots, int iter) {
+    CHECK_H(h);
+    CHECK_PTR(psi, "psi");
+    if (!h->jet_set) return fail(ISF_EINVAL, "isf_jet_configure must be called first");
+    if (iter < 1) return fail(ISF_EINVAL, "iter must be >= 1");
+    if (nslots < 0 || (nslots > 0 && !parts)) return fail(ISF_EINVAL, "bad particle buffer");
+    return ISF_OK;
+}
+
+int isf_step(isf_t h, float* psi, float* parts, long long nslots, int iter) {
+    int r = step_args_ok(h, psi, parts, nslots, iter);
+    if (r) return r;
+    return step_impl(h, psi, parts, nslots, iter, nullptr);
+}
+
+int isf_step_stage_count(void) { return kNumStages; }
+const char* isf_step_stage_name(int i) { return (i >= 0 && i < kNumStages) ? kStageNames[i] : ""; }
+
+int isf_step_profile(isf_t h, float* psi, float* parts, long long nslots, int iter, float* stage_ms) {
+    int r = step_args_ok(h, psi, parts, nslots, iter);
+    if (r) return r;
+    CHECK_PTR(stage_ms, "stage_ms");
+    hipEvent_t ev[kNumStages + 1] = {};
+    for (int i = 0; i <= kNumStages; i++) {
+        hipError_t e = hipEventCreate(&ev[i]);
+        if (e != hipSuccess) {
+            for (int k = 0; k < i; k++) (void)hipEventDestroy(ev[k]);
+            return hip_fail(e, "hipEventCreate");
+        }
+    }
+    r = step_impl(h, psi, parts, nslots, iter, ev);
+    if (!r) {
+        hipError_t e = hipEventSynchronize(ev[kNumStages]);
+        if (e != hipSuccess) r = hip_fail(e, "hipEventSynchronize");
+        for (int i = 0; !r && i < kNumStages; i++) {
+            e = hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]);
+            if (e != hipSuccess) r = hip_fail(e, "hipEventElapsedTime");
+        }
+    }
+    for (int i = 0; i <= kNumStages; i++) (void)hipEventDestroy(ev[i]);
+    return r;
+}
+
+int isf_norm2(isf_t h, const float* psi, double* out) {
+    CHECK_H(h);
+    CHECK_PTR(psi, "psi");
+    CHECK_PTR(out, "out");
+    size_t np = 0;
+    LAUNCH_CHECK(launch_norm2(h->g, reinterpret_cast<const float4*>(psi), h->norm_parts, &np, h->stream), "norm2");
+    LAUNCH_CHECK(launch_reduce_sum(h->norm_parts, np, h->norm_out, h->stream), "norm2 reduce");
+    HIP_CHECK(hipMemcpyAsync(out, h->norm_out, sizeof(double), hipMemcpyDeviceToHost, h->stream), "norm2 copy");
+    HIP_CHECK(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+    return ISF_OK;
+}
+
+int isf_step_norm2(isf_t h, double* out) {
+    CHECK_H(h);
+    CHECK_PTR(out, "out");
+    LAUNCH_CHECK(launch_reduce_sum(h->norm_parts, c2r_x_blocks(h->g), h->norm_out, h->stream), "norm2 reduce");
+    HIP_CHECK(hipMemcpyAsync(out, h->norm_out, sizeof(double), hipMemcpyDeviceToHost, h->stream), "norm2 copy");
+    HIP_CHECK(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+    return ISF_OK;
+}
+
+int isf_timer_start(isf_t h) {
+    CHECK_H(h);
+    HIP_CHECK(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
+    return ISF_OK;
+}
+int isf_timer_stop(isf_t h, float* ms) {
+    CHECK_H(h);
+    CHECK_PTR(ms, "ms");
+    HIP_CHECK(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
+    HIP_CHECK(hipEventSynchronize(h->ev1), "hipEventSynchronize");
+    HIP_CHECK(hipEventElapsedTime(ms, h->ev0, h->ev1), "hipEventElapsedTime");
+    return ISF_OK;
+}
+
+}  // extern "C"

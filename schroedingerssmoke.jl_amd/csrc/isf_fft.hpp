@@ -1,0 +1,200 @@
+// isf_fft.hpp -- device building blocks of the power-of-two complex FFT used by the ISF
+// kernels (gfx950, 64-wide wavefronts).
+//
+// A line of N complex values lives in LDS; every thread of the line holds E = N/T
+// elements in registers and performs radix-R butterflies (R = min(E, remaining length))
+// of the Stockham autosort formulation (Govindaraju et al., "High performance discrete
+// Fourier transforms on graphics processors", SC'08): pass p reads element
+// j + r*N/R, multiplies by exp(-2 pi i r (j mod Ns) / (Ns R)), runs an R-point DFT and
+// writes element (j/Ns)*Ns*R + (j mod Ns) + r*Ns.  No bit reversal is needed.
+//
+// Twiddles come from a table of N-th roots of unity, tw_N[m] = exp(-2 pi i m / N),
+// computed in double on the host and rounded once; the table for length N starts at
+// element N of the combined table (lengths 2..512 fit in 1024 entries).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace isf {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// a * (DIR * i): DIR = -1 is the forward transform's -i, DIR = +1 the inverse's +i.
+template <int DIR>
+__device__ __forceinline__ float2 mul_i(float2 a) {
+    return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+// forward multiplies by w = exp(-i theta), inverse by conj(w)
+template <int DIR>
+__device__ __forceinline__ float2 twmul(float2 a, float2 w) {
+    return DIR < 0 ? make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x)
+                   : make_float2(a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y);
+}
+// multiply by the constant (c, DIR*s)
+template <int DIR>
+__device__ __forceinline__ float2 rot(float2 a, float c, float s) {
+    const float sd = DIR < 0 ? -s : s;
+    return make_float2(a.x * c - a.y * sd, a.x * sd + a.y * c);
+}
+
+// ------------------------------------------------------------------ R-point DFTs
+// X_k = sum_n x_n exp(DIR * 2 pi i n k / R), computed in place on v[0..R-1].
+template <int R, int DIR> struct Dft;
+
+template <int DIR> struct Dft<1, DIR> {
+    __device__ __forceinline__ static void run(float2*) {}
+};
+
+template <int DIR> struct Dft<2, DIR> {
+    __device__ __forceinline__ static void run(float2* v) {
+        float2 a = v[0];
+        v[0] = cadd(a, v[1]);
+        v[1] = csub(a, v[1]);
+    }
+};
+
+template <int DIR> struct Dft<4, DIR> {
+    __device__ __forceinline__ static void run(float2* v) {
+        float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        float2 a2 = cadd(v[1], v[3]), a3 = mul_i<DIR>(csub(v[1], v[3]));
+        v[0] = cadd(a0, a2);
+        v[2] = csub(a0, a2);
+        v[1] = cadd(a1, a3);
+        v[3] = csub(a1, a3);
+    }
+};
+
+template <int DIR> struct Dft<8, DIR> {
+    __device__ __forceinline__ static void run(float2* v) {
+        constexpr float C = 0.70710678118654752440f;
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, DIR>::run(e);
+        Dft<4, DIR>::run(o);
+        o[1] = rot<DIR>(o[1], C, C);
+        o[2] = mul_i<DIR>(o[2]);
+        o[3] = rot<DIR>(o[3], -C, C);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = cadd(e[k], o[k]);
+            v[k + 4] = csub(e[k], o[k]);
+        }
+    }
+};
+
+template <int DIR> struct Dft<16, DIR> {
+    __device__ __forceinline__ static void run(float2* v) {
+        constexpr float C1 = 0.92387953251128675613f;  // cos(pi/8)
+        constexpr float S1 = 0.38268343236508977173f;  // sin(pi/8)
+        constexpr float C = 0.70710678118654752440f;
+        float2 e[8], o[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            e[k] = v[2 * k];
+            o[k] = v[2 * k + 1];
+        }
+        Dft<8, DIR>::run(e);
+        Dft<8, DIR>::run(o);
+        o[1] = rot<DIR>(o[1], C1, S1);
+        o[2] = rot<DIR>(o[2], C, C);
+        o[3] = rot<DIR>(o[3], S1, C1);
+        o[4] = mul_i<DIR>(o[4]);
+        o[5] = rot<DIR>(o[5], -S1, C1);
+        o[6] = rot<DIR>(o[6], -C, C);
+        o[7] = rot<DIR>(o[7], -C1, S1);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            v[k] = cadd(e[k], o[k]);
+            v[k + 8] = csub(e[k], o[k]);
+        }
+    }
+};
+
+// ------------------------------------------------------------------ LDS line layouts
+// Contiguous line with one pad slot every 16 elements (breaks the stride-R bank
+// pattern of the Stockham stores).
+__device__ __forceinline__ int pad16(int e) { return e + (e >> 4); }
+// Padded length of a contiguous line of n elements, made odd so that lines placed back
+// to back start on different banks.
+__host__ __device__ constexpr int padded_len(int n) { return n + n / 16 + 1; }
+
+struct LayContig {
+    float2* p;
+    __device__ __forceinline__ float2 ld(int e) const { return p[pad16(e)]; }
+    __device__ __forceinline__ void st(int e, float2 v) const { p[pad16(e)] = v; }
+};
+struct LayStrided {
+    float2* p;
+    int s;
+    __device__ __forceinline__ float2 ld(int e) const { return p[e * s]; }
+    __device__ __forceinline__ void st(int e, float2 v) const { p[e * s] = v; }
+};
+
+// ------------------------------------------------------------------ Stockham passes
+// One radix-R pass over a line of N held in LDS by T = N/E threads (thread t of the
+// line).  All threads of the block call it (it contains block barriers); threads with
+// active == false only take part in the barriers.
+template <class Lay, int N, int R, int NS, int E, int DIR>
+__device__ __forceinline__ void stockham_pass(const Lay& lay, int t, bool active, const float2* __restrict__ tw) {
+    constexpr int T = N / E;
+    constexpr int NB = E / R;
+    static_assert(E <= N && NB >= 1 && R * NB == E, "bad FFT schedule");
+    float2 v[NB][R];
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int j = t + T * b;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[b][r] = lay.ld(j + r * (N / R));
+            if (NS > 1) {
+                const int kk = j % NS;
+#pragma unroll
+                for (int r = 1; r < R; r++) v[b][r] = twmul<DIR>(v[b][r], tw[r * kk * (N / (NS * R))]);
+            }
+            Dft<R, DIR>::run(v[b]);
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int j = t + T * b;
+            const int kk = j % NS;
+            const int idxd = (j / NS) * NS * R + kk;
+#pragma unroll
+            for (int r = 0; r < R; r++) lay.st(idxd + r * NS, v[b][r]);
+        }
+    }
+    __syncthreads();
+}
+
+template <class Lay, int N, int E, int NS, int DIR>
+struct Passes {
+    __device__ __forceinline__ static void run(const Lay& lay, int t, bool active, const float2* __restrict__ tw) {
+        constexpr int REM = N / NS;
+        constexpr int R = E < REM ? E : REM;
+        stockham_pass<Lay, N, R, NS, E, DIR>(lay, t, active, tw);
+        Passes<Lay, N, E, NS * R, DIR>::run(lay, t, active, tw);
+    }
+};
+template <class Lay, int N, int E, int DIR>
+struct Passes<Lay, N, E, N, DIR> {
+    __device__ __forceinline__ static void run(const Lay&, int, bool, const float2* __restrict__) {}
+};
+
+// Full length-N transform of one LDS line (DIR = -1 forward, +1 inverse, unnormalised).
+// tw_all is the combined twiddle table; the length-N table starts at tw_all + N.
+template <int N, int E, int DIR, class Lay>
+__device__ __forceinline__ void fft_line(const Lay& lay, int t, bool active, const float2* __restrict__ tw_all) {
+    Passes<Lay, N, E, 1, DIR>::run(lay, t, active, tw_all + N);
+}
+
+// Elements per thread used for a line of length N.
+template <int N> struct DefaultE { static constexpr int value = (N == 16 || N == 128 || N == 256) ? 16 : (N < 8 ? N : 8); };
+
+}  // namespace isf

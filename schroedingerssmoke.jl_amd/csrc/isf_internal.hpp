@@ -1,0 +1,74 @@
+// isf_internal.hpp -- types shared by the kernel launchers (isf_kernels.hip) and the
+// C-ABI front end (isf_capi.cpp).  Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace isf {
+
+// Grid geometry of one ISF instance (src/parallel.jl:11-13,44-46).
+struct GridInfo {
+    int nx, ny, nz;     // grid_res
+    float dx, dy, dz;   // d = physical_size ./ grid_res, rounded to Float32
+    float Lx, Ly, Lz;   // physical_size (integers in the reference, stored as float here)
+};
+
+// Jet constraint parameters (src/particles.jl:12-27, test/runtests.jl:7-12,23-25).
+struct JetParams {
+    float kx, ky, kz;   // kvec = jet_velocity / hbar
+    float omgterm;      // omega * t
+    float cx, cy, cz;   // nozzle_cen
+    float len, rad;     // nozzle_len, nozzle_rad
+    int phase_mode;     // 0: k.x - omega t ; 1: reference tuple_dot sum(k .+ x) - omega t
+};
+
+// Device-resident per-instance tables (all tiny: O(nx + ny + nz) entries).
+struct DevTables {
+    float2* tw;          // combined twiddle table, length-N roots at tw + N (N <= 512)
+    float2* mask_x;      // exp(i c kx^2) / N  (Schroedinger mask, x factor incl. 1/N)
+    float2* mask_y;      // exp(i c ky^2)
+    float2* mask_z;      // exp(i c kz^2)
+    double* s2x;         // (sin(pi k / nx) / dx)^2, FP64 Poisson eigenvalue table
+    double* s2y;
+    double* s2z;
+};
+
+// Launch-time choices of the x-direction passes.
+int x_tile_lines(int nx, int ny);
+
+// ---- transform passes (return hipError_t as int)
+int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s);
+int launch_flow_plane(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s);
+int launch_flow_x_inv(const GridInfo& g, const float2* S, float4* psi, bool normalize, const DevTables& t,
+                      hipStream_t s);
+int launch_r2c_x(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s);
+int launch_poisson_plane(const GridInfo& g, float2* P, double scale, const DevTables& t, hipStream_t s);
+// mode 0: write q (float per cell) ; mode 1: gauge psi ; mode 2: gauge psi + restrict
+int launch_c2r_x(const GridInfo& g, const float2* P, int mode, float* q, float4* psi, const JetParams* jet,
+                 double* norm_partials, const DevTables& t, hipStream_t s);
+size_t c2r_x_blocks(const GridInfo& g);
+
+// ---- stencil / pointwise
+int launch_div_psi(const GridInfo& g, const float4* psi, float* f, hipStream_t s);
+int launch_velocity(const GridInfo& g, const float4* psi, float hbar, int sharp, float* vel, hipStream_t s);
+int launch_div_vel(const GridInfo& g, const float* vel, float* f, hipStream_t s);
+int launch_normalize(const GridInfo& g, float4* psi, hipStream_t s);
+int launch_gauge(const GridInfo& g, float4* psi, const float* q, hipStream_t s);
+int launch_isjet(const GridInfo& g, const JetParams& jet, unsigned char* out, hipStream_t s);
+int launch_restrict(const GridInfo& g, float4* psi, const unsigned char* isjet, const JetParams& jet, hipStream_t s);
+int launch_norm2(const GridInfo& g, const float4* psi, double* partials, size_t* nparts, hipStream_t s);
+int launch_reduce_sum(const double* partials, size_t n, double* out, hipStream_t s);
+
+// ---- particles
+int launch_spawn(float* parts, long long nslots, long long first, long long n, const JetParams& jet,
+                 unsigned long long seed, unsigned step, hipStream_t s);
+int launch_fill_cylinder(float* parts, long long nslots, long long first, long long n, const JetParams& jet,
+                         unsigned long long seed, hipStream_t s);
+int launch_advect(const GridInfo& g, float* parts, long long nslots, const float* vel, float dt, hipStream_t s);
+
+// Supported axis lengths.
+bool x_len_supported(int n);
+bool plane_supported(int ny, int nz);
+
+}  // namespace isf

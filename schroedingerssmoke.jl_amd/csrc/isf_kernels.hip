@@ -1,0 +1,720 @@
+// isf_kernels.hip -- hand-written gfx950 kernels of the ISF step and their launchers.
+//
+// Data layouts in HBM (x fastest, the reference's column-major order):
+//   psi  : float4 per cell {re1, im1, re2, im2}   (test/runtests.jl:19)
+//   vel  : 3 floats per cell {vx, vy, vz}          (src/parallel.jl:14,66)
+//   f, q : 1 float per cell
+//   S    : Schroedinger spectrum, planar complex, plane (c, kx) = S + (c*nx + kx)*ny*nz,
+//          element (ky, kz) at kz*ny + ky
+//   P    : Poisson half spectrum, plane kx in [0, nx/2] = P + kx*ny*nz, same plane layout
+//   parts: 3 floats per slot                       (test/runtests.jl:33)
+//
+// One 3-D transform = an x pass (contiguous lines, written transposed into kx planes) and
+// a plane pass (y and z transforms of a whole kx plane held in LDS, with the spectral
+// multiplier applied between the forward and the inverse transforms).
+#include <hip/hip_runtime.h>
+
+#include "isf_fft.hpp"
+#include "isf_internal.hpp"
+#include "isf_philox.hpp"
+
+namespace isf {
+
+// ======================================================================== helpers
+
+__device__ __forceinline__ float angle_of(float re, float im) { return atan2f(im, re); }
+
+// angle(conj(a1) b1 + conj(a2) b2) for psi cells a, b (src/parallel.jl:108-111)
+__device__ __forceinline__ float edge_angle(float4 a, float4 b) {
+    const float re = (a.x * b.x + a.y * b.y) + (a.z * b.z + a.w * b.w);
+    const float im = (a.x * b.y - a.y * b.x) + (a.z * b.w - a.w * b.z);
+    return angle_of(re, im);
+}
+
+// isjet, src/particles.jl:6-10, at position (i-1)*d (src/parallel.jl:75-77).  Rounded
+// operations (no contraction) so the predicate is the one the Float32 reference evaluates.
+__device__ __forceinline__ bool is_jet_cell(int x, int y, int z, const GridInfo& g, const JetParams& j) {
+    const float px = __fmul_rn((float)x, g.dx), py = __fmul_rn((float)y, g.dy), pz = __fmul_rn((float)z, g.dz);
+    const float ax = fabsf(__fsub_rn(px, j.cx));
+    const float dy = __fsub_rn(py, j.cy), dz = __fsub_rn(pz, j.cz);
+    const float r2 = __fadd_rn(__fmul_rn(dy, dy), __fmul_rn(dz, dz));
+    return (ax <= j.len * 0.5f) & (r2 <= __fmul_rn(j.rad, j.rad));
+}
+
+// restrict_kernel inside the jet, src/particles.jl:12-20: psi_a <- |psi_a| exp(i phase)
+__device__ __forceinline__ float4 jet_phase(float4 p, int x, int y, int z, const GridInfo& g, const JetParams& j) {
+    const float px = __fmul_rn((float)x, g.dx), py = __fmul_rn((float)y, g.dy), pz = __fmul_rn((float)z, g.dz);
+    float ph;
+    if (j.phase_mode == 1)
+        ph = __fadd_rn(__fadd_rn(__fadd_rn(j.kx, px), __fadd_rn(j.ky, py)), __fadd_rn(j.kz, pz));
+    else
+        ph = __fadd_rn(__fadd_rn(__fmul_rn(j.kx, px), __fmul_rn(j.ky, py)), __fmul_rn(j.kz, pz));
+    ph = __fsub_rn(ph, j.omgterm);
+    float s, c;
+    sincosf(ph, &s, &c);
+    const float a1 = hypotf(p.x, p.y), a2 = hypotf(p.z, p.w);
+    return make_float4(a1 * c, a1 * s, a2 * c, a2 * s);
+}
+
+// block-wide sum of a double (NT a power of two); result valid in thread 0.
+// Fixed tree order, so the sum is deterministic.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double red[NT];
+    red[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+template <int NX> struct XCfg {
+    static constexpr int E = DefaultE<NX>::value;
+    static constexpr int T = NX / E;
+    static constexpr int LS = padded_len(NX);
+};
+
+// ======================================================================== Schroedinger x passes
+
+// Forward x transform of TY psi lines (fixed z), both components, written into the
+// (c, kx) planes of S with y fastest (src/parallel.jl:179-181, x part of fft!).
+template <int NX, int TY>
+__global__ void __launch_bounds__(2 * TY * XCfg<NX>::T)
+k_flow_x_fwd(const float4* __restrict__ psi, float2* __restrict__ S, int ny, int nz, const float2* __restrict__ tw) {
+    constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
+    constexpr int NT = 2 * TY * T;
+    __shared__ float2 lds[2 * TY * LS];
+    const int tid = threadIdx.x;
+    const int y0 = blockIdx.x * TY, z = blockIdx.y;
+    const float4* src = psi + ((size_t)z * ny + y0) * NX;
+    for (int i = tid; i < TY * NX; i += NT) {
+        const int l = i / NX, x = i % NX;
+        const float4 a = src[i];
+        lds[l * LS + pad16(x)] = make_float2(a.x, a.y);
+        lds[(TY + l) * LS + pad16(x)] = make_float2(a.z, a.w);
+    }
+    __syncthreads();
+    const int line = tid / T;
+    fft_line<NX, E, -1>(LayContig{lds + line * LS}, tid % T, true, tw);
+    for (int i = tid; i < 2 * NX * TY; i += NT) {
+        const int ly = i % TY, kx = (i / TY) % NX, c = i / (TY * NX);
+        S[(((size_t)c * NX + kx) * nz + z) * ny + y0 + ly] = lds[(c * TY + ly) * LS + pad16(kx)];
+    }
+}
+
+// Inverse x transform from S back to psi lines; optionally normalize_psi
+// (src/parallel.jl:162-165,184-185).  The 1/N of ifft is folded into the mask.
+template <int NX, int TY, bool NORMALIZE>
+__global__ void __launch_bounds__(2 * TY * XCfg<NX>::T)
+k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, int ny, int nz, const float2* __restrict__ tw) {
+    constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
+    constexpr int NT = 2 * TY * T;
+    __shared__ float2 lds[2 * TY * LS];
+    const int tid = threadIdx.x;
+    const int y0 = blockIdx.x * TY, z = blockIdx.y;
+    for (int i = tid; i < 2 * NX * TY; i += NT) {
+        const int ly = i % TY, kx = (i / TY) % NX, c = i / (TY * NX);
+        lds[(c * TY + ly) * LS + pad16(kx)] = S[(((size_t)c * NX + kx) * nz + z) * ny + y0 + ly];
+    }
+    __syncthreads();
+    const int line = tid / T;
+    fft_line<NX, E, 1>(LayContig{lds + line * LS}, tid % T, true, tw);
+    float4* dst = psi + ((size_t)z * ny + y0) * NX;
+    for (int i = tid; i < TY * NX; i += NT) {
+        const int l = i / NX, x = i % NX;
+        const float2 a = lds[l * LS + pad16(x)], b = lds[(TY + l) * LS + pad16(x)];
+        float4 p = make_float4(a.x, a.y, b.x, b.y);
+        if (NORMALIZE) {
+            const float nrm = hypotf(hypotf(p.x, p.y), hypotf(p.z, p.w));
+            p = make_float4(p.x / nrm, p.y / nrm, p.z / nrm, p.w / nrm);
+        }
+        dst[i] = p;
+    }
+}
+
+// ======================================================================== plane pass
+
+template <int NY, int NZ> struct PlaneCfg {
+    static constexpr int EY = DefaultE<NY>::value, EZ = DefaultE<NZ>::value;
+    static constexpr int TYT = NY / EY, TZT = NZ / EZ;          // threads per y line / z line
+    static constexpr int WANT = (NZ * TYT > NY * TZT) ? NZ * TYT : NY * TZT;
+    static constexpr int NT0 = WANT > 1024 ? 1024 : WANT;
+    static constexpr int NT = NT0 < 64 ? 64 : NT0;
+    static constexpr int LSY = padded_len(NY);
+    static constexpr int LDS_BYTES = NZ * LSY * 8;
+};
+
+enum { PLANE_FLOW = 0, PLANE_POISSON = 1 };
+
+// y and z transforms of one kx plane held in LDS, spectral multiplier, inverse z and y.
+//   PLANE_FLOW   : plane = c*nx + kx, multiplier mask_x[kx] mask_y[ky] mask_z[kz]
+//                  (src/parallel.jl:58-62,182-183; 1/N folded into mask_x)
+//   PLANE_POISSON: plane = kx in [0, nx/2], multiplier scale / (s2x+s2y+s2z), 0 at k = 0
+//                  (src/parallel.jl:53-55,63,171; scale folds -0.25 and the 2/N of the
+//                  half-length real inverse)
+template <int NY, int NZ, int MODE>
+__global__ void __launch_bounds__((PlaneCfg<NY, NZ>::NT))
+k_plane(float2* __restrict__ base, int nx, double scale, DevTables tb) {
+    using C = PlaneCfg<NY, NZ>;
+    constexpr int NT = C::NT, LSY = C::LSY;
+    __shared__ float2 lds[NZ * LSY];
+    const int tid = threadIdx.x;
+    const int plane = blockIdx.x;
+    float2* P = base + (size_t)plane * NY * NZ;
+    for (int i = tid; i < NY * NZ; i += NT) lds[(i / NY) * LSY + pad16(i % NY)] = P[i];
+    __syncthreads();
+
+    constexpr int LPY = NT / C::TYT;  // y lines per round
+    constexpr int LPZ = NT / C::TZT;  // z lines per round
+#pragma unroll 1
+    for (int l0 = 0; l0 < NZ; l0 += LPY) {
+        const int line = l0 + tid / C::TYT;
+        const bool act = line < NZ;
+        fft_line<NY, C::EY, -1>(LayContig{lds + (act ? line : 0) * LSY}, tid % C::TYT, act, tb.tw);
+    }
+#pragma unroll 1
+    for (int l0 = 0; l0 < NY; l0 += LPZ) {
+        const int y = l0 + tid % LPZ;
+        const bool act = y < NY;
+        fft_line<NZ, C::EZ, -1>(LayStrided{lds + pad16(act ? y : 0), LSY}, tid / LPZ, act, tb.tw);
+    }
+    if (MODE == PLANE_FLOW) {
+        const int kx = plane % nx;
+        const float2 wx = tb.mask_x[kx];
+        for (int i = tid; i < NY * NZ; i += NT) {
+            const int y = i % NY, z = i / NY;
+            const float2 w = cmul(cmul(wx, tb.mask_y[y]), tb.mask_z[z]);
+            float2& e = lds[z * LSY + pad16(y)];
+            e = cmul(e, w);
+        }
+    } else {
+        const int kx = plane;
+        const double sx = tb.s2x[kx];
+        for (int i = tid; i < NY * NZ; i += NT) {
+            const int y = i % NY, z = i / NY;
+            const double den = sx + tb.s2y[y] + tb.s2z[z];
+            const float m = (kx == 0 && y == 0 && z == 0) ? 0.0f : (float)(scale / den);
+            float2& e = lds[z * LSY + pad16(y)];
+            e = cscale(e, m);
+        }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int l0 = 0; l0 < NY; l0 += LPZ) {
+        const int y = l0 + tid % LPZ;
+        const bool act = y < NY;
+        fft_line<NZ, C::EZ, 1>(LayStrided{lds + pad16(act ? y : 0), LSY}, tid / LPZ, act, tb.tw);
+    }
+#pragma unroll 1
+    for (int l0 = 0; l0 < NZ; l0 += LPY) {
+        const int line = l0 + tid / C::TYT;
+        const bool act = line < NZ;
+        fft_line<NY, C::EY, 1>(LayContig{lds + (act ? line : 0) * LSY}, tid % C::TYT, act, tb.tw);
+    }
+    for (int i = tid; i < NY * NZ; i += NT) P[i] = lds[(i / NY) * LSY + pad16(i % NY)];
+}
+
+// ======================================================================== Poisson x passes
+
+// Real-to-complex x transform of TY lines of f (src/parallel.jl:169-170, x part):
+// pack z[m] = f[2m] + i f[2m+1], length-nx/2 FFT, split into the half spectrum
+// F[k] = E[k] + W^k O[k], k = 0..nx/2, written into the kx planes of P.
+template <int NX, int TY>
+__global__ void __launch_bounds__(TY * XCfg<NX / 2>::T)
+k_r2c_x(const float* __restrict__ f, float2* __restrict__ P, int ny, int nz, const float2* __restrict__ tw) {
+    constexpr int M = NX / 2;
+    constexpr int T = XCfg<M>::T, E = XCfg<M>::E, LS = XCfg<M>::LS;
+    constexpr int NT = TY * T;
+    __shared__ float2 lds[TY * LS];
+    const int tid = threadIdx.x;
+    const int y0 = blockIdx.x * TY, z = blockIdx.y;
+    const float2* src = reinterpret_cast<const float2*>(f + ((size_t)z * ny + y0) * NX);
+    for (int i = tid; i < TY * M; i += NT) lds[(i / M) * LS + pad16(i % M)] = src[i];
+    __syncthreads();
+    fft_line<M, E, -1>(LayContig{lds + (tid / T) * LS}, tid % T, true, tw);
+    const float2* twn = tw + NX;
+    for (int i = tid; i < (M + 1) * TY; i += NT) {
+        const int ly = i % TY, k = i / TY;
+        const float2 zk = lds[ly * LS + pad16(k % M)];
+        const float2 zm = cconj(lds[ly * LS + pad16((M - k) % M)]);
+        const float2 e = cscale(cadd(zk, zm), 0.5f);
+        const float2 dd = csub(zk, zm);
+        const float2 o = make_float2(0.5f * dd.y, -0.5f * dd.x);  // (zk - zm) / (2i)
+        P[((size_t)k * nz + z) * ny + y0 + ly] = cadd(e, cmul(twn[k], o));
+    }
+}
+
+enum { C2R_Q = 0, C2R_GAUGE = 1, C2R_GAUGE_RESTRICT = 2 };
+
+// Complex-to-real inverse x transform of TY lines of P (src/parallel.jl:172) and, in the
+// gauge modes, psi <- psi * exp(-i q) (src/parallel.jl:125-129,159) with the jet
+// constraint applied afterwards in C2R_GAUGE_RESTRICT (src/particles.jl:12-27).
+// Optional FP64 partial sums of |psi1|^2 + |psi2|^2 per block.
+template <int NX, int TY, int MODE>
+__global__ void __launch_bounds__(TY * XCfg<NX / 2>::T)
+k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restrict__ psi, GridInfo g, JetParams jet,
+        double* __restrict__ partials, const float2* __restrict__ tw) {
+    constexpr int M = NX / 2;
+    constexpr int T = XCfg<M>::T, E = XCfg<M>::E;
+    constexpr int LS = padded_len(M + 1);
+    constexpr int NT = TY * T;
+    __shared__ float2 lds[TY * LS];
+    const int tid = threadIdx.x;
+    const int ny = g.ny, nz = g.nz;
+    const int y0 = blockIdx.x * TY, z = blockIdx.y;
+    for (int i = tid; i < (M + 1) * TY; i += NT) {
+        const int ly = i % TY, k = i / TY;
+        lds[ly * LS + pad16(k)] = P[((size_t)k * nz + z) * ny + y0 + ly];
+    }
+    __syncthreads();
+    const float2* twn = tw + NX;
+    for (int i = tid; i < (M / 2 + 1) * TY; i += NT) {
+        const int ly = i % TY, k = i / TY;
+        float2* L = lds + ly * LS;
+        const float2 a = L[pad16(k)], b = L[pad16(M - k)];
+        // Z[k] = (A + conj B)/2 + i (A - conj B) exp(+2 pi i k/nx) / 2
+        {
+            const float2 e = cscale(cadd(a, cconj(b)), 0.5f);
+            const float2 o = cscale(cmul(csub(a, cconj(b)), cconj(twn[k])), 0.5f);
+            L[pad16(k)] = make_float2(e.x - o.y, e.y + o.x);
+        }
+        if (k > 0 && k < M / 2) {
+            const float2 e = cscale(cadd(b, cconj(a)), 0.5f);
+            const float2 o = cscale(cmul(csub(b, cconj(a)), cconj(twn[M - k])), 0.5f);
+            L[pad16(M - k)] = make_float2(e.x - o.y, e.y + o.x);
+        }
+    }
+    __syncthreads();
+    fft_line<M, E, 1>(LayContig{lds + (tid / T) * LS}, tid % T, true, tw);
+    const size_t row0 = ((size_t)z * ny + y0) * NX;
+    double acc = 0.0;
+    for (int i = tid; i < TY * NX; i += NT) {
+        const int l = i / NX, x = i % NX;
+        const float2 zz = lds[l * LS + pad16(x >> 1)];
+        const float q = (x & 1) ? zz.y : zz.x;
+        if (MODE == C2R_Q) {
+            qout[row0 + i] = q;
+        } else {
+            float4 p = psi[row0 + i];
+            float s, c;
+            sincosf(q, &s, &c);  // exp(-i q) = (c, -s)
+            p = make_float4(p.x * c + p.y * s, p.y * c - p.x * s, p.z * c + p.w * s, p.w * c - p.z * s);
+            if (MODE == C2R_GAUGE_RESTRICT && is_jet_cell(x, y0 + l, z, g, jet)) p = jet_phase(p, x, y0 + l, z, g, jet);
+            if (partials) acc += (double)p.x * p.x + (double)p.y * p.y + (double)p.z * p.z + (double)p.w * p.w;
+            psi[row0 + i] = p;
+        }
+    }
+    if (MODE != C2R_Q && partials) {
+        const double s = block_sum<NT>(acc);
+        if (tid == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// ======================================================================== stencils
+
+__device__ __forceinline__ void cell_xyz(size_t c, const GridInfo& g, int& x, int& y, int& z) {
+    x = (int)(c % g.nx);
+    const size_t r = c / g.nx;
+    y = (int)(r % g.ny);
+    z = (int)(r / g.ny);
+}
+
+// Fused velocity_one_form (hbar = 1) + div (src/parallel.jl:100-113,131-143,156-157):
+// f(c) = sum_a (v_a(c) - v_a(c - e_a)) / d_a^2.
+__global__ void __launch_bounds__(256) k_div_psi(const float4* __restrict__ psi, float* __restrict__ f, GridInfo g) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N = (size_t)g.nx * g.ny * g.nz;
+    if (c >= N) return;
+    int x, y, z;
+    cell_xyz(c, g, x, y, z);
+    const size_t sx = 1, sy = g.nx, sz = (size_t)g.nx * g.ny;
+    const size_t xp = x + 1 < g.nx ? c + sx : c - (size_t)(g.nx - 1) * sx;
+    const size_t xm = x > 0 ? c - sx : c + (size_t)(g.nx - 1) * sx;
+    const size_t yp = y + 1 < g.ny ? c + sy : c - (size_t)(g.ny - 1) * sy;
+    const size_t ym = y > 0 ? c - sy : c + (size_t)(g.ny - 1) * sy;
+    const size_t zp = z + 1 < g.nz ? c + sz : c - (size_t)(g.nz - 1) * sz;
+    const size_t zm = z > 0 ? c - sz : c + (size_t)(g.nz - 1) * sz;
+    const float4 p0 = psi[c];
+    const float vx = edge_angle(p0, psi[xp]), vxm = edge_angle(psi[xm], p0);
+    const float vy = edge_angle(p0, psi[yp]), vym = edge_angle(psi[ym], p0);
+    const float vz = edge_angle(p0, psi[zp]), vzm = edge_angle(psi[zm], p0);
+    const float ds0 = 1.0f / (g.dx * g.dx), ds1 = 1.0f / (g.dy * g.dy), ds2 = 1.0f / (g.dz * g.dz);
+    f[c] = (vx - vxm) * ds0 + (vy - vym) * ds1 + (vz - vzm) * ds2;
+}
+
+// velocity_one_form! with hbar (src/parallel.jl:115-122) and, if sharp, the driver's
+// in-place StaggeredSharp v_a *= 1/d_a (test/runtests.jl:61-62).
+__global__ void __launch_bounds__(256)
+k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, float hbar, int sharp) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N = (size_t)g.nx * g.ny * g.nz;
+    if (c >= N) return;
+    int x, y, z;
+    cell_xyz(c, g, x, y, z);
+    const size_t sy = g.nx, sz = (size_t)g.nx * g.ny;
+    const size_t xp = x + 1 < g.nx ? c + 1 : c - (size_t)(g.nx - 1);
+    const size_t yp = y + 1 < g.ny ? c + sy : c - (size_t)(g.ny - 1) * sy;
+    const size_t zp = z + 1 < g.nz ? c + sz : c - (size_t)(g.nz - 1) * sz;
+    const float4 p0 = psi[c];
+    float v0 = edge_angle(p0, psi[xp]) * hbar;
+    float v1 = edge_angle(p0, psi[yp]) * hbar;
+    float v2 = edge_angle(p0, psi[zp]) * hbar;
+    if (sharp) {
+        v0 = v0 * (1.0f / g.dx);
+        v1 = v1 * (1.0f / g.dy);
+        v2 = v2 * (1.0f / g.dz);
+    }
+    vel[3 * c + 0] = v0;
+    vel[3 * c + 1] = v1;
+    vel[3 * c + 2] = v2;
+}
+
+// div! from a velocity field (src/parallel.jl:131-152).
+__global__ void __launch_bounds__(256) k_div_vel(const float* __restrict__ vel, float* __restrict__ f, GridInfo g) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N = (size_t)g.nx * g.ny * g.nz;
+    if (c >= N) return;
+    int x, y, z;
+    cell_xyz(c, g, x, y, z);
+    const size_t sy = g.nx, sz = (size_t)g.nx * g.ny;
+    const size_t xm = x > 0 ? c - 1 : c + (size_t)(g.nx - 1);
+    const size_t ym = y > 0 ? c - sy : c + (size_t)(g.ny - 1) * sy;
+    const size_t zm = z > 0 ? c - sz : c + (size_t)(g.nz - 1) * sz;
+    const float ds0 = 1.0f / (g.dx * g.dx), ds1 = 1.0f / (g.dy * g.dy), ds2 = 1.0f / (g.dz * g.dz);
+    f[c] = (vel[3 * c] - vel[3 * xm]) * ds0 + (vel[3 * c + 1] - vel[3 * ym + 1]) * ds1 +
+           (vel[3 * c + 2] - vel[3 * zm + 2]) * ds2;
+}
+
+// ======================================================================== pointwise
+
+__global__ void __launch_bounds__(256) k_normalize(float4* __restrict__ psi, size_t N) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    float4 p = psi[c];
+    const float nrm = hypotf(hypotf(p.x, p.y), hypotf(p.z, p.w));
+    psi[c] = make_float4(p.x / nrm, p.y / nrm, p.z / nrm, p.w / nrm);
+}
+
+__global__ void __launch_bounds__(256) k_gauge(float4* __restrict__ psi, const float* __restrict__ q, size_t N) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    float4 p = psi[c];
+    float s, co;
+    sincosf(q[c], &s, &co);
+    psi[c] = make_float4(p.x * co + p.y * s, p.y * co - p.x * s, p.z * co + p.w * s, p.w * co - p.z * s);
+}
+
+__global__ void __launch_bounds__(256) k_isjet(unsigned char* __restrict__ out, GridInfo g, JetParams j) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N = (size_t)g.nx * g.ny * g.nz;
+    if (c >= N) return;
+    int x, y, z;
+    cell_xyz(c, g, x, y, z);
+    out[c] = is_jet_cell(x, y, z, g, j) ? 1 : 0;
+}
+
+// restrict_velocity! with an explicit isjet array (src/particles.jl:25-27), or the
+// analytic predicate when isjet is null.
+__global__ void __launch_bounds__(256)
+k_restrict(float4* __restrict__ psi, const unsigned char* __restrict__ isjet, GridInfo g, JetParams j) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N = (size_t)g.nx * g.ny * g.nz;
+    if (c >= N) return;
+    int x, y, z;
+    cell_xyz(c, g, x, y, z);
+    const bool in = isjet ? (isjet[c] != 0) : is_jet_cell(x, y, z, g, j);
+    if (in) psi[c] = jet_phase(psi[c], x, y, z, g, j);
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_norm2(const float4* __restrict__ psi, size_t N, double* __restrict__ partials) {
+    double acc = 0.0;
+    for (size_t c = (size_t)blockIdx.x * NT + threadIdx.x; c < N; c += (size_t)gridDim.x * NT) {
+        const float4 p = psi[c];
+        acc += (double)p.x * p.x + (double)p.y * p.y + (double)p.z * p.z + (double)p.w * p.w;
+    }
+    const double s = block_sum<NT>(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256) k_reduce_sum(const double* __restrict__ partials, size_t n, double* __restrict__ out) {
+    double acc = 0.0;
+    for (size_t i = threadIdx.x; i < n; i += 256) acc += partials[i];
+    const double s = block_sum<256>(acc);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// ======================================================================== particles
+
+// Julia mod(x, y) for floats (result has the sign of y), src/parallel.jl:220.
+__device__ __forceinline__ float jl_mod(float x, float y) {
+    float r = fmodf(x, y);
+    if (r != 0.0f && ((r < 0.0f) != (y < 0.0f))) r += y;
+    return r;
+}
+
+// staggered_velocity, src/parallel.jl:219-246.  Weights are not divided by d (:238).
+// An index that mod() rounds onto the upper face wraps to the first cell.
+__device__ __forceinline__ void stag_vel(const float* __restrict__ vel, const float p_in[3], const GridInfo& g, float u[3]) {
+    const float d[3] = {g.dx, g.dy, g.dz};
+    const float L[3] = {g.Lx, g.Ly, g.Lz};
+    const int n[3] = {g.nx, g.ny, g.nz};
+    float p[3];
+    int i[3], ip[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        p[a] = jl_mod(p_in[a], L[a]);
+        int ia = (int)floorf(p[a] / d[a]);
+        if (ia >= n[a]) ia -= n[a];
+        i[a] = ia;
+        ip[a] = ia + 1 == n[a] ? 0 : ia + 1;
+    }
+    const size_t sy = g.nx, sz = (size_t)g.nx * g.ny;
+    auto V = [&](int ix, int iy, int iz, int comp) { return vel[3 * (ix + sy * iy + sz * iz) + comp]; };
+    const float v0[3] = {V(i[0], i[1], i[2], 0), V(i[0], i[1], i[2], 1), V(i[0], i[1], i[2], 2)};
+    const float vn[3] = {V(i[0], ip[1], ip[2], 0), V(ip[0], i[1], ip[2], 1), V(ip[0], ip[1], i[2], 2)};
+    const float pp[3] = {V(i[0], ip[1], i[2], 0), V(ip[0], i[1], i[2], 1), V(ip[0], i[1], i[2], 2)};
+    const float pp2[3] = {V(i[0], i[1], ip[2], 0), V(i[0], i[1], ip[2], 1), V(i[0], ip[1], i[2], 2)};
+    float w[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) w[a] = p[a] - (float)i[a] * d[a];
+    const float w1[3] = {w[2], w[2], w[1]}, w2[3] = {w[1], w[0], w[0]};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        u[a] = (1.0f - w1[a]) * ((1.0f - w2[a]) * v0[a] + w2[a] * pp[a]) + w1[a] * ((1.0f - w2[a]) * pp2[a] + w2[a] * vn[a]);
+}
+
+// staggered_advect! (src/parallel.jl:190-216,248-261): RK4 for slots strictly inside
+// the box, the dead marker (-1,-1,-1) otherwise.
+__global__ void __launch_bounds__(256)
+k_advect(float* __restrict__ parts, long long nslots, const float* __restrict__ vel, GridInfo g, float dt) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    float p[3] = {parts[3 * s], parts[3 * s + 1], parts[3 * s + 2]};
+    const bool in = (p[0] > 0.0f) & (p[0] < g.Lx) & (p[1] > 0.0f) & (p[1] < g.Ly) & (p[2] > 0.0f) & (p[2] < g.Lz);
+    if (!in) {
+        parts[3 * s] = -1.0f;
+        parts[3 * s + 1] = -1.0f;
+        parts[3 * s + 2] = -1.0f;
+        return;
+    }
+    float k1[3], k2[3], k3[3], k4[3], t[3];
+    stag_vel(vel, p, g, k1);
+#pragma unroll
+    for (int a = 0; a < 3; a++) t[a] = p[a] + k1[a] * dt * 0.5f;
+    stag_vel(vel, t, g, k2);
+#pragma unroll
+    for (int a = 0; a < 3; a++) t[a] = p[a] + k2[a] * dt * 0.5f;
+    stag_vel(vel, t, g, k3);
+#pragma unroll
+    for (int a = 0; a < 3; a++) t[a] = p[a] + k3[a] * dt;
+    stag_vel(vel, t, g, k4);
+#pragma unroll
+    for (int a = 0; a < 3; a++) parts[3 * s + a] = p[a] + dt / 6.0f * (((k1[a] + 2.0f * k2[a]) + 2.0f * k3[a]) + k4[a]);
+}
+
+// add_particles! (src/particles.jl:34-54,75-94): slot (first + j) mod nslots gets
+// (c_x - 0.1, c_y + 0.9 rad cos(rt), c_z + 0.9 rad sin(rt)) + u*0.1 - 0.05.
+__global__ void __launch_bounds__(256)
+k_spawn(float* __restrict__ parts, long long nslots, long long first, long long n, JetParams j,
+        unsigned long long seed, unsigned step) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long s = (first + i) % nslots;
+    float u[4];
+    philox_uniform4((uint32_t)s, step, 0u, seed, u);
+    const float rt = u[0] * 2.0f * 3.14159265358979323846f;
+    float sn, cs;
+    sincosf(rt, &sn, &cs);
+    parts[3 * s + 0] = (j.cx - 0.1f) + (u[1] * 0.1f - 0.05f);
+    parts[3 * s + 1] = (j.cy + 0.9f * j.rad * cs) + (u[2] * 0.1f - 0.05f);
+    parts[3 * s + 2] = (j.cz + 0.9f * j.rad * sn) + (u[3] * 0.1f - 0.05f);
+}
+
+// Initial cloud uniform in the nozzle cylinder (no reference counterpart).
+__global__ void __launch_bounds__(256)
+k_fill_cylinder(float* __restrict__ parts, long long nslots, long long first, long long n, JetParams j,
+                unsigned long long seed) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long s = (first + i) % nslots;
+    float u[4];
+    philox_uniform4((uint32_t)s, 0xFFFFFFFFu, 1u, seed, u);
+    const float r = j.rad * sqrtf(u[1]);
+    const float th = u[2] * 2.0f * 3.14159265358979323846f;
+    float sn, cs;
+    sincosf(th, &sn, &cs);
+    parts[3 * s + 0] = j.cx + j.len * (u[0] - 0.5f);
+    parts[3 * s + 1] = j.cy + r * cs;
+    parts[3 * s + 2] = j.cz + r * sn;
+}
+
+// ======================================================================== launchers
+
+bool x_len_supported(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
+
+static bool yz_len_ok(int n) { return n == 16 || n == 32 || n == 64 || n == 128; }
+bool plane_supported(int ny, int nz) { return yz_len_ok(ny) && yz_len_ok(nz) && ny * nz <= 16384; }
+
+int x_tile_lines(int nx, int ny) {
+    const int ty = nx >= 512 ? 8 : 16;
+    return ty < ny ? ty : ny;
+}
+
+#define ISF_LAUNCH(kernel, grid, block, stream, ...)                      \
+    do {                                                                  \
+        hipLaunchKernelGGL(kernel, grid, block, 0, stream, __VA_ARGS__);  \
+        return (int)hipGetLastError();                                    \
+    } while (0)
+
+template <int NX, int TY>
+static int flow_x_fwd_t(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s) {
+    dim3 grid(g.ny / TY, g.nz);
+    ISF_LAUNCH((k_flow_x_fwd<NX, TY>), grid, dim3(2 * TY * XCfg<NX>::T), s, psi, S, g.ny, g.nz, t.tw);
+}
+template <int NX, int TY>
+static int flow_x_inv_t(const GridInfo& g, const float2* S, float4* psi, bool nrm, const DevTables& t, hipStream_t s) {
+    dim3 grid(g.ny / TY, g.nz);
+    if (nrm) ISF_LAUNCH((k_flow_x_inv<NX, TY, true>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, g.ny, g.nz, t.tw);
+    ISF_LAUNCH((k_flow_x_inv<NX, TY, false>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, g.ny, g.nz, t.tw);
+}
+template <int NX, int TY>
+static int r2c_x_t(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s) {
+    dim3 grid(g.ny / TY, g.nz);
+    ISF_LAUNCH((k_r2c_x<NX, TY>), grid, dim3(TY * XCfg<NX / 2>::T), s, f, P, g.ny, g.nz, t.tw);
+}
+template <int NX, int TY>
+static int c2r_x_t(const GridInfo& g, const float2* P, int mode, float* q, float4* psi, const JetParams* jet,
+                   double* parts, const DevTables& t, hipStream_t s) {
+    dim3 grid(g.ny / TY, g.nz);
+    dim3 block(TY * XCfg<NX / 2>::T);
+    JetParams j{};
+    if (jet) j = *jet;
+    if (mode == C2R_Q) ISF_LAUNCH((k_c2r_x<NX, TY, C2R_Q>), grid, block, s, P, q, psi, g, j, parts, t.tw);
+    if (mode == C2R_GAUGE) ISF_LAUNCH((k_c2r_x<NX, TY, C2R_GAUGE>), grid, block, s, P, q, psi, g, j, parts, t.tw);
+    ISF_LAUNCH((k_c2r_x<NX, TY, C2R_GAUGE_RESTRICT>), grid, block, s, P, q, psi, g, j, parts, t.tw);
+}
+
+// dispatch on (nx, tile) for the x passes
+#define ISF_X_DISPATCH(FN, g, ...)                                     \
+    do {                                                               \
+        const int ty = x_tile_lines(g.nx, g.ny);                       \
+        switch (g.nx) {                                                \
+            case 16:                                                   \
+                return ty == 16 ? FN<16, 16>(g, __VA_ARGS__) : FN<16, 8>(g, __VA_ARGS__);     \
+            case 32:                                                   \
+                return ty == 16 ? FN<32, 16>(g, __VA_ARGS__) : FN<32, 8>(g, __VA_ARGS__);     \
+            case 64:                                                   \
+                return ty == 16 ? FN<64, 16>(g, __VA_ARGS__) : FN<64, 8>(g, __VA_ARGS__);     \
+            case 128:                                                  \
+                return ty == 16 ? FN<128, 16>(g, __VA_ARGS__) : FN<128, 8>(g, __VA_ARGS__);   \
+            case 256:                                                  \
+                return ty == 16 ? FN<256, 16>(g, __VA_ARGS__) : FN<256, 8>(g, __VA_ARGS__);   \
+            case 512:                                                  \
+                return FN<512, 8>(g, __VA_ARGS__);                     \
+        }                                                              \
+        return (int)hipErrorInvalidValue;                              \
+    } while (0)
+
+int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s) {
+    ISF_X_DISPATCH(flow_x_fwd_t, g, psi, S, t, s);
+}
+int launch_flow_x_inv(const GridInfo& g, const float2* S, float4* psi, bool normalize, const DevTables& t,
+                      hipStream_t s) {
+    ISF_X_DISPATCH(flow_x_inv_t, g, S, psi, normalize, t, s);
+}
+int launch_r2c_x(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s) {
+    ISF_X_DISPATCH(r2c_x_t, g, f, P, t, s);
+}
+int launch_c2r_x(const GridInfo& g, const float2* P, int mode, float* q, float4* psi, const JetParams* jet,
+                 double* norm_partials, const DevTables& t, hipStream_t s) {
+    ISF_X_DISPATCH(c2r_x_t, g, P, mode, q, psi, jet, norm_partials, t, s);
+}
+size_t c2r_x_blocks(const GridInfo& g) { return (size_t)(g.ny / x_tile_lines(g.nx, g.ny)) * g.nz; }
+
+template <int NY, int NZ>
+static int plane_t(int mode, float2* base, int nplanes, int nx, double scale, const DevTables& t, hipStream_t s) {
+    using C = PlaneCfg<NY, NZ>;
+    if (mode == PLANE_FLOW) ISF_LAUNCH((k_plane<NY, NZ, PLANE_FLOW>), dim3(nplanes), dim3(C::NT), s, base, nx, scale, t);
+    ISF_LAUNCH((k_plane<NY, NZ, PLANE_POISSON>), dim3(nplanes), dim3(C::NT), s, base, nx, scale, t);
+}
+
+#define ISF_PLANE_Z(NY)                                                         \
+    switch (nz) {                                                               \
+        case 16: return plane_t<NY, 16>(mode, base, nplanes, nx, scale, t, s);  \
+        case 32: return plane_t<NY, 32>(mode, base, nplanes, nx, scale, t, s);  \
+        case 64: return plane_t<NY, 64>(mode, base, nplanes, nx, scale, t, s);  \
+        case 128: return plane_t<NY, 128>(mode, base, nplanes, nx, scale, t, s); \
+    }                                                                           \
+    return (int)hipErrorInvalidValue;
+
+static int plane_dispatch(int mode, int ny, int nz, float2* base, int nplanes, int nx, double scale,
+                          const DevTables& t, hipStream_t s) {
+    if (!plane_supported(ny, nz)) return (int)hipErrorInvalidValue;
+    switch (ny) {
+        case 16: { ISF_PLANE_Z(16) }
+        case 32: { ISF_PLANE_Z(32) }
+        case 64: { ISF_PLANE_Z(64) }
+        case 128: { ISF_PLANE_Z(128) }
+    }
+    return (int)hipErrorInvalidValue;
+}
+
+int launch_flow_plane(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s) {
+    return plane_dispatch(PLANE_FLOW, g.ny, g.nz, S, 2 * g.nx, g.nx, 0.0, t, s);
+}
+int launch_poisson_plane(const GridInfo& g, float2* P, double scale, const DevTables& t, hipStream_t s) {
+    return plane_dispatch(PLANE_POISSON, g.ny, g.nz, P, g.nx / 2 + 1, g.nx, scale, t, s);
+}
+
+static inline dim3 grid1d(size_t n, int nt) { return dim3((unsigned)((n + nt - 1) / nt)); }
+static inline size_t ncells(const GridInfo& g) { return (size_t)g.nx * g.ny * g.nz; }
+
+int launch_div_psi(const GridInfo& g, const float4* psi, float* f, hipStream_t s) {
+    ISF_LAUNCH(k_div_psi, grid1d(ncells(g), 256), dim3(256), s, psi, f, g);
+}
+int launch_velocity(const GridInfo& g, const float4* psi, float hbar, int sharp, float* vel, hipStream_t s) {
+    ISF_LAUNCH(k_velocity, grid1d(ncells(g), 256), dim3(256), s, psi, vel, g, hbar, sharp);
+}
+int launch_div_vel(const GridInfo& g, const float* vel, float* f, hipStream_t s) {
+    ISF_LAUNCH(k_div_vel, grid1d(ncells(g), 256), dim3(256), s, vel, f, g);
+}
+int launch_normalize(const GridInfo& g, float4* psi, hipStream_t s) {
+    ISF_LAUNCH(k_normalize, grid1d(ncells(g), 256), dim3(256), s, psi, ncells(g));
+}
+int launch_gauge(const GridInfo& g, float4* psi, const float* q, hipStream_t s) {
+    ISF_LAUNCH(k_gauge, grid1d(ncells(g), 256), dim3(256), s, psi, q, ncells(g));
+}
+int launch_isjet(const GridInfo& g, const JetParams& jet, unsigned char* out, hipStream_t s) {
+    ISF_LAUNCH(k_isjet, grid1d(ncells(g), 256), dim3(256), s, out, g, jet);
+}
+int launch_restrict(const GridInfo& g, float4* psi, const unsigned char* isjet, const JetParams& jet, hipStream_t s) {
+    ISF_LAUNCH(k_restrict, grid1d(ncells(g), 256), dim3(256), s, psi, isjet, g, jet);
+}
+int launch_norm2(const GridInfo& g, const float4* psi, double* partials, size_t* nparts, hipStream_t s) {
+    const size_t nb = 1024;
+    *nparts = nb;
+    ISF_LAUNCH(k_norm2<256>, dim3((unsigned)nb), dim3(256), s, psi, ncells(g), partials);
+}
+int launch_reduce_sum(const double* partials, size_t n, double* out, hipStream_t s) {
+    ISF_LAUNCH(k_reduce_sum, dim3(1), dim3(256), s, partials, n, out);
+}
+int launch_spawn(float* parts, long long nslots, long long first, long long n, const JetParams& jet,
+                 unsigned long long seed, unsigned step, hipStream_t s) {
+    if (n <= 0) return 0;
+    ISF_LAUNCH(k_spawn, grid1d((size_t)n, 256), dim3(256), s, parts, nslots, first, n, jet, seed, step);
+}
+int launch_fill_cylinder(float* parts, long long nslots, long long first, long long n, const JetParams& jet,
+                         unsigned long long seed, hipStream_t s) {
+    if (n <= 0) return 0;
+    ISF_LAUNCH(k_fill_cylinder, grid1d((size_t)n, 256), dim3(256), s, parts, nslots, first, n, jet, seed);
+}
+int launch_advect(const GridInfo& g, float* parts, long long nslots, const float* vel, float dt, hipStream_t s) {
+    if (nslots <= 0) return 0;
+    ISF_LAUNCH(k_advect, grid1d((size_t)nslots, 256), dim3(256), s, parts, nslots, vel, g, dt);
+}
+
+}  // namespace isf

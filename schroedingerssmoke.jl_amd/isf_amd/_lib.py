@@ -1,0 +1,92 @@
+"""Loader of libisf.so (the HIP/gfx950 library behind include/isf.h).
+
+There is no fallback: if the library is missing or a call fails, an exception says so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libisf.so")
+
+_lib = None
+
+c_float_p = C.POINTER(C.c_float)
+c_int3 = C.c_int * 3
+c_float3 = C.c_float * 3
+
+# (name, argtypes) of every exported entry point of include/isf.h
+SIGNATURES = {
+    "isf_last_error": ([], C.c_char_p),
+    "isf_version": ([], C.c_char_p),
+    "isf_device_count": ([C.POINTER(C.c_int)], C.c_int),
+    "isf_malloc": ([C.POINTER(C.c_void_p), C.c_size_t], C.c_int),
+    "isf_free": ([C.c_void_p], C.c_int),
+    "isf_memcpy_h2d": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
+    "isf_memcpy_d2h": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
+    "isf_memcpy_d2d": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
+    "isf_memset": ([C.c_void_p, C.c_int, C.c_size_t], C.c_int),
+    "isf_create": ([C.POINTER(C.c_void_p), c_int3, c_int3, C.c_float, C.c_float], C.c_int),
+    "isf_destroy": ([C.c_void_p], C.c_int),
+    "isf_synchronize": ([C.c_void_p], C.c_int),
+    "isf_get_grid": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_velocity_ptr": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
+    "isf_schroedinger_flow": ([C.c_void_p, C.c_void_p], C.c_int),
+    "isf_normalize": ([C.c_void_p, C.c_void_p], C.c_int),
+    "isf_velocity_one_form": ([C.c_void_p, C.c_void_p, C.c_float, C.c_int, C.c_void_p], C.c_int),
+    "isf_div": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_poisson_solve": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_gauge_transform": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_pressure_project": ([C.c_void_p, C.c_void_p], C.c_int),
+    "isf_isjet": ([C.c_void_p, c_float3, C.c_float, C.c_float, C.c_void_p], C.c_int),
+    "isf_restrict_velocity": ([C.c_void_p, C.c_void_p, C.c_void_p, c_float3, C.c_float, C.c_int], C.c_int),
+    "isf_add_particles": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong, c_float3, C.c_float,
+                           C.c_ulonglong, C.c_uint], C.c_int),
+    "isf_staggered_advect": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p], C.c_int),
+    "isf_jet_configure": ([C.c_void_p, c_float3, C.c_float, c_float3, C.c_float, C.c_float, C.c_longlong,
+                           C.c_ulonglong, C.c_int], C.c_int),
+    "isf_fill_nozzle": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong, C.c_ulonglong], C.c_int),
+    "isf_constrain": ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),
+    "isf_step": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_int], C.c_int),
+    "isf_norm2": ([C.c_void_p, C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "isf_step_norm2": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "isf_step_stage_count": ([], C.c_int),
+    "isf_step_stage_name": ([C.c_int], C.c_char_p),
+    "isf_step_profile": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.POINTER(C.c_float)], C.c_int),
+    "isf_timer_start": ([C.c_void_p], C.c_int),
+    "isf_timer_stop": ([C.c_void_p, C.POINTER(C.c_float)], C.c_int),
+}
+
+
+class ISFError(RuntimeError):
+    pass
+
+
+def build(verbose: bool = False) -> str:
+    """Compile libisf.so for gfx950 with hipcc (the package Makefile)."""
+    out = None if verbose else subprocess.DEVNULL
+    subprocess.run(["make", "-s", "-C", PKG_ROOT, "-j4"], check=True, stdout=out)
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ISFError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = load().isf_last_error().decode(errors="replace")
+        raise ISFError(f"{what}: libisf error {rc}: {msg}")

@@ -1,0 +1,218 @@
+"""Parity of the HIP path (libisf through the C ABI) with the CPU oracle.
+
+Tolerances (the bar for ComplexF32 work is 1e-4 relative L2 on psi against the
+reference CPU path):
+  * single operators on psi / q / f: rel L2 <= 2e-6 vs the FP64 oracle
+  * full steps: rel L2 of psi <= 1e-4 vs the FP64 oracle, with the FP32 oracle's own
+    error printed beside it
+  * particles: 99th percentile of |dp| <= 1e-4 over slots alive in both, alive-status
+    mismatches <= 0.1 % of slots (a particle within ~1e-6 of a cell face can sample a
+    different cell on GPU and CPU because interpolation weights are not divided by d,
+    src/parallel.jl:238)
+  * integer / byte / RNG work (isjet, spawn counters): bit-exact or float-identical.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import JetConfig, Oracle
+from _harness import GpuRun, oracle_run, particle_report, rel_l2, wrap_diff
+
+pytestmark = pytest.mark.gpu
+
+O64 = Oracle(np.float64)
+O32 = Oracle(np.float32)
+
+
+@pytest.fixture(scope="module")
+def isf_mod():
+    import isf_amd
+    return isf_amd
+
+
+def rnd(shape, seed):
+    return np.random.default_rng(seed).standard_normal(shape).astype(np.float32)
+
+
+def smooth_psi(n, L, seed=0):
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(*[np.arange(n[a]) * (L[a] / n[a]) for a in (2, 1, 0)], indexing="ij")
+    ph1 = 0.3 * np.sin(2 * np.pi * x / L[0]) + 0.2 * np.cos(2 * np.pi * (y / L[1] + z / L[2]))
+    ph2 = 0.25 * np.cos(4 * np.pi * x / L[0]) + 0.1 * np.sin(2 * np.pi * z / L[2])
+    a1 = 1.0 + 0.1 * rng.random(x.shape)
+    a2 = 0.1 + 0.05 * rng.random(x.shape)
+    return np.stack([a1 * np.cos(ph1), a1 * np.sin(ph1), a2 * np.cos(ph2), a2 * np.sin(ph2)], -1).astype(np.float32)
+
+
+SHAPES = [((16, 16, 16), (2, 2, 2)), ((64, 32, 32), (4, 2, 2)), ((128, 64, 64), (4, 2, 2)),
+          ((256, 128, 128), (8, 4, 4)), ((512, 64, 32), (8, 2, 1)), ((32, 128, 16), (2, 4, 1))]
+
+
+@pytest.mark.parametrize("n,L", SHAPES)
+def test_schroedinger_flow(isf_mod, n, L):
+    print(f"[flow] n={n}", flush=True)
+    isf = isf_mod.ISF(L, n, 0.1, 1 / 48)
+    psi = rnd((n[2], n[1], n[0], 4), 1)
+    _, mask = O64.tables(n, L, float(np.float32(0.1)), float(np.float32(1 / 48)))
+    d = isf.psi_array(psi)
+    isf.schroedinger_flow_(d)
+    got = d.to_host()
+    ref = O64.schroedinger_flow(psi.astype(np.float64), mask)
+    err = rel_l2(got, ref)
+    print(f"  rel L2 vs FP64 oracle = {err:.3e}")
+    assert err < 2e-6
+
+
+@pytest.mark.parametrize("n,L", SHAPES[:4])
+def test_poisson_solve(isf_mod, n, L):
+    print(f"[poisson] n={n}", flush=True)
+    isf = isf_mod.ISF(L, n, 0.1, 1 / 48)
+    f = rnd((n[2], n[1], n[0]), 2)
+    fac, _ = O64.tables(n, L, 0.1, 1 / 48)
+    q = isf.poisson_solve(isf_mod.DeviceArray.from_host(f)).to_host()
+    ref = O64.poisson_solve(f.astype(np.float64), fac)
+    err = rel_l2(q, ref[..., 0])
+    print(f"  rel L2 vs FP64 oracle = {err:.3e}")
+    assert err < 2e-6
+
+
+@pytest.mark.parametrize("n,L", SHAPES[:3])
+def test_pointwise_and_stencils(isf_mod, n, L):
+    print(f"[pointwise] n={n}", flush=True)
+    isf = isf_mod.ISF(L, n, 0.1, 1 / 48)
+    d = np.array(isf.d, np.float64)
+    psi = rnd((n[2], n[1], n[0], 4), 3)
+    # normalize
+    dp = isf.psi_array(psi)
+    isf.normalize_(dp)
+    assert rel_l2(dp.to_host(), O64.normalize(psi.astype(np.float64))) < 1e-6
+    # velocity one-form, hbar and sharp, angles compared modulo 2 pi
+    dp = isf.psi_array(psi)
+    for hbar, sharp in ((1.0, False), (0.1, True)):
+        v = isf.velocity_one_form_(dp, hbar=hbar, sharp=sharp).to_host()
+        ref = O64.velocity_one_form(psi.astype(np.float64), d, hbar=np.float32(hbar), sharp=sharp)
+        scale = hbar / d if sharp else np.full(3, hbar)
+        assert np.max(wrap_diff(v / scale, ref / scale)) < 2e-5
+    # div from a given velocity field
+    vel = rnd((n[2], n[1], n[0], 3), 4)
+    f = isf_mod.DeviceArray(isf.shape3, np.float32)
+    isf.div_(f, isf_mod.DeviceArray.from_host(vel))
+    assert rel_l2(f.to_host(), O64.div(vel.astype(np.float64), d)) < 1e-6
+    # gauge with a real q
+    q = rnd(isf.shape3, 5)
+    dp = isf.psi_array(psi)
+    isf.gauge_transform_(dp, isf_mod.DeviceArray.from_host(q))
+    qq = np.stack([q, np.zeros_like(q)], -1).astype(np.float64)
+    assert rel_l2(dp.to_host(), O64.gauge_transform(psi.astype(np.float64), qq)) < 1e-6
+    # pressure projection of a smooth field (no one-form wraps)
+    s = smooth_psi(n, L, 6)
+    dp = isf.psi_array(s)
+    isf.pressure_project_(dp)
+    fac, _ = O64.tables(n, L, 0.1, 1 / 48)
+    ref = O64.pressure_project(s.astype(np.float64), fac, d)
+    err = rel_l2(dp.to_host(), ref)
+    print(f"  pressure_project rel L2 = {err:.3e}")
+    assert err < 2e-6
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_isjet_and_restrict(isf_mod, mode):
+    n, L = (64, 32, 32), (4, 2, 2)
+    isf = isf_mod.ISF(L, n, 0.1, 1 / 48)
+    cen = np.float32([0.3, 0.966, 1.066])
+    jet = isf.isjet(cen, 0.5, 0.5).to_host()
+    ref = O32.isjet(n, np.array(isf.d, np.float32), cen, np.float32(0.5), np.float32(0.5))
+    assert np.array_equal(jet, ref) and jet.sum() > 0
+    psi = rnd((n[2], n[1], n[0], 4), 7)
+    kvec = np.float32([10.0, 0.5, -0.25])
+    dp = isf.psi_array(psi)
+    isf.restrict_velocity_(dp, kvec, isf_mod.DeviceArray.from_host(jet), np.float32(0.3), mode)
+    got = dp.to_host()
+    want = O64.restrict_velocity(psi.astype(np.float64), jet, np.array(isf.d, np.float64), kvec.astype(np.float64),
+                                 np.float64(np.float32(0.3)), mode)
+    m = jet.astype(bool)
+    assert np.array_equal(got[~m], psi[~m])
+    assert np.max(np.abs(got[m] - want[m])) < 2e-5
+
+
+def test_add_particles_matches_oracle(isf_mod):
+    isf = isf_mod.ISF((4, 2, 2), (64, 32, 32), 0.1, 1 / 48)
+    cen = np.float32([0.3, 0.966, 1.066])
+    parts = np.full((1000, 3), -1.0, np.float32)
+    dp = isf_mod.DeviceArray.from_host(parts)
+    isf.add_particles_(dp, 950, 100, cen, np.float32(0.5), seed=99, step=7)
+    got = dp.to_host()
+    want = O32.add_particles(parts, 950, 100, cen, np.float32(0.5), 99, 7)
+    assert np.array_equal(got == -1.0, want == -1.0)
+    assert np.max(np.abs(got - want)) < 2e-6
+
+
+@pytest.mark.parametrize("n,L", [((64, 32, 32), (4, 2, 2)), ((256, 128, 128), (8, 4, 4))])
+def test_staggered_advect(isf_mod, n, L):
+    isf = isf_mod.ISF(L, n, 0.1, 1 / 48)
+    vel = rnd((n[2], n[1], n[0], 3), 8) * 2.0
+    rng = np.random.default_rng(9)
+    P = 200_000
+    parts = (rng.random((P, 3)) * np.array(L) * 1.02 - 0.01 * np.array(L)).astype(np.float32)
+    parts[:3] = [[-1, -1, -1], [0.0, 1.0, 1.0], [1.0, 1.0, L[2]]]
+    dp = isf_mod.DeviceArray.from_host(parts)
+    isf.staggered_advect_(dp, isf_mod.DeviceArray.from_host(vel))
+    got = dp.to_host()
+    want = O32.staggered_advect(parts, vel, isf.dt, np.array(isf.d), np.array(L, np.float32))
+    rep = particle_report(got, want, L)
+    print(f"  advect {rep}")
+    assert np.all(got[:3] == -1.0)
+    assert rep["status_mismatch"] <= P // 1000
+    assert rep["p99"] < 1e-5
+
+
+CFG_C1 = JetConfig(vol_size=(4, 2, 2), dims=(64, 32, 32), n_spawn=50, n_slots=48 * 50, constraint_iters=10)
+
+
+def test_constraint_init_matches_oracle(isf_mod):
+    g = GpuRun(CFG_C1)
+    o = oracle_run(CFG_C1, np.float64)
+    err = rel_l2(g.host_psi(), o.psi)
+    print(f"  init rel L2 = {err:.3e}")
+    assert err < 1e-5
+
+
+@pytest.mark.parametrize("phase_mode", [0, 1])
+def test_full_steps_c1(isf_mod, phase_mode):
+    """The CPU-runnable jet (64x32x32, 48 steps, 50 particles per step)."""
+    cfg = JetConfig(**{**CFG_C1.__dict__, "phase_mode": phase_mode})
+    g = GpuRun(cfg)
+    o = oracle_run(cfg, np.float64)
+    o32 = oracle_run(cfg, np.float32)
+    steps = 48
+    g.step(steps)
+    o.step(steps)
+    o32.step(steps)
+    err = rel_l2(g.host_psi(), o.psi)
+    err32 = rel_l2(o32.psi, o.psi)
+    rep = particle_report(g.host_parts(), o.parts, cfg.vol_size)
+    print(f"  C1 psi rel L2 vs FP64 = {err:.3e} (FP32 oracle: {err32:.3e}); particles {rep}")
+    assert err < 1e-4
+    assert rep["alive_ref"] > 0
+    assert rep["status_mismatch"] <= max(2, cfg.n_slots // 1000)
+    assert rep["p99"] < 1e-4
+    nrm = g.isf.step_norm2()
+    assert abs(nrm - np.prod(cfg.dims)) < 1e-5 * np.prod(cfg.dims)
+
+
+def test_full_steps_c2_shape(isf_mod):
+    """At-scale grid (256x128x128, box 8x4x4) with 1M slots seeded in the nozzle; a few
+    steps against the FP64 oracle plus size-independent invariants."""
+    cfg = JetConfig(vol_size=(8, 4, 4), dims=(256, 128, 128), nozzle_cen=(0.3, 2 - 0.034, 2 + 0.066),
+                    n_spawn=1000, n_slots=1_000_000, n_init=1_000_000, constraint_iters=10)
+    g = GpuRun(cfg)
+    g.step(3)
+    o = oracle_run(cfg, np.float64)
+    o.step(3)
+    err = rel_l2(g.host_psi(), o.psi)
+    rep = particle_report(g.host_parts(), o.parts, cfg.vol_size)
+    print(f"  C2 psi rel L2 vs FP64 = {err:.3e}; particles {rep}")
+    assert err < 1e-4
+    assert rep["status_mismatch"] <= cfg.n_slots // 1000
+    assert rep["p99"] < 1e-4
+    nrm = g.isf.step_norm2()
+    assert abs(nrm - np.prod(cfg.dims)) < 1e-5 * np.prod(cfg.dims)
