@@ -45,6 +45,7 @@ const char* isf_version(void);
 int isf_device_count(int* out);
 int isf_malloc(void** ptr, size_t bytes);
 int isf_free(void* ptr);
+/* The copy/set helpers are synchronous and first wait for all work queued on the device. */
 int isf_memcpy_h2d(void* dst, const void* src, size_t bytes);  /* ArrayType(hostarray), test/runtests.jl:19 */
 int isf_memcpy_d2h(void* dst, const void* src, size_t bytes);  /* Array(devarray), examples/jet.jl:83 */
 int isf_memcpy_d2d(void* dst, const void* src, size_t bytes);

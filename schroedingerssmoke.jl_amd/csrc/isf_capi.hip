@@ -106,19 +106,25 @@ int isf_free(void* ptr) {
     if (ptr) HIP_CHECK(hipFree(ptr), "hipFree");
     return ISF_OK;
 }
+// The copy and set helpers are synchronous: they first wait for all work queued on the
+// device (including every handle's stream), so results of earlier calls are complete.
 int isf_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    HIP_CHECK(hipDeviceSynchronize(), "hipDeviceSynchronize");
     HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
     return ISF_OK;
 }
 int isf_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    HIP_CHECK(hipDeviceSynchronize(), "hipDeviceSynchronize");
     HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     return ISF_OK;
 }
 int isf_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+    HIP_CHECK(hipDeviceSynchronize(), "hipDeviceSynchronize");
     HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
     return ISF_OK;
 }
 int isf_memset(void* dst, int value, size_t bytes) {
+    HIP_CHECK(hipDeviceSynchronize(), "hipDeviceSynchronize");
     HIP_CHECK(hipMemset(dst, value, bytes), "hipMemset");
     return ISF_OK;
 }
@@ -158,6 +164,15 @@ int isf_create(isf_t* out, const int res[3], const int box[3], float hbar, float
     g.Lx = (float)box[0];
     g.Ly = (float)box[1];
     g.Lz = (float)box[2];
+    g.idx = 1.0f / g.dx;
+    g.idy = 1.0f / g.dy;
+    g.idz = 1.0f / g.dz;
+    {
+        int ex;
+        const float dd3[3] = {g.dx, g.dy, g.dz};
+        g.d_pow2 = 1;
+        for (int a = 0; a < 3; a++) g.d_pow2 &= (std::frexp(dd3[a], &ex) == 0.5f);
+    }
     h->hbar = hbar;
     h->dt = dt;
     h->N = (size_t)g.nx * g.ny * g.nz;
@@ -232,7 +247,8 @@ int isf_create(isf_t* out, const int res[3], const int box[3], float hbar, float
     if ((e = hipMemset(h->vel, 0, 12 * h->N)) != hipSuccess) return cleanup(hip_fail(e, "hipMemset"));
     if ((e = hipMemset(h->norm_parts, 0, sizeof(double) * (h->norm_nparts + 1))) != hipSuccess)
         return cleanup(hip_fail(e, "hipMemset"));
-    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
+    // a blocking stream: it also orders against the null stream used by plain hipMemcpy
+    if ((e = hipStreamCreate(&h->stream)) != hipSuccess)
         return cleanup(hip_fail(e, "hipStreamCreate"));
     if ((e = hipEventCreate(&h->ev0)) != hipSuccess) return cleanup(hip_fail(e, "hipEventCreate"));
     if ((e = hipEventCreate(&h->ev1)) != hipSuccess) return cleanup(hip_fail(e, "hipEventCreate"));

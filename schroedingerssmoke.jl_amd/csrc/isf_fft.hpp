@@ -10,7 +10,8 @@
 //
 // Twiddles come from a table of N-th roots of unity, tw_N[m] = exp(-2 pi i m / N),
 // computed in double on the host and rounded once; the table for length N starts at
-// element N of the combined table (lengths 2..512 fit in 1024 entries).
+// element N of the combined table in global memory (lengths 2..512 fit in 1024
+// entries) and each kernel copies the tables it needs into LDS first.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -140,7 +141,7 @@ struct LayStrided {
 // line).  All threads of the block call it (it contains block barriers); threads with
 // active == false only take part in the barriers.
 template <class Lay, int N, int R, int NS, int E, int DIR>
-__device__ __forceinline__ void stockham_pass(const Lay& lay, int t, bool active, const float2* __restrict__ tw) {
+__device__ __forceinline__ void stockham_pass(const Lay& lay, int t, bool active, const float2* tw) {
     constexpr int T = N / E;
     constexpr int NB = E / R;
     static_assert(E <= N && NB >= 1 && R * NB == E, "bad FFT schedule");
@@ -175,7 +176,7 @@ __device__ __forceinline__ void stockham_pass(const Lay& lay, int t, bool active
 
 template <class Lay, int N, int E, int NS, int DIR>
 struct Passes {
-    __device__ __forceinline__ static void run(const Lay& lay, int t, bool active, const float2* __restrict__ tw) {
+    __device__ __forceinline__ static void run(const Lay& lay, int t, bool active, const float2* tw) {
         constexpr int REM = N / NS;
         constexpr int R = E < REM ? E : REM;
         stockham_pass<Lay, N, R, NS, E, DIR>(lay, t, active, tw);
@@ -184,14 +185,20 @@ struct Passes {
 };
 template <class Lay, int N, int E, int DIR>
 struct Passes<Lay, N, E, N, DIR> {
-    __device__ __forceinline__ static void run(const Lay&, int, bool, const float2* __restrict__) {}
+    __device__ __forceinline__ static void run(const Lay&, int, bool, const float2*) {}
 };
 
 // Full length-N transform of one LDS line (DIR = -1 forward, +1 inverse, unnormalised).
-// tw_all is the combined twiddle table; the length-N table starts at tw_all + N.
+// tw is the length-N table exp(-2 pi i m / N), m < N (normally a copy in LDS).
 template <int N, int E, int DIR, class Lay>
-__device__ __forceinline__ void fft_line(const Lay& lay, int t, bool active, const float2* __restrict__ tw_all) {
-    Passes<Lay, N, E, 1, DIR>::run(lay, t, active, tw_all + N);
+__device__ __forceinline__ void fft_line(const Lay& lay, int t, bool active, const float2* tw) {
+    Passes<Lay, N, E, 1, DIR>::run(lay, t, active, tw);
+}
+
+// Copy the length-N twiddle table (at tw_all + N) into LDS; the caller synchronises.
+template <int N, int NT>
+__device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restrict__ tw_all) {
+    for (int i = threadIdx.x; i < N; i += NT) dst[i] = tw_all[N + i];
 }
 
 // Elements per thread used for a line of length N.

@@ -12,6 +12,8 @@ struct GridInfo {
     int nx, ny, nz;     // grid_res
     float dx, dy, dz;   // d = physical_size ./ grid_res, rounded to Float32
     float Lx, Ly, Lz;   // physical_size (integers in the reference, stored as float here)
+    float idx, idy, idz;  // 1/d, used instead of a division only when d is a power of two
+    int d_pow2;           // all three d are powers of two (then p * (1/d) == p / d exactly)
 };
 
 // Jet constraint parameters (src/particles.jl:12-27, test/runtests.jl:7-12,23-25).

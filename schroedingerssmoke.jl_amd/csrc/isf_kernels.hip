@@ -87,8 +87,10 @@ k_flow_x_fwd(const float4* __restrict__ psi, float2* __restrict__ S, int ny, int
     constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
     constexpr int NT = 2 * TY * T;
     __shared__ float2 lds[2 * TY * LS];
+    __shared__ float2 twl[NX];
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * TY, z = blockIdx.y;
+    load_twiddles<NX, NT>(twl, tw);
     const float4* src = psi + ((size_t)z * ny + y0) * NX;
     for (int i = tid; i < TY * NX; i += NT) {
         const int l = i / NX, x = i % NX;
@@ -98,7 +100,7 @@ k_flow_x_fwd(const float4* __restrict__ psi, float2* __restrict__ S, int ny, int
     }
     __syncthreads();
     const int line = tid / T;
-    fft_line<NX, E, -1>(LayContig{lds + line * LS}, tid % T, true, tw);
+    fft_line<NX, E, -1>(LayContig{lds + line * LS}, tid % T, true, twl);
     for (int i = tid; i < 2 * NX * TY; i += NT) {
         const int ly = i % TY, kx = (i / TY) % NX, c = i / (TY * NX);
         S[(((size_t)c * NX + kx) * nz + z) * ny + y0 + ly] = lds[(c * TY + ly) * LS + pad16(kx)];
@@ -113,15 +115,17 @@ k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, int ny, int
     constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
     constexpr int NT = 2 * TY * T;
     __shared__ float2 lds[2 * TY * LS];
+    __shared__ float2 twl[NX];
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * TY, z = blockIdx.y;
+    load_twiddles<NX, NT>(twl, tw);
     for (int i = tid; i < 2 * NX * TY; i += NT) {
         const int ly = i % TY, kx = (i / TY) % NX, c = i / (TY * NX);
         lds[(c * TY + ly) * LS + pad16(kx)] = S[(((size_t)c * NX + kx) * nz + z) * ny + y0 + ly];
     }
     __syncthreads();
     const int line = tid / T;
-    fft_line<NX, E, 1>(LayContig{lds + line * LS}, tid % T, true, tw);
+    fft_line<NX, E, 1>(LayContig{lds + line * LS}, tid % T, true, twl);
     float4* dst = psi + ((size_t)z * ny + y0) * NX;
     for (int i = tid; i < TY * NX; i += NT) {
         const int l = i / NX, x = i % NX;
@@ -161,9 +165,13 @@ k_plane(float2* __restrict__ base, int nx, double scale, DevTables tb) {
     using C = PlaneCfg<NY, NZ>;
     constexpr int NT = C::NT, LSY = C::LSY;
     __shared__ float2 lds[NZ * LSY];
+    __shared__ float2 twy[NY];
+    __shared__ float2 twz[NZ];
     const int tid = threadIdx.x;
     const int plane = blockIdx.x;
     float2* P = base + (size_t)plane * NY * NZ;
+    load_twiddles<NY, NT>(twy, tb.tw);
+    load_twiddles<NZ, NT>(twz, tb.tw);
     for (int i = tid; i < NY * NZ; i += NT) lds[(i / NY) * LSY + pad16(i % NY)] = P[i];
     __syncthreads();
 
@@ -173,13 +181,13 @@ k_plane(float2* __restrict__ base, int nx, double scale, DevTables tb) {
     for (int l0 = 0; l0 < NZ; l0 += LPY) {
         const int line = l0 + tid / C::TYT;
         const bool act = line < NZ;
-        fft_line<NY, C::EY, -1>(LayContig{lds + (act ? line : 0) * LSY}, tid % C::TYT, act, tb.tw);
+        fft_line<NY, C::EY, -1>(LayContig{lds + (act ? line : 0) * LSY}, tid % C::TYT, act, twy);
     }
 #pragma unroll 1
     for (int l0 = 0; l0 < NY; l0 += LPZ) {
         const int y = l0 + tid % LPZ;
         const bool act = y < NY;
-        fft_line<NZ, C::EZ, -1>(LayStrided{lds + pad16(act ? y : 0), LSY}, tid / LPZ, act, tb.tw);
+        fft_line<NZ, C::EZ, -1>(LayStrided{lds + pad16(act ? y : 0), LSY}, tid / LPZ, act, twz);
     }
     if (MODE == PLANE_FLOW) {
         const int kx = plane % nx;
@@ -206,13 +214,13 @@ k_plane(float2* __restrict__ base, int nx, double scale, DevTables tb) {
     for (int l0 = 0; l0 < NY; l0 += LPZ) {
         const int y = l0 + tid % LPZ;
         const bool act = y < NY;
-        fft_line<NZ, C::EZ, 1>(LayStrided{lds + pad16(act ? y : 0), LSY}, tid / LPZ, act, tb.tw);
+        fft_line<NZ, C::EZ, 1>(LayStrided{lds + pad16(act ? y : 0), LSY}, tid / LPZ, act, twz);
     }
 #pragma unroll 1
     for (int l0 = 0; l0 < NZ; l0 += LPY) {
         const int line = l0 + tid / C::TYT;
         const bool act = line < NZ;
-        fft_line<NY, C::EY, 1>(LayContig{lds + (act ? line : 0) * LSY}, tid % C::TYT, act, tb.tw);
+        fft_line<NY, C::EY, 1>(LayContig{lds + (act ? line : 0) * LSY}, tid % C::TYT, act, twy);
     }
     for (int i = tid; i < NY * NZ; i += NT) P[i] = lds[(i / NY) * LSY + pad16(i % NY)];
 }
@@ -229,13 +237,16 @@ k_r2c_x(const float* __restrict__ f, float2* __restrict__ P, int ny, int nz, con
     constexpr int T = XCfg<M>::T, E = XCfg<M>::E, LS = XCfg<M>::LS;
     constexpr int NT = TY * T;
     __shared__ float2 lds[TY * LS];
+    __shared__ float2 twm[M];
+    __shared__ float2 twn[NX];
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * TY, z = blockIdx.y;
+    load_twiddles<M, NT>(twm, tw);
+    load_twiddles<NX, NT>(twn, tw);
     const float2* src = reinterpret_cast<const float2*>(f + ((size_t)z * ny + y0) * NX);
     for (int i = tid; i < TY * M; i += NT) lds[(i / M) * LS + pad16(i % M)] = src[i];
     __syncthreads();
-    fft_line<M, E, -1>(LayContig{lds + (tid / T) * LS}, tid % T, true, tw);
-    const float2* twn = tw + NX;
+    fft_line<M, E, -1>(LayContig{lds + (tid / T) * LS}, tid % T, true, twm);
     for (int i = tid; i < (M + 1) * TY; i += NT) {
         const int ly = i % TY, k = i / TY;
         const float2 zk = lds[ly * LS + pad16(k % M)];
@@ -262,15 +273,18 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
     constexpr int LS = padded_len(M + 1);
     constexpr int NT = TY * T;
     __shared__ float2 lds[TY * LS];
+    __shared__ float2 twm[M];
+    __shared__ float2 twn[NX];
     const int tid = threadIdx.x;
     const int ny = g.ny, nz = g.nz;
     const int y0 = blockIdx.x * TY, z = blockIdx.y;
+    load_twiddles<M, NT>(twm, tw);
+    load_twiddles<NX, NT>(twn, tw);
     for (int i = tid; i < (M + 1) * TY; i += NT) {
         const int ly = i % TY, k = i / TY;
         lds[ly * LS + pad16(k)] = P[((size_t)k * nz + z) * ny + y0 + ly];
     }
     __syncthreads();
-    const float2* twn = tw + NX;
     for (int i = tid; i < (M / 2 + 1) * TY; i += NT) {
         const int ly = i % TY, k = i / TY;
         float2* L = lds + ly * LS;
@@ -288,7 +302,7 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
         }
     }
     __syncthreads();
-    fft_line<M, E, 1>(LayContig{lds + (tid / T) * LS}, tid % T, true, tw);
+    fft_line<M, E, 1>(LayContig{lds + (tid / T) * LS}, tid % T, true, twm);
     const size_t row0 = ((size_t)z * ny + y0) * NX;
     double acc = 0.0;
     for (int i = tid; i < TY * NX; i += NT) {
@@ -449,8 +463,14 @@ __global__ void __launch_bounds__(256) k_reduce_sum(const double* __restrict__ p
 
 // ======================================================================== particles
 
-// Julia mod(x, y) for floats (result has the sign of y), src/parallel.jl:220.
+// Julia mod(x, y) for floats (result has the sign of y), src/parallel.jl:220:
+// r = rem(x, y); r != 0 && sign(r) != sign(y) ? r + y : r.  For y > 0 and the ranges
+// a particle position takes, rem is exact and simple: x in [0, y) -> x;
+// x in (-y, 0) -> x + y; x in [y, 2y) -> x - y (exact by Sterbenz); otherwise fmodf.
 __device__ __forceinline__ float jl_mod(float x, float y) {
+    if (x >= 0.0f && x < y) return x;
+    if (x < 0.0f && x > -y) return x + y;
+    if (x >= y && x < 2.0f * y) return x - y;
     float r = fmodf(x, y);
     if (r != 0.0f && ((r < 0.0f) != (y < 0.0f))) r += y;
     return r;
@@ -460,6 +480,7 @@ __device__ __forceinline__ float jl_mod(float x, float y) {
 // An index that mod() rounds onto the upper face wraps to the first cell.
 __device__ __forceinline__ void stag_vel(const float* __restrict__ vel, const float p_in[3], const GridInfo& g, float u[3]) {
     const float d[3] = {g.dx, g.dy, g.dz};
+    const float id[3] = {g.idx, g.idy, g.idz};
     const float L[3] = {g.Lx, g.Ly, g.Lz};
     const int n[3] = {g.nx, g.ny, g.nz};
     float p[3];
@@ -467,7 +488,7 @@ __device__ __forceinline__ void stag_vel(const float* __restrict__ vel, const fl
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         p[a] = jl_mod(p_in[a], L[a]);
-        int ia = (int)floorf(p[a] / d[a]);
+        int ia = (int)floorf(g.d_pow2 ? p[a] * id[a] : p[a] / d[a]);
         if (ia >= n[a]) ia -= n[a];
         i[a] = ia;
         ip[a] = ia + 1 == n[a] ? 0 : ia + 1;
