@@ -42,6 +42,7 @@ def stage_bytes(nx, ny, nz, nslots, n_spawn):
         "flow_x_inv_normalize": 16 * N + 16 * N,
         "spawn": 12 * n_spawn,
         "velocity_sharp": 16 * N + 12 * N,
+        "particle_order": 0,
         "advect": 24 * nslots + 12 * N,
     }
     for p in ("p1", "p2"):
@@ -153,7 +154,7 @@ def main():
     dom = max(stage_ms, key=stage_ms.get)
     achieved = sb[dom] / (stage_ms[dom] * 1e-3) / 1e9
     stages = {k: {"ms": round(stage_ms[k], 5), "GB": round(sb[k] / 1e9, 4),
-                  "GB_s": round(sb[k] / (stage_ms[k] * 1e-3) / 1e9, 1)} for k in stage_ms}
+                  "GB_s": round(sb[k] / max(stage_ms[k] * 1e-3, 1e-9) / 1e9, 1)} for k in stage_ms}
     balg = b_alg_step(N, w["n_slots"])
 
     out = {

@@ -113,6 +113,10 @@ int isf_constrain(isf_t h, float* psi, int iterations);
 /* One simulation step number iter (1-based, t = iter*dt):
  * flow, normalize, project, spawn, restrict, project, one-form(hbar) * 1/d, RK4 advect. */
 int isf_step(isf_t h, float* psi, float* parts, long long nslots, int iter);
+/* isf_step advects the particle slots in a spatial order (slots grouped by cell) that
+ * it recomputes every `steps` steps (default 32; 0 = slot order).  Each slot is advanced
+ * independently, so the order never changes a result.  No reference counterpart. */
+int isf_set_sort_interval(isf_t h, int steps);
 /* FP64 sum over cells of |psi1|^2 + |psi2|^2 (no reference counterpart; health check). */
 int isf_norm2(isf_t h, const float* psi, double* out);
 /* The same sum accumulated by the last isf_step (read after isf_synchronize). */

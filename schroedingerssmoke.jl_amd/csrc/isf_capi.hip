@@ -65,6 +65,15 @@ struct isf_handle {
     float omega;
     long long n_spawn;
     unsigned long long seed;
+    // spatial processing order of particle slots for the advection
+    int* order;
+    void* order_scratch;
+    size_t order_scratch_bytes;
+    long long order_cap;
+    const float* order_parts;
+    long long order_n;
+    int order_iter;
+    int sort_interval;
 };
 
 static float* f_of(isf_t h) { return reinterpret_cast<float*>(h->S); }
@@ -176,6 +185,7 @@ int isf_create(isf_t* out, const int res[3], const int box[3], float hbar, float
     h->hbar = hbar;
     h->dt = dt;
     h->N = (size_t)g.nx * g.ny * g.nz;
+    h->sort_interval = 32;
 
     // ---- per-axis tables (host, FP64)
     const int nmax = 512;
@@ -265,6 +275,8 @@ int isf_destroy(isf_t h) {
     if (h->S) (void)hipFree(h->S);
     if (h->vel) (void)hipFree(h->vel);
     if (h->norm_parts) (void)hipFree(h->norm_parts);
+    if (h->order) (void)hipFree(h->order);
+    if (h->order_scratch) (void)hipFree(h->order_scratch);
     if (h->table_mem) (void)hipFree(h->table_mem);
     h->magic = 0;
     delete h;
@@ -430,7 +442,7 @@ int isf_staggered_advect(isf_t h, float* parts, long long nslots, const float* v
     CHECK_H(h);
     CHECK_PTR(parts, "parts");
     if (nslots < 0) return fail(ISF_EINVAL, "bad nslots");
-    LAUNCH_CHECK(launch_advect(h->g, parts, nslots, vel ? vel : h->vel, h->dt, h->stream), "staggered_advect");
+    LAUNCH_CHECK(launch_advect(h->g, parts, nslots, nullptr, vel ? vel : h->vel, h->dt, h->stream), "staggered_advect");
     return ISF_OK;
 }
 
@@ -477,13 +489,15 @@ int isf_constrain(isf_t h, float* psi, int iterations) {
     return ISF_OK;
 }
 
+static int ensure_order(isf_t h, const float* parts, long long nslots, int iter, const int** order);
+
 // Stage names of one step, in launch order (isf_step_profile reports one time each).
 static const char* kStageNames[] = {
     "flow_x_fwd", "flow_plane", "flow_x_inv_normalize",                          // :48-49
     "p1_div", "p1_r2c_x", "p1_plane", "p1_c2r_x_gauge_restrict",                 // :50, :57
     "spawn",                                                                     // :52-54
     "p2_div", "p2_r2c_x", "p2_plane", "p2_c2r_x_gauge",                          // :58
-    "velocity_sharp", "advect"};                                                 // :59-63
+    "velocity_sharp", "particle_order", "advect"};                               // :59-63
 static constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 
 // One simloop body (test/runtests.jl:45-64).  If ev is non-null (kNumStages + 1 events),
@@ -535,9 +549,53 @@ static int step_impl(isf_t h, float* psi, float* parts, long long nslots, int it
     STAGE(launch_c2r_x(h->g, P_of(h), 1, nullptr, p, nullptr, h->norm_parts, h->tb, s), "poisson x inverse + gauge");
     // :59-62 velocity_one_form!(isf, psi, hbar) and the in-place StaggeredSharp
     STAGE(launch_velocity(h->g, p, h->hbar, 1, h->vel, s), "velocity_one_form");
-    // :63 staggered_advect!
-    STAGE(launch_advect(h->g, parts, nslots, h->vel, h->dt, s), "staggered_advect");
+    // :63 staggered_advect!, slots processed in a spatial order refreshed every
+    // sort_interval steps (particles move less than a cell per step)
+    const int* order = nullptr;
+    int r = ensure_order(h, parts, nslots, iter, &order);
+    if (r) return r;
+    if ((r = mark())) return r;
+    STAGE(launch_advect(h->g, parts, nslots, order, h->vel, h->dt, s), "staggered_advect");
 #undef STAGE
+    return ISF_OK;
+}
+
+// Refresh the particle processing order when the buffer changed or it is stale.
+static int ensure_order(isf_t h, const float* parts, long long nslots, int iter, const int** order) {
+    *order = nullptr;
+    if (nslots <= 0 || h->sort_interval <= 0) return ISF_OK;
+    if (nslots > h->order_cap) {
+        if (h->order) (void)hipFree(h->order);
+        if (h->order_scratch) (void)hipFree(h->order_scratch);
+        h->order = nullptr;
+        h->order_scratch = nullptr;
+        h->order_cap = 0;
+        const size_t sb = order_scratch_bytes(h->g, nslots);
+        hipError_t e = hipMalloc(&h->order, sizeof(int) * nslots);
+        if (e == hipSuccess) e = hipMalloc(&h->order_scratch, sb);
+        if (e != hipSuccess) return fail(ISF_ENOMEM, "particle order: %s", hipGetErrorString(e));
+        h->order_scratch_bytes = sb;
+        h->order_cap = nslots;
+        h->order_parts = nullptr;
+    }
+    if (parts != h->order_parts || nslots != h->order_n || iter < h->order_iter ||
+        iter - h->order_iter >= h->sort_interval) {
+        LAUNCH_CHECK(launch_particle_order(h->g, parts, nslots, h->order, h->order_scratch, h->order_scratch_bytes,
+                                           h->stream),
+                     "particle order");
+        h->order_parts = parts;
+        h->order_n = nslots;
+        h->order_iter = iter;
+    }
+    *order = h->order;
+    return ISF_OK;
+}
+
+int isf_set_sort_interval(isf_t h, int steps) {
+    CHECK_H(h);
+    if (steps < 0) return fail(ISF_EINVAL, "sort interval must be >= 0");
+    h->sort_interval = steps;
+    h->order_parts = nullptr;
     return ISF_OK;
 }
 

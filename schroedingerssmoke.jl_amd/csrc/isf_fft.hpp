@@ -201,6 +201,51 @@ __device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restr
     for (int i = threadIdx.x; i < N; i += NT) dst[i] = tw_all[N + i];
 }
 
+// ------------------------------------------------------------------ register passes
+// The same Stockham pass split into load / compute / store, for kernels that keep a
+// thread's E elements in registers between passes.  Thread t handles butterflies
+// j = t*NB + b (b < NB, blocked so that a thread's elements are adjacent in the first
+// and last passes); butterfly b uses v[b*R .. b*R+R-1], reads elements in(t,b,r) and
+// writes out(t,b,r).  When pass B reads exactly the elements pass A wrote (same t, b, r
+// -> same index), the LDS round trip between them is skipped.
+template <int N, int R, int NS, int E>
+struct RPass {
+    static constexpr int T = N / E;
+    static constexpr int NB = E / R;
+    static_assert(E <= N && NB >= 1 && R * NB == E, "bad FFT schedule");
+    __device__ __forceinline__ static int in(int t, int b, int r) { return t * NB + b + r * (N / R); }
+    __device__ __forceinline__ static int out(int t, int b, int r) {
+        const int j = t * NB + b;
+        return (j / NS) * NS * R + (j % NS) + r * NS;
+    }
+    template <int DIR>
+    __device__ __forceinline__ static void compute(float2* v, int t, const float2* tw) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            if (NS > 1) {
+                const int kk = (t * NB + b) % NS;
+#pragma unroll
+                for (int r = 1; r < R; r++) v[b * R + r] = twmul<DIR>(v[b * R + r], tw[r * kk * (N / (NS * R))]);
+            }
+            Dft<R, DIR>::run(v + b * R);
+        }
+    }
+    template <class Lay>
+    __device__ __forceinline__ static void load(float2* v, const Lay& lay, int t) {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int b = 0; b < NB; b++) v[b * R + r] = lay.ld(in(t, b, r));
+    }
+    template <class Lay>
+    __device__ __forceinline__ static void store(const float2* v, const Lay& lay, int t) {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int b = 0; b < NB; b++) lay.st(out(t, b, r), v[b * R + r]);
+    }
+};
+
 // Elements per thread used for a line of length N.
 template <int N> struct DefaultE { static constexpr int value = (N == 16 || N == 128 || N == 256) ? 16 : (N < 8 ? N : 8); };
 

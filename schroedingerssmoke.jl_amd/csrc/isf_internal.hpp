@@ -67,7 +67,13 @@ int launch_spawn(float* parts, long long nslots, long long first, long long n, c
                  unsigned long long seed, unsigned step, hipStream_t s);
 int launch_fill_cylinder(float* parts, long long nslots, long long first, long long n, const JetParams& jet,
                          unsigned long long seed, hipStream_t s);
-int launch_advect(const GridInfo& g, float* parts, long long nslots, const float* vel, float dt, hipStream_t s);
+int launch_advect(const GridInfo& g, float* parts, long long nslots, const int* order, const float* vel, float dt,
+                  hipStream_t s);
+// Slot processing order for the advection: slots sorted by bucket of 8 x-cells (stable
+// radix sort of (bucket, slot) pairs).  order_scratch_bytes(g, n) bytes of scratch.
+size_t order_scratch_bytes(const GridInfo& g, long long nslots);
+int launch_particle_order(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
+                          size_t scratch_bytes, hipStream_t s);
 
 // Supported axis lengths.
 bool x_len_supported(int n);

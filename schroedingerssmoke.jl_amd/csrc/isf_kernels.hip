@@ -14,6 +14,9 @@
 // multiplier applied between the forward and the inverse transforms).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "isf_fft.hpp"
 #include "isf_internal.hpp"
 #include "isf_philox.hpp"
@@ -141,19 +144,36 @@ k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, int ny, int
 
 // ======================================================================== plane pass
 
+// Contiguous line in global memory.
+struct LayGlobal {
+    float2* p;
+    __device__ __forceinline__ float2 ld(int e) const { return p[e]; }
+    __device__ __forceinline__ void st(int e, float2 v) const { p[e] = v; }
+};
+
+// Every thread owns E = 16 elements of one y line during the y transforms and of one z
+// line during the z transforms, so NT = NY * NZ / 16 threads.  With 1024 threads a thread
+// has 128 VGPRs, too few for a second 16-element buffer, so the next-plane prefetch is
+// used only when NT <= 512.
 template <int NY, int NZ> struct PlaneCfg {
-    static constexpr int EY = DefaultE<NY>::value, EZ = DefaultE<NZ>::value;
-    static constexpr int TYT = NY / EY, TZT = NZ / EZ;          // threads per y line / z line
-    static constexpr int WANT = (NZ * TYT > NY * TZT) ? NZ * TYT : NY * TZT;
-    static constexpr int NT0 = WANT > 1024 ? 1024 : WANT;
-    static constexpr int NT = NT0 < 64 ? 64 : NT0;
+    static constexpr int E = 16;
+    static constexpr int TY = NY / E, TZ = NZ / E;   // threads per y line / z line
+    static constexpr int NT = NZ * TY;
+    static constexpr int RY2 = NY / 16, RZ2 = NZ / 16;  // second radix (1: single pass)
     static constexpr int LSY = padded_len(NY);
-    static constexpr int LDS_BYTES = NZ * LSY * 8;
+    static constexpr int LDS_BYTES = NZ * LSY * 8 + 16 * (NY + NZ);
+    static constexpr bool PREFETCH = NT <= 512;
+    static_assert(NY >= 16 && NZ >= 16 && NT <= 1024, "plane too large for one workgroup");
 };
 
 enum { PLANE_FLOW = 0, PLANE_POISSON = 1 };
 
-// y and z transforms of one kx plane held in LDS, spectral multiplier, inverse z and y.
+// y and z transforms of one kx plane, spectral multiplier, inverse z and y; the block
+// walks planes blockIdx.x, blockIdx.x + gridDim.x, ... and prefetches the next plane into
+// registers while it transforms the current one.  Schedules: forward [16, n/16],
+// inverse [n/16, 16]; the multiplier and the first inverse z pass act on the registers of
+// the last forward z pass (same elements), the first forward y pass reads global memory
+// and the last inverse y pass writes it.
 //   PLANE_FLOW   : plane = c*nx + kx, multiplier mask_x[kx] mask_y[ky] mask_z[kz]
 //                  (src/parallel.jl:58-62,182-183; 1/N folded into mask_x)
 //   PLANE_POISSON: plane = kx in [0, nx/2], multiplier scale / (s2x+s2y+s2z), 0 at k = 0
@@ -161,68 +181,125 @@ enum { PLANE_FLOW = 0, PLANE_POISSON = 1 };
 //                  half-length real inverse)
 template <int NY, int NZ, int MODE>
 __global__ void __launch_bounds__((PlaneCfg<NY, NZ>::NT))
-k_plane(float2* __restrict__ base, int nx, double scale, DevTables tb) {
+k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables tb) {
     using C = PlaneCfg<NY, NZ>;
-    constexpr int NT = C::NT, LSY = C::LSY;
+    constexpr int E = C::E, NT = C::NT, LSY = C::LSY;
+    constexpr int RY2 = C::RY2, RZ2 = C::RZ2;
     __shared__ float2 lds[NZ * LSY];
     __shared__ float2 twy[NY];
     __shared__ float2 twz[NZ];
+    __shared__ double mtab[NY + NZ];  // FLOW: float2 mask_y, mask_z ; POISSON: s2y, s2z
     const int tid = threadIdx.x;
-    const int plane = blockIdx.x;
-    float2* P = base + (size_t)plane * NY * NZ;
     load_twiddles<NY, NT>(twy, tb.tw);
     load_twiddles<NZ, NT>(twz, tb.tw);
-    for (int i = tid; i < NY * NZ; i += NT) lds[(i / NY) * LSY + pad16(i % NY)] = P[i];
-    __syncthreads();
+    float2* mt2 = reinterpret_cast<float2*>(mtab);
+    for (int i = tid; i < NY + NZ; i += NT) {
+        if (MODE == PLANE_FLOW) mt2[i] = i < NY ? tb.mask_y[i] : tb.mask_z[i - NY];
+        else mtab[i] = i < NY ? tb.s2y[i] : tb.s2z[i - NY];
+    }
+    const int zl = tid / C::TY, ty = tid % C::TY;  // y transforms: line z = zl
+    const int yl = tid % NY, tz = tid / NY;        // z transforms: line y = yl
+    const LayContig ly{lds + zl * LSY};
+    const LayStrided lz{lds + pad16(yl), LSY};
+    using Y1 = RPass<NY, 16, 1, E>;
+    using Z1 = RPass<NZ, 16, 1, E>;
 
-    constexpr int LPY = NT / C::TYT;  // y lines per round
-    constexpr int LPZ = NT / C::TZT;  // z lines per round
+    float2 v[E], pf[C::PREFETCH ? E : 1];
+    int plane = blockIdx.x;
+    if (C::PREFETCH && plane < nplanes) Y1::load(pf, LayGlobal{base + (size_t)plane * NY * NZ + zl * NY}, ty);
 #pragma unroll 1
-    for (int l0 = 0; l0 < NZ; l0 += LPY) {
-        const int line = l0 + tid / C::TYT;
-        const bool act = line < NZ;
-        fft_line<NY, C::EY, -1>(LayContig{lds + (act ? line : 0) * LSY}, tid % C::TYT, act, twy);
-    }
-#pragma unroll 1
-    for (int l0 = 0; l0 < NY; l0 += LPZ) {
-        const int y = l0 + tid % LPZ;
-        const bool act = y < NY;
-        fft_line<NZ, C::EZ, -1>(LayStrided{lds + pad16(act ? y : 0), LSY}, tid / LPZ, act, twz);
-    }
-    if (MODE == PLANE_FLOW) {
-        const int kx = plane % nx;
-        const float2 wx = tb.mask_x[kx];
-        for (int i = tid; i < NY * NZ; i += NT) {
-            const int y = i % NY, z = i / NY;
-            const float2 w = cmul(cmul(wx, tb.mask_y[y]), tb.mask_z[z]);
-            float2& e = lds[z * LSY + pad16(y)];
-            e = cmul(e, w);
+    for (; plane < nplanes; plane += gridDim.x) {
+        float2* P = base + (size_t)plane * NY * NZ;
+        if constexpr (C::PREFETCH) {
+#pragma unroll
+            for (int e = 0; e < E; e++) v[e] = pf[e];
+            const int next = plane + gridDim.x;
+            if (next < nplanes) Y1::load(pf, LayGlobal{base + (size_t)next * NY * NZ + zl * NY}, ty);
+        } else {
+            Y1::load(v, LayGlobal{P + zl * NY}, ty);
         }
-    } else {
-        const int kx = plane;
-        const double sx = tb.s2x[kx];
-        for (int i = tid; i < NY * NZ; i += NT) {
-            const int y = i % NY, z = i / NY;
-            const double den = sx + tb.s2y[y] + tb.s2z[z];
-            const float m = (kx == 0 && y == 0 && z == 0) ? 0.0f : (float)(scale / den);
-            float2& e = lds[z * LSY + pad16(y)];
-            e = cscale(e, m);
+
+        // ---- forward y
+        Y1::template compute<-1>(v, ty, twy);
+        __syncthreads();  // the previous plane's LDS reads are done, tables loaded
+        Y1::store(v, ly, ty);
+        __syncthreads();
+        if constexpr (NY > 16) {
+            using Y2 = RPass<NY, RY2, 16, E>;
+            Y2::load(v, ly, ty);
+            Y2::template compute<-1>(v, ty, twy);
+            __syncthreads();
+            Y2::store(v, ly, ty);
+            __syncthreads();
+        }
+        // ---- forward z, multiplier, inverse z
+        Z1::load(v, lz, tz);
+        Z1::template compute<-1>(v, tz, twz);
+        using ZL = std::conditional_t<(NZ > 16), RPass<NZ, RZ2, 16, E>, Z1>;   // last forward pass
+        using ZI = std::conditional_t<(NZ > 16), RPass<NZ, RZ2, 1, E>, Z1>;    // first inverse pass
+        if constexpr (NZ > 16) {
+            __syncthreads();
+            Z1::store(v, lz, tz);
+            __syncthreads();
+            ZL::load(v, lz, tz);
+            ZL::template compute<-1>(v, tz, twz);
+        }
+        constexpr int RL = E / ZL::NB;  // radix of the last forward z pass
+        if (MODE == PLANE_FLOW) {
+            const int kx = plane % nx;
+            const float2 wxy = cmul(tb.mask_x[kx], mt2[yl]);
+#pragma unroll
+            for (int b = 0; b < ZL::NB; b++)
+#pragma unroll
+                for (int r = 0; r < RL; r++)
+                    v[b * RL + r] = cmul(v[b * RL + r], cmul(wxy, mt2[NY + ZL::out(tz, b, r)]));
+        } else {
+            // eigenvalue sum in FP64 (the per-axis tables are FP64), rounded once to FP32;
+            // scale = -0.5/N is a power of two, so scale * rcp(den) is within ~1.5 ulp of the
+            // exact factor (v_rcp_f32 is accurate to 1 ulp)
+            const int kx = plane;
+            const double sxy = tb.s2x[kx] + mtab[yl];
+            const float sc = (float)scale;
+#pragma unroll
+            for (int b = 0; b < ZL::NB; b++)
+#pragma unroll
+                for (int r = 0; r < RL; r++) {
+                    const int kz = ZL::out(tz, b, r);
+                    const float den = (float)(sxy + mtab[NY + kz]);
+                    const float m = (kx == 0 && yl == 0 && kz == 0) ? 0.0f : sc * __builtin_amdgcn_rcpf(den);
+                    v[b * RL + r] = cscale(v[b * RL + r], m);
+                }
+        }
+        ZI::template compute<1>(v, tz, twz);
+        __syncthreads();
+        ZI::store(v, lz, tz);
+        __syncthreads();
+        if constexpr (NZ > 16) {
+            using Z2I = RPass<NZ, 16, RZ2, E>;
+            Z2I::load(v, lz, tz);
+            Z2I::template compute<1>(v, tz, twz);
+            __syncthreads();
+            Z2I::store(v, lz, tz);
+            __syncthreads();
+        }
+        // ---- inverse y; the last pass writes global memory
+        if constexpr (NY > 16) {
+            using Y1I = RPass<NY, RY2, 1, E>;
+            using Y2I = RPass<NY, 16, RY2, E>;
+            Y1I::load(v, ly, ty);
+            Y1I::template compute<1>(v, ty, twy);
+            __syncthreads();
+            Y1I::store(v, ly, ty);
+            __syncthreads();
+            Y2I::load(v, ly, ty);
+            Y2I::template compute<1>(v, ty, twy);
+            Y2I::store(v, LayGlobal{P + zl * NY}, ty);
+        } else {
+            Y1::load(v, ly, ty);
+            Y1::template compute<1>(v, ty, twy);
+            Y1::store(v, LayGlobal{P + zl * NY}, ty);
         }
     }
-    __syncthreads();
-#pragma unroll 1
-    for (int l0 = 0; l0 < NY; l0 += LPZ) {
-        const int y = l0 + tid % LPZ;
-        const bool act = y < NY;
-        fft_line<NZ, C::EZ, 1>(LayStrided{lds + pad16(act ? y : 0), LSY}, tid / LPZ, act, twz);
-    }
-#pragma unroll 1
-    for (int l0 = 0; l0 < NZ; l0 += LPY) {
-        const int line = l0 + tid / C::TYT;
-        const bool act = line < NZ;
-        fft_line<NY, C::EY, 1>(LayContig{lds + (act ? line : 0) * LSY}, tid % C::TYT, act, twy);
-    }
-    for (int i = tid; i < NY * NZ; i += NT) P[i] = lds[(i / NY) * LSY + pad16(i % NY)];
 }
 
 // ======================================================================== Poisson x passes
@@ -336,54 +413,88 @@ __device__ __forceinline__ void cell_xyz(size_t c, const GridInfo& g, int& x, in
     z = (int)(r / g.ny);
 }
 
+// ---- z-marching stencil tiles
+// A block of BX x 4 threads covers BX consecutive x and 4 consecutive y and walks TZ
+// consecutive z.  psi(c + e_z) of one z step is psi(c) of the next, psi(c + e_x) comes
+// from the neighbouring lane, so every forward-edge angle
+// v_a(c) = angle(conj psi1(c) psi1(c+e_a) + conj psi2(c) psi2(c+e_a)) (src/parallel.jl:100-113)
+// is evaluated once per cell (plus a thin halo).
+constexpr int ST_BY = 4, ST_TZ = 8;
+
+__device__ __forceinline__ float4 shfl4(float4 v, int src) {
+    return make_float4(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64), __shfl(v.w, src, 64));
+}
+
 // Fused velocity_one_form (hbar = 1) + div (src/parallel.jl:100-113,131-143,156-157):
 // f(c) = sum_a (v_a(c) - v_a(c - e_a)) / d_a^2.
-__global__ void __launch_bounds__(256) k_div_psi(const float4* __restrict__ psi, float* __restrict__ f, GridInfo g) {
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t N = (size_t)g.nx * g.ny * g.nz;
-    if (c >= N) return;
-    int x, y, z;
-    cell_xyz(c, g, x, y, z);
-    const size_t sx = 1, sy = g.nx, sz = (size_t)g.nx * g.ny;
-    const size_t xp = x + 1 < g.nx ? c + sx : c - (size_t)(g.nx - 1) * sx;
-    const size_t xm = x > 0 ? c - sx : c + (size_t)(g.nx - 1) * sx;
-    const size_t yp = y + 1 < g.ny ? c + sy : c - (size_t)(g.ny - 1) * sy;
-    const size_t ym = y > 0 ? c - sy : c + (size_t)(g.ny - 1) * sy;
-    const size_t zp = z + 1 < g.nz ? c + sz : c - (size_t)(g.nz - 1) * sz;
-    const size_t zm = z > 0 ? c - sz : c + (size_t)(g.nz - 1) * sz;
-    const float4 p0 = psi[c];
-    const float vx = edge_angle(p0, psi[xp]), vxm = edge_angle(psi[xm], p0);
-    const float vy = edge_angle(p0, psi[yp]), vym = edge_angle(psi[ym], p0);
-    const float vz = edge_angle(p0, psi[zp]), vzm = edge_angle(psi[zm], p0);
+template <int BX>
+__global__ void __launch_bounds__(BX * ST_BY) k_div_psi(const float4* __restrict__ psi, float* __restrict__ f, GridInfo g) {
+    __shared__ float vy_s[2][ST_BY][BX];
+    const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
+    const int lane = threadIdx.x % 64;
+    const int x = blockIdx.x * BX + tx, y = blockIdx.y * ST_BY + ty, z0 = blockIdx.z * ST_TZ;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const size_t sy = nx, sz = (size_t)nx * ny;
+    const int xp = x + 1 < nx ? x + 1 : 0, xm = x > 0 ? x - 1 : nx - 1;
+    const int yp = y + 1 < ny ? y + 1 : 0, ym = y > 0 ? y - 1 : ny - 1;
     const float ds0 = 1.0f / (g.dx * g.dx), ds1 = 1.0f / (g.dy * g.dy), ds2 = 1.0f / (g.dz * g.dz);
-    f[c] = (vx - vxm) * ds0 + (vy - vym) * ds1 + (vz - vzm) * ds2;
+    const int zb = z0 > 0 ? z0 - 1 : nz - 1;
+    float4 pc = psi[x + sy * y + sz * z0];
+    float vzm = edge_angle(psi[x + sy * y + sz * zb], pc);  // v_z(c - e_z) for the first z
+#pragma unroll 1
+    for (int k = 0; k < ST_TZ; k++) {
+        const int z = z0 + k;
+        const int zp = z + 1 < nz ? z + 1 : 0;
+        const size_t row = sy * y + sz * z;
+        const float4 pz = psi[x + sy * y + sz * zp];
+        const float4 py = psi[x + sy * yp + sz * z];
+        float4 px = shfl4(pc, tx + 1 < BX ? lane + 1 : lane);
+        if (tx + 1 == BX) px = psi[xp + row];
+        const float vx = edge_angle(pc, px), vy = edge_angle(pc, py), vz = edge_angle(pc, pz);
+        float vxm = __shfl(vx, tx > 0 ? lane - 1 : lane, 64);
+        if (tx == 0) vxm = edge_angle(psi[xm + row], pc);
+        vy_s[k & 1][ty][tx] = vy;
+        __syncthreads();
+        const float vym = ty > 0 ? vy_s[k & 1][ty - 1][tx] : edge_angle(psi[x + sy * ym + sz * z], pc);
+        f[x + row] = (vx - vxm) * ds0 + (vy - vym) * ds1 + (vz - vzm) * ds2;
+        vzm = vz;
+        pc = pz;
+    }
 }
 
 // velocity_one_form! with hbar (src/parallel.jl:115-122) and, if sharp, the driver's
 // in-place StaggeredSharp v_a *= 1/d_a (test/runtests.jl:61-62).
-__global__ void __launch_bounds__(256)
+template <int BX>
+__global__ void __launch_bounds__(BX * ST_BY)
 k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, float hbar, int sharp) {
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t N = (size_t)g.nx * g.ny * g.nz;
-    if (c >= N) return;
-    int x, y, z;
-    cell_xyz(c, g, x, y, z);
-    const size_t sy = g.nx, sz = (size_t)g.nx * g.ny;
-    const size_t xp = x + 1 < g.nx ? c + 1 : c - (size_t)(g.nx - 1);
-    const size_t yp = y + 1 < g.ny ? c + sy : c - (size_t)(g.ny - 1) * sy;
-    const size_t zp = z + 1 < g.nz ? c + sz : c - (size_t)(g.nz - 1) * sz;
-    const float4 p0 = psi[c];
-    float v0 = edge_angle(p0, psi[xp]) * hbar;
-    float v1 = edge_angle(p0, psi[yp]) * hbar;
-    float v2 = edge_angle(p0, psi[zp]) * hbar;
-    if (sharp) {
-        v0 = v0 * (1.0f / g.dx);
-        v1 = v1 * (1.0f / g.dy);
-        v2 = v2 * (1.0f / g.dz);
+    const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
+    const int lane = threadIdx.x % 64;
+    const int x = blockIdx.x * BX + tx, y = blockIdx.y * ST_BY + ty, z0 = blockIdx.z * ST_TZ;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const size_t sy = nx, sz = (size_t)nx * ny;
+    const int xp = x + 1 < nx ? x + 1 : 0, yp = y + 1 < ny ? y + 1 : 0;
+    const float s0 = sharp ? 1.0f / g.dx : 1.0f, s1 = sharp ? 1.0f / g.dy : 1.0f, s2 = sharp ? 1.0f / g.dz : 1.0f;
+    float4 pc = psi[x + sy * y + sz * z0];
+#pragma unroll 1
+    for (int k = 0; k < ST_TZ; k++) {
+        const int z = z0 + k;
+        const int zp = z + 1 < nz ? z + 1 : 0;
+        const size_t c = x + sy * y + sz * z;
+        const float4 pz = psi[x + sy * y + sz * zp];
+        const float4 py = psi[x + sy * yp + sz * z];
+        float4 px = shfl4(pc, tx + 1 < BX ? lane + 1 : lane);
+        if (tx + 1 == BX) px = psi[xp + sy * y + sz * z];
+        float v0 = edge_angle(pc, px) * hbar, v1 = edge_angle(pc, py) * hbar, v2 = edge_angle(pc, pz) * hbar;
+        if (sharp) {
+            v0 = v0 * s0;
+            v1 = v1 * s1;
+            v2 = v2 * s2;
+        }
+        vel[3 * c + 0] = v0;
+        vel[3 * c + 1] = v1;
+        vel[3 * c + 2] = v2;
+        pc = pz;
     }
-    vel[3 * c + 0] = v0;
-    vel[3 * c + 1] = v1;
-    vel[3 * c + 2] = v2;
 }
 
 // div! from a velocity field (src/parallel.jl:131-152).
@@ -476,6 +587,12 @@ __device__ __forceinline__ float jl_mod(float x, float y) {
     return r;
 }
 
+struct F3 {
+    float x, y, z;
+};
+// three consecutive floats (the compiler merges them into one dwordx3 load)
+__device__ __forceinline__ F3 ld3(const float* __restrict__ p) { return F3{p[0], p[1], p[2]}; }
+
 // staggered_velocity, src/parallel.jl:219-246.  Weights are not divided by d (:238).
 // An index that mod() rounds onto the upper face wraps to the first cell.
 __device__ __forceinline__ void stag_vel(const float* __restrict__ vel, const float p_in[3], const GridInfo& g, float u[3]) {
@@ -493,12 +610,18 @@ __device__ __forceinline__ void stag_vel(const float* __restrict__ vel, const fl
         i[a] = ia;
         ip[a] = ia + 1 == n[a] ? 0 : ia + 1;
     }
-    const size_t sy = g.nx, sz = (size_t)g.nx * g.ny;
-    auto V = [&](int ix, int iy, int iz, int comp) { return vel[3 * (ix + sy * iy + sz * iz) + comp]; };
-    const float v0[3] = {V(i[0], i[1], i[2], 0), V(i[0], i[1], i[2], 1), V(i[0], i[1], i[2], 2)};
-    const float vn[3] = {V(i[0], ip[1], ip[2], 0), V(ip[0], i[1], ip[2], 1), V(ip[0], ip[1], i[2], 2)};
-    const float pp[3] = {V(i[0], ip[1], i[2], 0), V(ip[0], i[1], i[2], 1), V(ip[0], i[1], i[2], 2)};
-    const float pp2[3] = {V(i[0], i[1], ip[2], 0), V(i[0], i[1], ip[2], 1), V(i[0], ip[1], i[2], 2)};
+    // the 7 cells as 4 rows: (j,k) and (j+1,k), (j,k+1) at x = i and i+1, (j+1,k+1) at i
+    const unsigned sy = (unsigned)g.nx, sz = (unsigned)g.nx * (unsigned)g.ny;
+    const unsigned rjk = sy * i[1] + sz * i[2], rjpk = sy * ip[1] + sz * i[2];
+    const unsigned rjkp = sy * i[1] + sz * ip[2], rjpkp = sy * ip[1] + sz * ip[2];
+    const F3 a0 = ld3(vel + 3u * (i[0] + rjk)), a1 = ld3(vel + 3u * (ip[0] + rjk));
+    const F3 b0 = ld3(vel + 3u * (i[0] + rjpk)), b1 = ld3(vel + 3u * (ip[0] + rjpk));
+    const F3 c0 = ld3(vel + 3u * (i[0] + rjkp)), c1 = ld3(vel + 3u * (ip[0] + rjkp));
+    const float d0x = vel[3u * (i[0] + rjpkp)];
+    const float v0[3] = {a0.x, a0.y, a0.z};   // velocity[i, j, k]
+    const float vn[3] = {d0x, c1.y, b1.z};    // (v[i,j+1,k+1].x, v[i+1,j,k+1].y, v[i+1,j+1,k].z)
+    const float pp[3] = {b0.x, a1.y, a1.z};   // (pyp.x, pxp.y, pxp.z)
+    const float pp2[3] = {c0.x, c0.y, b0.z};  // (pzp.x, pzp.y, pyp.z)
     float w[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) w[a] = p[a] - (float)i[a] * d[a];
@@ -509,12 +632,17 @@ __device__ __forceinline__ void stag_vel(const float* __restrict__ vel, const fl
 }
 
 // staggered_advect! (src/parallel.jl:190-216,248-261): RK4 for slots strictly inside
-// the box, the dead marker (-1,-1,-1) otherwise.
+// the box, the dead marker (-1,-1,-1) otherwise.  Thread k processes slot order[k]
+// (order == nullptr: slot k); every slot is independent, so the order only changes
+// which gathers share cache lines, never a result.
 __global__ void __launch_bounds__(256)
-k_advect(float* __restrict__ parts, long long nslots, const float* __restrict__ vel, GridInfo g, float dt) {
-    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    float p[3] = {parts[3 * s], parts[3 * s + 1], parts[3 * s + 2]};
+k_advect(float* __restrict__ parts, long long nslots, const int* __restrict__ order, const float* __restrict__ vel,
+         GridInfo g, float dt) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nslots) return;
+    const long long s = order ? (long long)order[k] : k;
+    const F3 p3 = ld3(parts + 3 * s);
+    float p[3] = {p3.x, p3.y, p3.z};
     const bool in = (p[0] > 0.0f) & (p[0] < g.Lx) & (p[1] > 0.0f) & (p[1] < g.Ly) & (p[2] > 0.0f) & (p[2] < g.Lz);
     if (!in) {
         parts[3 * s] = -1.0f;
@@ -656,11 +784,27 @@ int launch_c2r_x(const GridInfo& g, const float2* P, int mode, float* q, float4*
 }
 size_t c2r_x_blocks(const GridInfo& g) { return (size_t)(g.ny / x_tile_lines(g.nx, g.ny)) * g.nz; }
 
+static int cu_count() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cached[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
 template <int NY, int NZ>
 static int plane_t(int mode, float2* base, int nplanes, int nx, double scale, const DevTables& t, hipStream_t s) {
     using C = PlaneCfg<NY, NZ>;
-    if (mode == PLANE_FLOW) ISF_LAUNCH((k_plane<NY, NZ, PLANE_FLOW>), dim3(nplanes), dim3(C::NT), s, base, nx, scale, t);
-    ISF_LAUNCH((k_plane<NY, NZ, PLANE_POISSON>), dim3(nplanes), dim3(C::NT), s, base, nx, scale, t);
+    const int by_lds = (160 * 1024) / C::LDS_BYTES, by_threads = 2048 / C::NT;
+    const int per_cu = std::max(1, std::min(by_lds, by_threads));
+    const int blocks = std::min(nplanes, cu_count() * per_cu);
+    if (mode == PLANE_FLOW)
+        ISF_LAUNCH((k_plane<NY, NZ, PLANE_FLOW>), dim3(blocks), dim3(C::NT), s, base, nplanes, nx, scale, t);
+    ISF_LAUNCH((k_plane<NY, NZ, PLANE_POISSON>), dim3(blocks), dim3(C::NT), s, base, nplanes, nx, scale, t);
 }
 
 #define ISF_PLANE_Z(NY)                                                         \
@@ -694,11 +838,17 @@ int launch_poisson_plane(const GridInfo& g, float2* P, double scale, const DevTa
 static inline dim3 grid1d(size_t n, int nt) { return dim3((unsigned)((n + nt - 1) / nt)); }
 static inline size_t ncells(const GridInfo& g) { return (size_t)g.nx * g.ny * g.nz; }
 
+static dim3 stencil_grid(const GridInfo& g, int bx) { return dim3(g.nx / bx, g.ny / ST_BY, g.nz / ST_TZ); }
+
 int launch_div_psi(const GridInfo& g, const float4* psi, float* f, hipStream_t s) {
-    ISF_LAUNCH(k_div_psi, grid1d(ncells(g), 256), dim3(256), s, psi, f, g);
+    if (g.nx >= 64) ISF_LAUNCH(k_div_psi<64>, stencil_grid(g, 64), dim3(64 * ST_BY), s, psi, f, g);
+    if (g.nx == 32) ISF_LAUNCH(k_div_psi<32>, stencil_grid(g, 32), dim3(32 * ST_BY), s, psi, f, g);
+    ISF_LAUNCH(k_div_psi<16>, stencil_grid(g, 16), dim3(16 * ST_BY), s, psi, f, g);
 }
 int launch_velocity(const GridInfo& g, const float4* psi, float hbar, int sharp, float* vel, hipStream_t s) {
-    ISF_LAUNCH(k_velocity, grid1d(ncells(g), 256), dim3(256), s, psi, vel, g, hbar, sharp);
+    if (g.nx >= 64) ISF_LAUNCH(k_velocity<64>, stencil_grid(g, 64), dim3(64 * ST_BY), s, psi, vel, g, hbar, sharp);
+    if (g.nx == 32) ISF_LAUNCH(k_velocity<32>, stencil_grid(g, 32), dim3(32 * ST_BY), s, psi, vel, g, hbar, sharp);
+    ISF_LAUNCH(k_velocity<16>, stencil_grid(g, 16), dim3(16 * ST_BY), s, psi, vel, g, hbar, sharp);
 }
 int launch_div_vel(const GridInfo& g, const float* vel, float* f, hipStream_t s) {
     ISF_LAUNCH(k_div_vel, grid1d(ncells(g), 256), dim3(256), s, vel, f, g);
@@ -733,9 +883,10 @@ int launch_fill_cylinder(float* parts, long long nslots, long long first, long l
     if (n <= 0) return 0;
     ISF_LAUNCH(k_fill_cylinder, grid1d((size_t)n, 256), dim3(256), s, parts, nslots, first, n, jet, seed);
 }
-int launch_advect(const GridInfo& g, float* parts, long long nslots, const float* vel, float dt, hipStream_t s) {
+int launch_advect(const GridInfo& g, float* parts, long long nslots, const int* order, const float* vel, float dt,
+                  hipStream_t s) {
     if (nslots <= 0) return 0;
-    ISF_LAUNCH(k_advect, grid1d((size_t)nslots, 256), dim3(256), s, parts, nslots, vel, g, dt);
+    ISF_LAUNCH(k_advect, grid1d((size_t)nslots, 256), dim3(256), s, parts, nslots, order, vel, g, dt);
 }
 
 }  // namespace isf

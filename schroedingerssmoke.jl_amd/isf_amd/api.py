@@ -217,6 +217,9 @@ class ISF:
               "isf_step_profile")
         return {self.lib.isf_step_stage_name(i).decode(): float(ms[i]) for i in range(n)}
 
+    def set_sort_interval(self, steps):
+        check(self.lib.isf_set_sort_interval(self.h, int(steps)), "isf_set_sort_interval")
+
     def norm2(self, psi):
         out = C.c_double()
         check(self.lib.isf_norm2(self.h, _ptr(psi), C.byref(out)), "isf_norm2")
