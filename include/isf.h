@@ -92,8 +92,9 @@ int isf_isjet(isf_t h, const float cen[3], float len, float rad, unsigned char* 
 int isf_restrict_velocity(isf_t h, float* psi, const unsigned char* isjet, const float kvec[3], float omgterm,
                           int phase_mode);
 /* add_particles!(particles, first+1:first+n, cen, rad), src/particles.jl:34-94.
- * 0-based slots first..first+n-1 (wrapped modulo nslots); the draws come from
- * Philox4x32-10 with counter (slot, step, 0, 0) and key = seed. */
+ * 0-based slots first..first+n-1 (wrapped modulo nslots).  Draw j comes from
+ * Philox4x32-10 with counter (j, step, 0, 0) and key = seed; each chunk of 1024 draws
+ * fills its slots in ring-angle order (neighbouring slots, neighbouring particles). */
 int isf_add_particles(isf_t h, float* parts, long long nslots, long long first, long long n, const float cen[3],
                       float rad, unsigned long long seed, unsigned step);
 /* staggered_advect!(particles, isf), src/parallel.jl:190-261 (vel == NULL: handle's). */
@@ -106,16 +107,21 @@ int isf_staggered_advect(isf_t h, float* parts, long long nslots, const float* v
 int isf_jet_configure(isf_t h, const float kvec[3], float omega, const float cen[3], float len, float rad,
                       long long n_spawn, unsigned long long seed, int phase_mode);
 /* Fill slots first..first+n-1 uniformly inside the configured nozzle cylinder (initial
- * particle cloud of the at-scale workloads; no reference counterpart). */
+ * particle cloud of the at-scale workloads; no reference counterpart).  Draw i comes from
+ * Philox4x32-10 with counter (i, 0xFFFFFFFF, 1, 0); draws are stored in increasing order
+ * of a 24-bit key of their raw words (x slab, theta wedge, r) so neighbouring slots hold
+ * neighbouring particles.  Synchronous. */
 int isf_fill_nozzle(isf_t h, float* parts, long long nslots, long long first, long long n, unsigned long long seed);
 /* The 10 constraint iterations of the setup, test/runtests.jl:27-30. */
 int isf_constrain(isf_t h, float* psi, int iterations);
 /* One simulation step number iter (1-based, t = iter*dt):
  * flow, normalize, project, spawn, restrict, project, one-form(hbar) * 1/d, RK4 advect. */
 int isf_step(isf_t h, float* psi, float* parts, long long nslots, int iter);
-/* isf_step advects the particle slots in a spatial order (slots grouped by cell) that
- * it recomputes every `steps` steps (default 32; 0 = slot order).  Each slot is advanced
- * independently, so the order never changes a result.  No reference counterpart. */
+/* isf_step measures how scattered the slot order of the particle buffer is (when the
+ * buffer changes and every 256 steps); if it is scattered, slots are advected in a
+ * spatial order (slots grouped by cell) recomputed every `steps` steps (default 32;
+ * 0 = always slot order).  Each slot is advanced independently, so the order never
+ * changes a result.  No reference counterpart. */
 int isf_set_sort_interval(isf_t h, int steps);
 /* FP64 sum over cells of |psi1|^2 + |psi2|^2 (no reference counterpart; health check). */
 int isf_norm2(isf_t h, const float* psi, double* out);

@@ -20,6 +20,50 @@
 #define M_PI 3.14159265358979323846
 #endif
 
+/* Slot order of the nozzle fill: draw indices sorted by the 24-bit key
+ * (x: top 5 bits of word 0, theta: top 7 bits of word 2, r: top 12 bits of word 1) of
+ * their raw Philox words, ties by index. */
+typedef struct { unsigned key; long long i; } isf_keyed;
+static int isf_keyed_cmp(const void* a, const void* b) {
+    const isf_keyed* x = (const isf_keyed*)a;
+    const isf_keyed* y = (const isf_keyed*)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->i < y->i ? -1 : (x->i > y->i);
+}
+static void isf_fill_order(long long n, unsigned long long seed, long long* order) {
+    isf_keyed* k = (isf_keyed*)malloc(sizeof(isf_keyed) * (size_t)(n > 0 ? n : 1));
+    for (long long i = 0; i < n; i++) {
+        unsigned ctr[4] = {(unsigned)i, 0xFFFFFFFFu, 1u, 0u};
+        unsigned key[2] = {(unsigned)seed, (unsigned)(seed >> 32)};
+        unsigned o[4];
+        isf_philox4x32_10(ctr, key, o);
+        k[i].key = ((o[0] >> 27) << 19) | ((o[2] >> 25) << 12) | (o[1] >> 20);
+        k[i].i = i;
+    }
+    qsort(k, (size_t)n, sizeof(isf_keyed), isf_keyed_cmp);
+    for (long long i = 0; i < n; i++) order[i] = k[i].i;
+    free(k);
+}
+
+/* Rank order of a spawn chunk: draws c0 .. c0+m-1 (m <= 1024) sorted by
+ * (top 22 bits of raw Philox word 0) << 10 | (index within the chunk). */
+static int isf_u32_cmp(const void* a, const void* b) {
+    const unsigned x = *(const unsigned*)a, y = *(const unsigned*)b;
+    return x < y ? -1 : (x > y);
+}
+static void isf_spawn_order(unsigned c0, int m, unsigned step, unsigned long long seed, int* rank_to_draw) {
+    unsigned keys[1024];
+    for (int t = 0; t < m; t++) {
+        unsigned ctr[4] = {c0 + (unsigned)t, step, 0u, 0u};
+        unsigned key[2] = {(unsigned)seed, (unsigned)(seed >> 32)};
+        unsigned o[4];
+        isf_philox4x32_10(ctr, key, o);
+        keys[t] = ((o[0] >> 10) << 10) | (unsigned)t;
+    }
+    qsort(keys, (size_t)m, sizeof(unsigned), isf_u32_cmp);
+    for (int k = 0; k < m; k++) rank_to_draw[k] = (int)(keys[k] & 1023u);
+}
+
 /* ---- Float32 instance */
 #define R float
 #define SUF _f32

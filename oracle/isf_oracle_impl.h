@@ -379,37 +379,49 @@ void FN(isfo_staggered_advect)(R* parts, long P, const R* vel, R dt, const R* d,
 }
 
 /* add_particles!, src/particles.jl:34-54 (CPU form: slot i for i in range).
- * Slots first .. first+n-1 (0-based, wrapped modulo P) are filled with
- *   (c_x - 0.1, c_y + 0.9 rad cos(rt), c_z + 0.9 rad sin(rt)) + (u1,u2,u3)*0.1 - 0.05,
- *   rt = u0 * 2 pi,
- * The reference draws u0..u3 from the unseeded rand(Float32); here they come from
- * Philox4x32-10 (isf_philox.h) with counter (slot, step, 0, 0) and key (seed_lo, seed_hi). */
+ * n particles at (c_x - 0.1, c_y + 0.9 rad cos(rt), c_z + 0.9 rad sin(rt))
+ * + (u1,u2,u3)*0.1 - 0.05 with rt = u0 * 2 pi, written to slots first .. first+n-1
+ * (0-based, wrapped modulo P).  The reference draws u0..u3 from the unseeded
+ * rand(Float32); here draw j of the batch comes from Philox4x32-10 (isf_philox.h) with
+ * counter (j, step, 0, 0) and key (seed_lo, seed_hi), and each chunk of 1024 consecutive
+ * draws fills the chunk's slots in increasing order of (top 22 bits of raw word 0) << 10
+ * | (j mod 1024) (isf_spawn_order in isf_oracle.c). */
 void FN(isfo_add_particles)(R* parts, long P, long first, long n, const R* cen, R rad,
                             unsigned long long seed, unsigned int step) {
-    for (long j = 0; j < n; j++) {
-        long s = (first + j) % P;
-        float u[4];
-        isf_philox_uniform4((unsigned)s, step, 0u, seed, u);
-        R rt = (R)u[0] * (R)2 * (R)M_PI;
-        R off[3];
-        for (int a = 0; a < 3; a++) off[a] = (R)u[a + 1] * (R)0.1 - (R)0.05;
-        R* p = parts + 3 * s;
-        p[0] = (cen[0] - (R)0.1) + off[0];
-        p[1] = (cen[1] + (R)0.9 * rad * FN(r_cos)(rt)) + off[1];
-        p[2] = (cen[2] + (R)0.9 * rad * FN(r_sin2)(rt)) + off[2];
+    for (long c0 = 0; c0 < n; c0 += 1024) {
+        const int m = (int)(n - c0 < 1024 ? n - c0 : 1024);
+        int rank_to_draw[1024];
+        isf_spawn_order((unsigned)c0, m, step, seed, rank_to_draw);
+        for (int k = 0; k < m; k++) {
+            const long s = (first + c0 + k) % P;
+            float u[4];
+            isf_philox_uniform4((unsigned)(c0 + rank_to_draw[k]), step, 0u, seed, u);
+            R rt = (R)u[0] * (R)2 * (R)M_PI;
+            R off[3];
+            for (int a = 0; a < 3; a++) off[a] = (R)u[a + 1] * (R)0.1 - (R)0.05;
+            R* p = parts + 3 * s;
+            p[0] = (cen[0] - (R)0.1) + off[0];
+            p[1] = (cen[1] + (R)0.9 * rad * FN(r_cos)(rt)) + off[1];
+            p[2] = (cen[2] + (R)0.9 * rad * FN(r_sin2)(rt)) + off[2];
+        }
     }
 }
 
-/* Initial particle cloud for the at-scale workloads: slots first .. first+n-1 drawn
- * uniformly inside the nozzle cylinder (axis along x, centre cen, length len,
- * radius rad).  No reference counterpart (the reference only spawns on a ring); the
- * draw uses Philox4x32-10 with counter (slot, 0xFFFFFFFF, 1, 0). */
-void FN(isfo_fill_cylinder)(R* parts, long P, long first, long n, const R* cen, R len, R rad,
+/* Initial particle cloud for the at-scale workloads: n particles uniform inside the
+ * nozzle cylinder (axis along x, centre cen, length len, radius rad).  No reference
+ * counterpart (the reference only spawns on a ring).  Draw i uses Philox4x32-10 with
+ * counter (i, 0xFFFFFFFF, 1, 0); x = c_x + len (u0 - 1/2), r = rad sqrt(u1),
+ * theta = 2 pi u2.  Draws go to slots first, first+1, ... (mod P) in increasing order of
+ * the 24-bit key (x slab, theta wedge, r) built from the top bits of the raw words
+ * (isf_fill_order in isf_oracle.c), ties by draw index. */
+void FN(isfo_fill_cylinder)(R* parts, long long P, long long first, long long n, const R* cen, R len, R rad,
                             unsigned long long seed) {
-    for (long j = 0; j < n; j++) {
-        long s = (first + j) % P;
+    long long* order = (long long*)malloc(sizeof(long long) * (size_t)(n > 0 ? n : 1));
+    isf_fill_order(n, seed, order);
+    for (long long k = 0; k < n; k++) {
+        const long long s = (first + k) % P;
         float u[4];
-        isf_philox_uniform4((unsigned)s, 0xFFFFFFFFu, 1u, seed, u);
+        isf_philox_uniform4((unsigned)order[k], 0xFFFFFFFFu, 1u, seed, u);
         R r = rad * (R)sqrt((double)u[1]);
         R th = (R)u[2] * (R)2 * (R)M_PI;
         R* p = parts + 3 * s;
@@ -417,6 +429,7 @@ void FN(isfo_fill_cylinder)(R* parts, long P, long first, long n, const R* cen, 
         p[1] = cen[1] + r * FN(r_cos)(th);
         p[2] = cen[2] + r * FN(r_sin2)(th);
     }
+    free(order);
 }
 
 /* ---------------------------------------------------------------- one step */

@@ -151,7 +151,7 @@ class Oracle:
 
     def fill_cylinder(self, parts, first, n, cen, length, rad, seed):
         parts = self._arr(parts).copy()
-        self._fn("isfo_fill_cylinder")(_p(parts), C.c_long(parts.shape[0]), C.c_long(first), C.c_long(n),
+        self._fn("isfo_fill_cylinder")(_p(parts), C.c_longlong(parts.shape[0]), C.c_longlong(first), C.c_longlong(n),
                                        _p(self._r3(cen)), self.R(length), self.R(rad), C.c_ulonglong(seed))
         return parts
 

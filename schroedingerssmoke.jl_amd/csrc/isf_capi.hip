@@ -50,8 +50,11 @@ struct isf_handle {
     GridInfo g;
     float hbar, dt;
     size_t N;
-    hipStream_t stream;
+    hipStream_t stream;   // the handle's stream: every operator, the grid part of isf_step
+    hipStream_t pstream;  // particle side of isf_step (spawn, order, advect), joined by events
     hipEvent_t ev0, ev1;
+    hipEvent_t ev_vel, ev_adv;
+    bool side_pending;    // an advect on pstream may still read vel / write particles
     void* table_mem;
     DevTables tb;
     float2* S;          // Schroedinger spectrum, 2N complex; f and P alias it in projections
@@ -74,6 +77,10 @@ struct isf_handle {
     long long order_n;
     int order_iter;
     int sort_interval;
+    int order_check_iter;
+    bool order_sorted;               // slots are advected through `order` (else slot order)
+    unsigned* jump_partial;          // per-block counts of the order measure
+    long long jump_cap;
 };
 
 static float* f_of(isf_t h) { return reinterpret_cast<float*>(h->S); }
@@ -86,6 +93,21 @@ static int check_handle(isf_t h) {
 #define CHECK_H(h)                              \
     do {                                        \
         int r_ = check_handle(h);               \
+        if (r_) return r_;                      \
+    } while (0)
+// Order the handle's stream after a pending particle-side advect of isf_step.
+static int join(isf_t h) {
+    if (h->side_pending) {
+        hipError_t e = hipStreamWaitEvent(h->stream, h->ev_adv, 0);
+        if (e != hipSuccess) return fail(ISF_EHIP, "hipStreamWaitEvent: %s", hipGetErrorString(e));
+        h->side_pending = false;
+    }
+    return ISF_OK;
+}
+#define CHECK_HJ(h)                             \
+    do {                                        \
+        int r_ = check_handle(h);               \
+        if (!r_) r_ = join(h);                  \
         if (r_) return r_;                      \
     } while (0)
 #define CHECK_PTR(p, name)                                                   \
@@ -260,6 +282,11 @@ int isf_create(isf_t* out, const int res[3], const int box[3], float hbar, float
     // a blocking stream: it also orders against the null stream used by plain hipMemcpy
     if ((e = hipStreamCreate(&h->stream)) != hipSuccess)
         return cleanup(hip_fail(e, "hipStreamCreate"));
+    if ((e = hipStreamCreate(&h->pstream)) != hipSuccess) return cleanup(hip_fail(e, "hipStreamCreate"));
+    if ((e = hipEventCreateWithFlags(&h->ev_vel, hipEventDisableTiming)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipEventCreate"));
+    if ((e = hipEventCreateWithFlags(&h->ev_adv, hipEventDisableTiming)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipEventCreate"));
     if ((e = hipEventCreate(&h->ev0)) != hipSuccess) return cleanup(hip_fail(e, "hipEventCreate"));
     if ((e = hipEventCreate(&h->ev1)) != hipSuccess) return cleanup(hip_fail(e, "hipEventCreate"));
     *out = h;
@@ -269,6 +296,10 @@ int isf_create(isf_t* out, const int res[3], const int box[3], float hbar, float
 int isf_destroy(isf_t h) {
     CHECK_H(h);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->pstream) (void)hipStreamSynchronize(h->pstream);
+    if (h->ev_vel) (void)hipEventDestroy(h->ev_vel);
+    if (h->ev_adv) (void)hipEventDestroy(h->ev_adv);
+    if (h->pstream) (void)hipStreamDestroy(h->pstream);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -277,6 +308,7 @@ int isf_destroy(isf_t h) {
     if (h->norm_parts) (void)hipFree(h->norm_parts);
     if (h->order) (void)hipFree(h->order);
     if (h->order_scratch) (void)hipFree(h->order_scratch);
+    if (h->jump_partial) (void)hipFree(h->jump_partial);
     if (h->table_mem) (void)hipFree(h->table_mem);
     h->magic = 0;
     delete h;
@@ -284,7 +316,7 @@ int isf_destroy(isf_t h) {
 }
 
 int isf_synchronize(isf_t h) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     HIP_CHECK(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
     return ISF_OK;
 }
@@ -312,7 +344,7 @@ int isf_get_grid(isf_t h, int res[3], float d[3], float box[3], float* hbar, flo
 }
 
 int isf_velocity_ptr(isf_t h, float** out) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(out, "out");
     *out = h->vel;
     return ISF_OK;
@@ -341,20 +373,20 @@ static int project(isf_t h, float* psi, int c2r_mode, const JetParams* jet, doub
 }
 
 int isf_schroedinger_flow(isf_t h, float* psi) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(psi, "psi");
     return flow(h, psi, false);
 }
 
 int isf_normalize(isf_t h, float* psi) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(psi, "psi");
     LAUNCH_CHECK(launch_normalize(h->g, reinterpret_cast<float4*>(psi), h->stream), "normalize");
     return ISF_OK;
 }
 
 int isf_velocity_one_form(isf_t h, const float* psi, float hbar, int sharp, float* vel) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(psi, "psi");
     LAUNCH_CHECK(launch_velocity(h->g, reinterpret_cast<const float4*>(psi), hbar, sharp, vel ? vel : h->vel, h->stream),
                  "velocity_one_form");
@@ -362,14 +394,14 @@ int isf_velocity_one_form(isf_t h, const float* psi, float hbar, int sharp, floa
 }
 
 int isf_div(isf_t h, const float* vel, float* f) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(f, "f");
     LAUNCH_CHECK(launch_div_vel(h->g, vel ? vel : h->vel, f, h->stream), "div");
     return ISF_OK;
 }
 
 int isf_poisson_solve(isf_t h, const float* f, float* q) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(f, "f");
     CHECK_PTR(q, "q");
     LAUNCH_CHECK(launch_r2c_x(h->g, f, P_of(h), h->tb, h->stream), "poisson x forward");
@@ -379,7 +411,7 @@ int isf_poisson_solve(isf_t h, const float* f, float* q) {
 }
 
 int isf_gauge_transform(isf_t h, float* psi, const float* q) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(psi, "psi");
     CHECK_PTR(q, "q");
     LAUNCH_CHECK(launch_gauge(h->g, reinterpret_cast<float4*>(psi), q, h->stream), "gauge_transform");
@@ -387,7 +419,7 @@ int isf_gauge_transform(isf_t h, float* psi, const float* q) {
 }
 
 int isf_pressure_project(isf_t h, float* psi) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(psi, "psi");
     return project(h, psi, 1, nullptr, nullptr);
 }
@@ -403,7 +435,7 @@ static JetParams jet_from(const float cen[3], float len, float rad) {
 }
 
 int isf_isjet(isf_t h, const float cen[3], float len, float rad, unsigned char* out) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(cen, "cen");
     CHECK_PTR(out, "out");
     LAUNCH_CHECK(launch_isjet(h->g, jet_from(cen, len, rad), out, h->stream), "isjet");
@@ -412,7 +444,7 @@ int isf_isjet(isf_t h, const float cen[3], float len, float rad, unsigned char* 
 
 int isf_restrict_velocity(isf_t h, float* psi, const unsigned char* isjet, const float kvec[3], float omgterm,
                           int phase_mode) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(psi, "psi");
     CHECK_PTR(kvec, "kvec");
     if (phase_mode != ISF_PHASE_DOT && phase_mode != ISF_PHASE_LEGACY) return fail(ISF_EINVAL, "bad phase_mode %d", phase_mode);
@@ -429,7 +461,7 @@ int isf_restrict_velocity(isf_t h, float* psi, const unsigned char* isjet, const
 
 int isf_add_particles(isf_t h, float* parts, long long nslots, long long first, long long n, const float cen[3],
                       float rad, unsigned long long seed, unsigned step) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(parts, "parts");
     CHECK_PTR(cen, "cen");
     if (nslots <= 0 || first < 0 || n < 0) return fail(ISF_EINVAL, "bad particle range");
@@ -439,7 +471,7 @@ int isf_add_particles(isf_t h, float* parts, long long nslots, long long first, 
 }
 
 int isf_staggered_advect(isf_t h, float* parts, long long nslots, const float* vel) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(parts, "parts");
     if (nslots < 0) return fail(ISF_EINVAL, "bad nslots");
     LAUNCH_CHECK(launch_advect(h->g, parts, nslots, nullptr, vel ? vel : h->vel, h->dt, h->stream), "staggered_advect");
@@ -467,7 +499,7 @@ int isf_jet_configure(isf_t h, const float kvec[3], float omega, const float cen
 }
 
 int isf_fill_nozzle(isf_t h, float* parts, long long nslots, long long first, long long n, unsigned long long seed) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(parts, "parts");
     if (!h->jet_set) return fail(ISF_EINVAL, "isf_jet_configure must be called first");
     if (nslots <= 0 || first < 0 || n < 0) return fail(ISF_EINVAL, "bad particle range");
@@ -476,7 +508,7 @@ int isf_fill_nozzle(isf_t h, float* parts, long long nslots, long long first, lo
 }
 
 int isf_constrain(isf_t h, float* psi, int iterations) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(psi, "psi");
     if (!h->jet_set) return fail(ISF_EINVAL, "isf_jet_configure must be called first");
     JetParams j = h->jet;
@@ -489,7 +521,8 @@ int isf_constrain(isf_t h, float* psi, int iterations) {
     return ISF_OK;
 }
 
-static int ensure_order(isf_t h, const float* parts, long long nslots, int iter, const int** order);
+static int ensure_order(isf_t h, const float* parts, long long nslots, int iter, const int** order,
+                        hipStream_t stream);
 
 // Stage names of one step, in launch order (isf_step_profile reports one time each).
 static const char* kStageNames[] = {
@@ -500,15 +533,21 @@ static const char* kStageNames[] = {
     "velocity_sharp", "particle_order", "advect"};                               // :59-63
 static constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 
-// One simloop body (test/runtests.jl:45-64).  If ev is non-null (kNumStages + 1 events),
-// ev[0] is recorded on the handle's stream before the first stage and ev[k + 1] after
-// stage k.
+// One simloop body (test/runtests.jl:45-64).  The grid part runs on the handle's stream;
+// the particle part (spawn, processing order, advect) runs on pstream: the spawn of step
+// k after the advect of step k-1, the advect of step k after the velocity of step k, and
+// the velocity of step k+1 after the advect of step k, so the latency-bound particle work
+// overlaps the next step's transforms.  If ev is non-null (profiling, kNumStages + 1
+// events) everything runs on the handle's stream in stage order, ev[0] recorded before
+// the first stage and ev[k + 1] after stage k.
 static int step_impl(isf_t h, float* psi, float* parts, long long nslots, int iter, hipEvent_t* ev) {
     const float t = (float)iter * h->dt;  // test/runtests.jl:46
     JetParams j = h->jet;
     j.omgterm = h->omega * t;             // test/runtests.jl:57
     float4* p = reinterpret_cast<float4*>(psi);
     hipStream_t s = h->stream;
+    const bool side = ev == nullptr && nslots > 0;
+    hipStream_t ps = side ? h->pstream : s;
     int k = 0;
     auto mark = [&]() -> int {
         if (ev) {
@@ -523,7 +562,19 @@ static int step_impl(isf_t h, float* psi, float* parts, long long nslots, int it
         int m_ = mark();               \
         if (m_) return m_;             \
     } while (0)
-    if (ev) HIP_CHECK(hipEventRecord(ev[0], s), "hipEventRecord");
+    if (ev) {
+        int r = join(h);
+        if (r) return r;
+        HIP_CHECK(hipEventRecord(ev[0], s), "hipEventRecord");
+    }
+    // :52-54 add_particles! into the ring buffer (first 0-based slot of the batch)
+    long long first0 = 0, nsp = 0;
+    if (nslots > 0 && h->n_spawn > 0) {
+        const long long start1 = ((long long)(iter - 1) * h->n_spawn + 1) % nslots;
+        first0 = start1 == 0 ? nslots - 1 : start1 - 1;
+        nsp = h->n_spawn;
+    }
+    if (side) LAUNCH_CHECK(launch_spawn(parts, nslots, first0, nsp, h->jet, h->seed, (unsigned)iter, ps), "add_particles");
     // :48-49 schroedinger_flow! + normalize
     STAGE(launch_flow_x_fwd(h->g, p, h->S, h->tb, s), "flow x forward");
     STAGE(launch_flow_plane(h->g, h->S, h->tb, s), "flow plane");
@@ -534,36 +585,65 @@ static int step_impl(isf_t h, float* psi, float* parts, long long nslots, int it
     STAGE(launch_r2c_x(h->g, f_of(h), P_of(h), h->tb, s), "poisson x forward");
     STAGE(launch_poisson_plane(h->g, P_of(h), sc, h->tb, s), "poisson plane");
     STAGE(launch_c2r_x(h->g, P_of(h), 2, nullptr, p, &j, nullptr, h->tb, s), "poisson x inverse + gauge + restrict");
-    // :52-54 add_particles! into the ring buffer
-    long long first0 = 0, nsp = 0;
-    if (nslots > 0 && h->n_spawn > 0) {
-        const long long start1 = ((long long)(iter - 1) * h->n_spawn + 1) % nslots;
-        first0 = start1 == 0 ? nslots - 1 : start1 - 1;
-        nsp = h->n_spawn;
-    }
-    STAGE(launch_spawn(parts, nslots, first0, nsp, h->jet, h->seed, (unsigned)iter, s), "add_particles");
+    if (!side) STAGE(launch_spawn(parts, nslots, first0, nsp, h->jet, h->seed, (unsigned)iter, s), "add_particles");
     // :58 pressure_project! (FP64 norm partials accumulated in its gauge pass)
     STAGE(launch_div_psi(h->g, p, f_of(h), s), "div(one_form(psi))");
     STAGE(launch_r2c_x(h->g, f_of(h), P_of(h), h->tb, s), "poisson x forward");
     STAGE(launch_poisson_plane(h->g, P_of(h), sc, h->tb, s), "poisson plane");
     STAGE(launch_c2r_x(h->g, P_of(h), 1, nullptr, p, nullptr, h->norm_parts, h->tb, s), "poisson x inverse + gauge");
-    // :59-62 velocity_one_form!(isf, psi, hbar) and the in-place StaggeredSharp
+    // :59-62 velocity_one_form!(isf, psi, hbar) and the in-place StaggeredSharp; the
+    // previous step's advect must be done reading vel
+    {
+        int r = join(h);
+        if (r) return r;
+    }
     STAGE(launch_velocity(h->g, p, h->hbar, 1, h->vel, s), "velocity_one_form");
-    // :63 staggered_advect!, slots processed in a spatial order refreshed every
-    // sort_interval steps (particles move less than a cell per step)
+    // :63 staggered_advect!, slots processed in a spatial order when the slot order is
+    // scattered (particles move less than a cell per step)
+    if (side) {
+        HIP_CHECK(hipEventRecord(h->ev_vel, s), "hipEventRecord");
+        HIP_CHECK(hipStreamWaitEvent(ps, h->ev_vel, 0), "hipStreamWaitEvent");
+    }
     const int* order = nullptr;
-    int r = ensure_order(h, parts, nslots, iter, &order);
+    int r = ensure_order(h, parts, nslots, iter, &order, ps);
     if (r) return r;
     if ((r = mark())) return r;
-    STAGE(launch_advect(h->g, parts, nslots, order, h->vel, h->dt, s), "staggered_advect");
+    STAGE(launch_advect(h->g, parts, nslots, order, h->vel, h->dt, ps), "staggered_advect");
+    if (side) {
+        HIP_CHECK(hipEventRecord(h->ev_adv, ps), "hipEventRecord");
+        h->side_pending = true;
+    }
 #undef STAGE
     return ISF_OK;
 }
 
-// Refresh the particle processing order when the buffer changed or it is stale.
-static int ensure_order(isf_t h, const float* parts, long long nslots, int iter, const int** order) {
+// Particle processing order.  When the buffer changes, and every 256 steps after, the
+// slot order is measured (one host synchronisation): if more than a quarter of the
+// consecutive slot pairs are more than two cells apart, slots are advected through a
+// spatial sort refreshed every sort_interval steps; otherwise in slot order.
+static int ensure_order(isf_t h, const float* parts, long long nslots, int iter, const int** order,
+                        hipStream_t stream) {
     *order = nullptr;
     if (nslots <= 0 || h->sort_interval <= 0) return ISF_OK;
+    const bool changed = parts != h->order_parts || nslots != h->order_n;
+    if (changed || iter < h->order_check_iter || iter - h->order_check_iter >= 256) {
+        if (nslots > h->jump_cap) {
+            if (h->jump_partial) (void)hipFree(h->jump_partial);
+            h->jump_partial = nullptr;
+            h->jump_cap = 0;
+            hipError_t e = hipMalloc(&h->jump_partial, sizeof(unsigned) * order_measure_ints(nslots));
+            if (e != hipSuccess) return fail(ISF_ENOMEM, "order measure: %s", hipGetErrorString(e));
+            h->jump_cap = nslots;
+        }
+        unsigned long long jumps = 0;
+        LAUNCH_CHECK(measure_order_jumps(h->g, parts, nslots, h->jump_partial, &jumps, stream), "particle order measure");
+        h->order_sorted = jumps * 4 > (unsigned long long)nslots;
+        h->order_check_iter = iter;
+        h->order_parts = parts;
+        h->order_n = nslots;
+        h->order_iter = iter - h->sort_interval;  // force a refresh below if sorting
+    }
+    if (!h->order_sorted) return ISF_OK;
     if (nslots > h->order_cap) {
         if (h->order) (void)hipFree(h->order);
         if (h->order_scratch) (void)hipFree(h->order_scratch);
@@ -576,15 +656,12 @@ static int ensure_order(isf_t h, const float* parts, long long nslots, int iter,
         if (e != hipSuccess) return fail(ISF_ENOMEM, "particle order: %s", hipGetErrorString(e));
         h->order_scratch_bytes = sb;
         h->order_cap = nslots;
-        h->order_parts = nullptr;
+        h->order_iter = iter - h->sort_interval;
     }
-    if (parts != h->order_parts || nslots != h->order_n || iter < h->order_iter ||
-        iter - h->order_iter >= h->sort_interval) {
+    if (iter < h->order_iter || iter - h->order_iter >= h->sort_interval) {
         LAUNCH_CHECK(launch_particle_order(h->g, parts, nslots, h->order, h->order_scratch, h->order_scratch_bytes,
-                                           h->stream),
+                                           stream),
                      "particle order");
-        h->order_parts = parts;
-        h->order_n = nslots;
         h->order_iter = iter;
     }
     *order = h->order;
@@ -596,6 +673,7 @@ int isf_set_sort_interval(isf_t h, int steps) {
     if (steps < 0) return fail(ISF_EINVAL, "sort interval must be >= 0");
     h->sort_interval = steps;
     h->order_parts = nullptr;
+    h->order_n = 0;
     return ISF_OK;
 }
 
@@ -643,7 +721,7 @@ int isf_step_profile(isf_t h, float* psi, float* parts, long long nslots, int it
 }
 
 int isf_norm2(isf_t h, const float* psi, double* out) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(psi, "psi");
     CHECK_PTR(out, "out");
     size_t np = 0;
@@ -655,7 +733,7 @@ int isf_norm2(isf_t h, const float* psi, double* out) {
 }
 
 int isf_step_norm2(isf_t h, double* out) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(out, "out");
     LAUNCH_CHECK(launch_reduce_sum(h->norm_parts, c2r_x_blocks(h->g), h->norm_out, h->stream), "norm2 reduce");
     HIP_CHECK(hipMemcpyAsync(out, h->norm_out, sizeof(double), hipMemcpyDeviceToHost, h->stream), "norm2 copy");
@@ -664,12 +742,12 @@ int isf_step_norm2(isf_t h, double* out) {
 }
 
 int isf_timer_start(isf_t h) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     HIP_CHECK(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
     return ISF_OK;
 }
 int isf_timer_stop(isf_t h, float* ms) {
-    CHECK_H(h);
+    CHECK_HJ(h);
     CHECK_PTR(ms, "ms");
     HIP_CHECK(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
     HIP_CHECK(hipEventSynchronize(h->ev1), "hipEventSynchronize");

@@ -65,8 +65,14 @@ int launch_reduce_sum(const double* partials, size_t n, double* out, hipStream_t
 // ---- particles
 int launch_spawn(float* parts, long long nslots, long long first, long long n, const JetParams& jet,
                  unsigned long long seed, unsigned step, hipStream_t s);
+// Uniform cloud in the nozzle cylinder in spatially coherent slot order (synchronous).
 int launch_fill_cylinder(float* parts, long long nslots, long long first, long long n, const JetParams& jet,
                          unsigned long long seed, hipStream_t s);
+// Number of consecutive in-box slot pairs more than two cells apart (synchronous);
+// dev_partial holds order_measure_ints(nslots) unsigned ints.
+size_t order_measure_ints(long long nslots);
+int measure_order_jumps(const GridInfo& g, const float* parts, long long nslots, unsigned* dev_partial,
+                        unsigned long long* host_count, hipStream_t s);
 int launch_advect(const GridInfo& g, float* parts, long long nslots, const int* order, const float* vel, float dt,
                   hipStream_t s);
 // Slot processing order for the advection: slots sorted by bucket of 8 x-cells (stable

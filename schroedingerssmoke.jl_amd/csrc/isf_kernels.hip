@@ -59,6 +59,14 @@ __device__ __forceinline__ float4 jet_phase(float4 p, int x, int y, int z, const
     return make_float4(a1 * c, a1 * s, a2 * c, a2 * s);
 }
 
+// normalize_psi (src/parallel.jl:162-165): psi_a / hypot(|psi1|, |psi2|).  |psi| is O(1)
+// here, so the norm is formed as sqrt(sum of squares) (the reference uses @fastmath
+// hypot) and applied as one reciprocal.
+__device__ __forceinline__ float4 normalized(float4 p) {
+    const float inv = 1.0f / sqrtf((p.x * p.x + p.y * p.y) + (p.z * p.z + p.w * p.w));
+    return make_float4(p.x * inv, p.y * inv, p.z * inv, p.w * inv);
+}
+
 // block-wide sum of a double (NT a power of two); result valid in thread 0.
 // Fixed tree order, so the sum is deterministic.
 template <int NT>
@@ -134,10 +142,7 @@ k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, int ny, int
         const int l = i / NX, x = i % NX;
         const float2 a = lds[l * LS + pad16(x)], b = lds[(TY + l) * LS + pad16(x)];
         float4 p = make_float4(a.x, a.y, b.x, b.y);
-        if (NORMALIZE) {
-            const float nrm = hypotf(hypotf(p.x, p.y), hypotf(p.z, p.w));
-            p = make_float4(p.x / nrm, p.y / nrm, p.z / nrm, p.w / nrm);
-        }
+        if (NORMALIZE) p = normalized(p);
         dst[i] = p;
     }
 }
@@ -518,9 +523,7 @@ __global__ void __launch_bounds__(256) k_div_vel(const float* __restrict__ vel, 
 __global__ void __launch_bounds__(256) k_normalize(float4* __restrict__ psi, size_t N) {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
-    float4 p = psi[c];
-    const float nrm = hypotf(hypotf(p.x, p.y), hypotf(p.z, p.w));
-    psi[c] = make_float4(p.x / nrm, p.y / nrm, p.z / nrm, p.w / nrm);
+    psi[c] = normalized(psi[c]);
 }
 
 __global__ void __launch_bounds__(256) k_gauge(float4* __restrict__ psi, const float* __restrict__ q, size_t N) {
@@ -665,40 +668,65 @@ k_advect(float* __restrict__ parts, long long nslots, const int* __restrict__ or
     for (int a = 0; a < 3; a++) parts[3 * s + a] = p[a] + dt / 6.0f * (((k1[a] + 2.0f * k2[a]) + 2.0f * k3[a]) + k4[a]);
 }
 
-// add_particles! (src/particles.jl:34-54,75-94): slot (first + j) mod nslots gets
-// (c_x - 0.1, c_y + 0.9 rad cos(rt), c_z + 0.9 rad sin(rt)) + u*0.1 - 0.05.
-__global__ void __launch_bounds__(256)
-k_spawn(float* __restrict__ parts, long long nslots, long long first, long long n, JetParams j,
+// add_particles! (src/particles.jl:34-54,75-94): n particles at
+// (c_x - 0.1, c_y + 0.9 rad cos(rt), c_z + 0.9 rad sin(rt)) + u*0.1 - 0.05, rt = 2 pi u0.
+// Draw j of the batch uses Philox4x32-10 with counter (j, step, 0, 0).  Each chunk of
+// 1024 consecutive draws is stored in the chunk's slots first + chunk*1024 + k (mod
+// nslots) in increasing order of key = (top 22 bits of raw word 0, i.e. the ring angle)
+// << 10 | (j mod 1024), so neighbouring slots hold neighbouring points of the ring.
+constexpr int SPAWN_CHUNK = 1024;
+
+__device__ __forceinline__ void spawn_draw(uint32_t j, unsigned step, unsigned long long seed, uint32_t raw[4]) {
+    raw[0] = j;
+    raw[1] = step;
+    raw[2] = 0u;
+    raw[3] = 0u;
+    philox4x32_10(raw, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+__global__ void __launch_bounds__(SPAWN_CHUNK)
+k_spawn(float* __restrict__ parts, long long nslots, long long first, long long n, JetParams jt,
         unsigned long long seed, unsigned step) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const long long s = (first + i) % nslots;
+    __shared__ uint32_t key[SPAWN_CHUNK];
+    const int t = threadIdx.x;
+    const long long base = (long long)blockIdx.x * SPAWN_CHUNK;
+    const int m = (int)min((long long)SPAWN_CHUNK, n - base);
+    uint32_t raw[4];
+    if (t < m) {
+        spawn_draw((uint32_t)(base + t), step, seed, raw);
+        key[t] = ((raw[0] >> 10) << 10) | (uint32_t)t;
+    } else {
+        key[t] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    // bitonic sort of the 1024 keys (unused entries sort last)
+    for (int size = 2; size <= SPAWN_CHUNK; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const int o = t ^ stride;
+            if (o > t) {
+                const uint32_t a = key[t], b = key[o];
+                const bool up = (t & size) == 0;
+                if ((a > b) == up) {
+                    key[t] = b;
+                    key[o] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (t >= m) return;
+    const uint32_t j = (uint32_t)base + (key[t] & (SPAWN_CHUNK - 1));
+    spawn_draw(j, step, seed, raw);
     float u[4];
-    philox_uniform4((uint32_t)s, step, 0u, seed, u);
+#pragma unroll
+    for (int i = 0; i < 4; i++) u[i] = (float)(raw[i] >> 8) * (1.0f / 16777216.0f);
     const float rt = u[0] * 2.0f * 3.14159265358979323846f;
     float sn, cs;
     sincosf(rt, &sn, &cs);
-    parts[3 * s + 0] = (j.cx - 0.1f) + (u[1] * 0.1f - 0.05f);
-    parts[3 * s + 1] = (j.cy + 0.9f * j.rad * cs) + (u[2] * 0.1f - 0.05f);
-    parts[3 * s + 2] = (j.cz + 0.9f * j.rad * sn) + (u[3] * 0.1f - 0.05f);
-}
-
-// Initial cloud uniform in the nozzle cylinder (no reference counterpart).
-__global__ void __launch_bounds__(256)
-k_fill_cylinder(float* __restrict__ parts, long long nslots, long long first, long long n, JetParams j,
-                unsigned long long seed) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const long long s = (first + i) % nslots;
-    float u[4];
-    philox_uniform4((uint32_t)s, 0xFFFFFFFFu, 1u, seed, u);
-    const float r = j.rad * sqrtf(u[1]);
-    const float th = u[2] * 2.0f * 3.14159265358979323846f;
-    float sn, cs;
-    sincosf(th, &sn, &cs);
-    parts[3 * s + 0] = j.cx + j.len * (u[0] - 0.5f);
-    parts[3 * s + 1] = j.cy + r * cs;
-    parts[3 * s + 2] = j.cz + r * sn;
+    const long long s = (first + base + t) % nslots;
+    parts[3 * s + 0] = (jt.cx - 0.1f) + (u[1] * 0.1f - 0.05f);
+    parts[3 * s + 1] = (jt.cy + 0.9f * jt.rad * cs) + (u[2] * 0.1f - 0.05f);
+    parts[3 * s + 2] = (jt.cz + 0.9f * jt.rad * sn) + (u[3] * 0.1f - 0.05f);
 }
 
 // ======================================================================== launchers
@@ -876,12 +904,7 @@ int launch_reduce_sum(const double* partials, size_t n, double* out, hipStream_t
 int launch_spawn(float* parts, long long nslots, long long first, long long n, const JetParams& jet,
                  unsigned long long seed, unsigned step, hipStream_t s) {
     if (n <= 0) return 0;
-    ISF_LAUNCH(k_spawn, grid1d((size_t)n, 256), dim3(256), s, parts, nslots, first, n, jet, seed, step);
-}
-int launch_fill_cylinder(float* parts, long long nslots, long long first, long long n, const JetParams& jet,
-                         unsigned long long seed, hipStream_t s) {
-    if (n <= 0) return 0;
-    ISF_LAUNCH(k_fill_cylinder, grid1d((size_t)n, 256), dim3(256), s, parts, nslots, first, n, jet, seed);
+    ISF_LAUNCH(k_spawn, grid1d((size_t)n, SPAWN_CHUNK), dim3(SPAWN_CHUNK), s, parts, nslots, first, n, jet, seed, step);
 }
 int launch_advect(const GridInfo& g, float* parts, long long nslots, const int* order, const float* vel, float dt,
                   hipStream_t s) {

@@ -7,9 +7,12 @@
 // order never changes a result.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "isf_internal.hpp"
+#include "isf_philox.hpp"
 
 namespace isf {
 
@@ -68,6 +71,110 @@ int launch_particle_order(const GridInfo& g, const float* parts, long long nslot
     if (e != hipSuccess) return (int)e;
     e = rocprim::radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, slots, order, (size_t)nslots, 0u, key_bits(g), s);
     return (int)e;
+}
+
+// ---- initial particle cloud uniform in the nozzle cylinder, stored in a spatially
+// coherent slot order.  Draw i uses Philox4x32-10 with counter (i, 0xFFFFFFFF, 1, 0):
+// x = c_x + len (u0 - 1/2), r = rad sqrt(u1), theta = 2 pi u2.  Draws are assigned to
+// slots in increasing order of the 24-bit key (x slab: top 5 bits of word 0, theta wedge:
+// top 7 bits of word 2, r: top 12 bits of word 1; ties by draw index), so a wavefront of
+// consecutive slots holds particles from a few neighbouring cells.  No reference
+// counterpart.
+__global__ void __launch_bounds__(256)
+k_fill_keys(long long n, unsigned long long seed, unsigned* __restrict__ keys, int* __restrict__ idx) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x[4] = {(uint32_t)i, 0xFFFFFFFFu, 1u, 0u};
+    philox4x32_10(x, (uint32_t)seed, (uint32_t)(seed >> 32));
+    keys[i] = ((x[0] >> 27) << 19) | ((x[2] >> 25) << 12) | (x[1] >> 20);
+    idx[i] = (int)i;
+}
+
+__global__ void __launch_bounds__(256)
+k_fill_ordered(float* __restrict__ parts, long long nslots, long long first, long long n, const int* __restrict__ draw,
+               JetParams j, unsigned long long seed) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const long long s = (first + k) % nslots;
+    float u[4];
+    philox_uniform4((uint32_t)draw[k], 0xFFFFFFFFu, 1u, seed, u);
+    const float r = j.rad * sqrtf(u[1]);
+    const float th = u[2] * 2.0f * 3.14159265358979323846f;
+    float sn, cs;
+    sincosf(th, &sn, &cs);
+    parts[3 * s + 0] = j.cx + j.len * (u[0] - 0.5f);
+    parts[3 * s + 1] = j.cy + r * cs;
+    parts[3 * s + 2] = j.cz + r * sn;
+}
+
+int launch_fill_cylinder(float* parts, long long nslots, long long first, long long n, const JetParams& jet,
+                         unsigned long long seed, hipStream_t s) {
+    if (n <= 0) return 0;
+    size_t tmp_bytes = 0;
+    (void)rocprim::radix_sort_pairs(nullptr, tmp_bytes, (unsigned*)nullptr, (unsigned*)nullptr, (int*)nullptr,
+                                    (int*)nullptr, (size_t)n, 0u, 24u);
+    const size_t a = align256(sizeof(int) * (size_t)n);
+    void* mem = nullptr;
+    hipError_t e = hipMalloc(&mem, 4 * a + align256(tmp_bytes));
+    if (e != hipSuccess) return (int)e;
+    char* p = static_cast<char*>(mem);
+    unsigned* keys_in = reinterpret_cast<unsigned*>(p);
+    unsigned* keys_out = reinterpret_cast<unsigned*>(p + a);
+    int* idx = reinterpret_cast<int*>(p + 2 * a);
+    int* draw = reinterpret_cast<int*>(p + 3 * a);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(k_fill_keys, grid, dim3(256), 0, s, n, seed, keys_in, idx);
+    e = hipGetLastError();
+    if (e == hipSuccess)
+        e = rocprim::radix_sort_pairs(p + 4 * a, tmp_bytes, keys_in, keys_out, idx, draw, (size_t)n, 0u, 24u, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_fill_ordered, grid, dim3(256), 0, s, parts, nslots, first, n, draw, jet, seed);
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(mem);
+    return (int)(e != hipSuccess ? e : e2);
+}
+
+// Order measure of a particle buffer: number of consecutive slot pairs, both inside the
+// box in x, more than two cells apart on some axis; one partial count per block.
+__global__ void __launch_bounds__(256)
+k_order_jumps(const float* __restrict__ parts, long long nslots, GridInfo g, unsigned* __restrict__ partial) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned jump = 0;
+    if (s + 1 < nslots) {
+        const float* a = parts + 3 * s;
+        const float* b = a + 3;
+        const bool in = (a[0] > 0.0f) & (a[0] < g.Lx) & (b[0] > 0.0f) & (b[0] < g.Lx);
+        jump = in && (fabsf(a[0] - b[0]) > 2.0f * g.dx || fabsf(a[1] - b[1]) > 2.0f * g.dy ||
+                      fabsf(a[2] - b[2]) > 2.0f * g.dz);
+    }
+    __shared__ unsigned sh[256];
+    sh[threadIdx.x] = jump;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+size_t order_measure_ints(long long nslots) { return (size_t)((nslots + 255) / 256); }
+
+int measure_order_jumps(const GridInfo& g, const float* parts, long long nslots, unsigned* dev_partial,
+                        unsigned long long* host_count, hipStream_t s) {
+    *host_count = 0;
+    if (nslots < 2) return 0;
+    const size_t nb = order_measure_ints(nslots);
+    hipLaunchKernelGGL(k_order_jumps, dim3((unsigned)nb), dim3(256), 0, s, parts, nslots, g, dev_partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    std::vector<unsigned> h(nb);
+    if ((e = hipMemcpyAsync(h.data(), dev_partial, sizeof(unsigned) * nb, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return (int)e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return (int)e;
+    for (unsigned v : h) *host_count += v;
+    return 0;
 }
 
 }  // namespace isf

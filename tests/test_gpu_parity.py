@@ -163,6 +163,19 @@ def test_add_particles_matches_oracle(isf_mod):
     assert np.max(np.abs(got - want)) < 2e-6
 
 
+def test_fill_nozzle_matches_oracle(isf_mod):
+    isf = isf_mod.ISF((8, 4, 4), (256, 128, 128), 0.1, 1 / 48)
+    cen = np.float32([0.3, 1.966, 2.066])
+    isf.jet_configure(np.float32([10, 0, 0]), np.float32(5), cen, np.float32(0.5), np.float32(0.5), 0, 7)
+    parts = np.full((300_000, 3), -1.0, np.float32)
+    dp = isf_mod.DeviceArray.from_host(parts)
+    isf.fill_nozzle_(dp, 1000, 250_000, 11)
+    got = dp.to_host()
+    want = O32.fill_cylinder(parts, 1000, 250_000, cen, np.float32(0.5), np.float32(0.5), 11)
+    assert np.array_equal(got[:1000], parts[:1000]) and np.array_equal(got[251_000:], parts[251_000:])
+    assert np.max(np.abs(got - want)) < 2e-6
+
+
 @pytest.mark.parametrize("n,L", [((64, 32, 32), (4, 2, 2)), ((256, 128, 128), (8, 4, 4))])
 def test_staggered_advect(isf_mod, n, L):
     isf = isf_mod.ISF(L, n, 0.1, 1 / 48)

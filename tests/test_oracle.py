@@ -284,6 +284,26 @@ def test_add_particles_ring_bounds_and_wrap():
     assert np.all(np.hypot(cyl[:, 1] - cen[1], cyl[:, 2] - cen[2]) <= 0.5 + 1e-12)
 
 
+def test_fill_cylinder_is_an_ordered_permutation_of_the_draws():
+    cen = np.array([0.3, 0.966, 1.066])
+    n = 4000
+    parts = O64.fill_cylinder(np.zeros((n, 3)), 0, n, cen, 0.5, 0.5, seed=3)
+    u = np.array([philox_uniform4(i, 0xFFFFFFFF, 1, 3) for i in range(n)], np.float64)
+    r = 0.5 * np.sqrt(u[:, 1])
+    th = u[:, 2].astype(np.float32).astype(np.float64) * 2 * np.pi
+    draws = np.stack([cen[0] + 0.5 * (u[:, 0] - 0.5), cen[1] + r * np.cos(th), cen[2] + r * np.sin(th)], -1)
+    a = parts[np.lexsort(parts.T)]
+    b = draws[np.lexsort(draws.T)]
+    assert np.max(np.abs(a - b)) < 1e-6
+    # slot order = increasing (x slab, theta wedge, r) key of the raw Philox words
+    raw = np.array([philox4x32_10([i, 0xFFFFFFFF, 1, 0], [3, 0]) for i in range(n)], np.uint64)
+    key = ((raw[:, 0] >> 27) << 19) | ((raw[:, 2] >> 25) << 12) | (raw[:, 1] >> 20)
+    order = np.argsort(key, kind="stable")
+    assert np.max(np.abs(parts - draws[order])) < 1e-6
+    assert np.all(np.abs(parts[:, 0] - cen[0]) <= 0.25)
+    assert np.all(np.hypot(parts[:, 1] - cen[1], parts[:, 2] - cen[2]) <= 0.5 + 1e-12)
+
+
 def test_advect_constant_field_out_of_box_and_matlabport():
     n, L = (16, 8, 8), (4, 2, 2)
     d = np.array([L[a] / n[a] for a in range(3)])
