@@ -1,0 +1,159 @@
+# SchroedingersSmokeAMD.jl -- Julia binding of libisf (include/isf.h), the MI355X (gfx950)
+# implementation of the SchroedingersSmoke.jl ISF step.
+#
+# It keeps the names and call forms of the reference's exports (src/parallel.jl:264-266,
+# src/particles.jl:97) so the driver loop of test/runtests.jl:40-66 runs unchanged apart
+# from `ArrayType = ISFArray`.  Every call goes through `ccall` into libisf; device arrays
+# are `ISFArray`s (device pointer + shape, freed by a finalizer).
+#
+# Julia is not installed in the build container, so this file is not exercised by the
+# test suite; the same C ABI is exercised through the Python mirror (isf_amd).
+module SchroedingersSmokeAMD
+
+export ISF, ISFArray, normalize_psi!, pressure_project!, velocity_one_form!, schroedinger_flow!
+export staggered_advect!, add_particles!, restrict_velocity!, isjet, synchronize
+
+const libisf = joinpath(@__DIR__, "..", "lib", "libisf.so")
+
+function check(rc::Cint, what::AbstractString)
+    if rc != 0
+        msg = unsafe_string(ccall((:isf_last_error, libisf), Cstring, ()))
+        error("$what: libisf error $rc: $msg")
+    end
+    nothing
+end
+
+# ---------------------------------------------------------------- device arrays
+"""
+    ISFArray{T,N}
+
+Device array in the reference's column-major layout (x fastest), the counterpart of
+`ArrayType(hostarray)` / `Array(devarray)` (test/runtests.jl:19,33, examples/jet.jl:83).
+"""
+mutable struct ISFArray{T,N} <: AbstractArray{T,N}
+    ptr::Ptr{Cvoid}
+    dims::NTuple{N,Int}
+    function ISFArray{T,N}(dims::NTuple{N,Int}) where {T,N}
+        p = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:isf_malloc, libisf), Cint, (Ref{Ptr{Cvoid}}, Csize_t), p, prod(dims) * sizeof(T)), "isf_malloc")
+        a = new{T,N}(p[], dims)
+        finalizer(x -> ccall((:isf_free, libisf), Cint, (Ptr{Cvoid},), x.ptr), a)
+        a
+    end
+end
+Base.size(a::ISFArray) = a.dims
+Base.getindex(::ISFArray, ::Int...) = error("scalar indexing of an ISFArray; copy it to the host with Array(a)")
+function ISFArray(h::Array{T,N}) where {T,N}
+    a = ISFArray{T,N}(size(h))
+    check(ccall((:isf_memcpy_h2d, libisf), Cint, (Ptr{Cvoid}, Ptr{T}, Csize_t), a.ptr, h, sizeof(h)), "isf_memcpy_h2d")
+    a
+end
+function Base.Array(a::ISFArray{T,N}) where {T,N}
+    h = Array{T,N}(undef, a.dims)
+    check(ccall((:isf_memcpy_d2h, libisf), Cint, (Ptr{T}, Ptr{Cvoid}, Csize_t), h, a.ptr, sizeof(h)), "isf_memcpy_d2h")
+    h
+end
+devptr(a::ISFArray) = a.ptr
+devptr(::Nothing) = C_NULL
+
+# ---------------------------------------------------------------- ISF
+"""
+    ISF{ISFArray, UInt32, Float32}(physical_size, dims, hbar, dt)
+
+The reference constructor (src/parallel.jl:41-98).  Fields read by the drivers:
+`d`, `dt`, `hbar`, `grid_res`, `velocity` (test/runtests.jl:25,44,59-63).
+"""
+mutable struct ISF{A,I,F}
+    h::Ptr{Cvoid}
+    grid_res::NTuple{3,Int}
+    physical_size::NTuple{3,Int}
+    d::NTuple{3,Float32}
+    hbar::Float32
+    dt::Float32
+    function ISF{A,I,F}(physical_size, dims, hbar, dt) where {A,I,F}
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        res = Cint.(collect(dims))
+        box = Cint.(collect(physical_size))
+        check(ccall((:isf_create, libisf), Cint, (Ref{Ptr{Cvoid}}, Ptr{Cint}, Ptr{Cint}, Cfloat, Cfloat),
+                    h, res, box, Float32(hbar), Float32(dt)), "isf_create")
+        d = Float32.(Tuple(collect(physical_size) ./ collect(dims)))
+        isf = new{A,I,F}(h[], Tuple(dims), Tuple(physical_size), d, Float32(hbar), Float32(dt))
+        finalizer(x -> ccall((:isf_destroy, libisf), Cint, (Ptr{Cvoid},), x.h), isf)
+        isf
+    end
+end
+
+function Base.getproperty(isf::ISF, s::Symbol)
+    if s === :velocity
+        p = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:isf_velocity_ptr, libisf), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), getfield(isf, :h), p),
+              "isf_velocity_ptr")
+        return p[]  # device pointer of the NTuple{3,Float32} field (borrowed)
+    end
+    getfield(isf, s)
+end
+
+synchronize(isf::ISF) = check(ccall((:isf_synchronize, libisf), Cint, (Ptr{Cvoid},), isf.h), "isf_synchronize")
+
+# ---------------------------------------------------------------- reference operators
+# psi is an ISFArray{NTuple{2,ComplexF32},3}; particles an ISFArray{NTuple{3,Float32},1}.
+schroedinger_flow!(isf::ISF, psi) =
+    (check(ccall((:isf_schroedinger_flow, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), isf.h, devptr(psi)),
+           "schroedinger_flow!"); psi)
+
+"`psi .= normalize_psi.(psi)` (src/parallel.jl:162-165, test/runtests.jl:49)."
+normalize_psi!(isf::ISF, psi) =
+    (check(ccall((:isf_normalize, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), isf.h, devptr(psi)), "normalize_psi"); psi)
+
+pressure_project!(isf::ISF, psi) =
+    (check(ccall((:isf_pressure_project, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), isf.h, devptr(psi)),
+           "pressure_project!"); psi)
+
+"`velocity_one_form!(isf, psi, hbar)` writes isf.velocity; `sharp=true` also applies the driver's
+in-place StaggeredSharp (test/runtests.jl:61-62)."
+velocity_one_form!(isf::ISF, psi, hbar = 1f0; sharp::Bool = false) =
+    check(ccall((:isf_velocity_one_form, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cfloat, Cint, Ptr{Cvoid}),
+                isf.h, devptr(psi), Float32(hbar), Cint(sharp), C_NULL), "velocity_one_form!")
+
+"`isjet.(isf.positions, cen, len, rad)` (src/particles.jl:6-10, test/runtests.jl:25)."
+function isjet(isf::ISF, nozzle_cen, nozzle_len, nozzle_rad)
+    out = ISFArray{UInt8,3}(isf.grid_res)
+    check(ccall((:isf_isjet, libisf), Cint, (Ptr{Cvoid}, Ptr{Cfloat}, Cfloat, Cfloat, Ptr{Cvoid}),
+                isf.h, Float32.(collect(nozzle_cen)), Float32(nozzle_len), Float32(nozzle_rad), out.ptr), "isjet")
+    out
+end
+
+"`restrict_velocity!(isf, psi, kvec, isjet, omgterm)` (src/particles.jl:25-27); `phase_mode` 1 reproduces
+the reference's `tuple_dot` sum, 0 (default) the k.x phase of the MatlabPort driver."
+restrict_velocity!(isf::ISF, psi, kvec, isjetarr, omgterm = 0f0; phase_mode::Integer = 0) =
+    (check(ccall((:isf_restrict_velocity, libisf), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cfloat}, Cfloat, Cint),
+                 isf.h, devptr(psi), devptr(isjetarr), Float32.(collect(kvec)), Float32(omgterm), Cint(phase_mode)),
+           "restrict_velocity!"); psi)
+
+"`add_particles!(particles, range, cen, rad)` (src/particles.jl:34-94) with a seeded draw."
+add_particles!(isf::ISF, particles::ISFArray, range::UnitRange, nozzle_cen, nozzle_rad; seed = 0, step = 0) =
+    check(ccall((:isf_add_particles, libisf), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Clonglong, Clonglong, Clonglong, Ptr{Cfloat}, Cfloat, Culonglong, Cuint),
+                isf.h, particles.ptr, length(particles), first(range) - 1, length(range),
+                Float32.(collect(nozzle_cen)), Float32(nozzle_rad), UInt64(seed), UInt32(step)), "add_particles!")
+
+"`staggered_advect!(particles, isf)` (src/parallel.jl:248-261), using isf.velocity."
+staggered_advect!(particles::ISFArray, isf::ISF) =
+    (check(ccall((:isf_staggered_advect, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Clonglong, Ptr{Cvoid}),
+                 isf.h, particles.ptr, length(particles), C_NULL), "staggered_advect!"); particles)
+
+# ---------------------------------------------------------------- fused driver step
+"Configure the jet of the driver (test/runtests.jl:7-12,23-25) for `step!`."
+jet_configure!(isf::ISF, kvec, omega, nozzle_cen, nozzle_len, nozzle_rad, n_spawn; seed = 0, phase_mode = 0) =
+    check(ccall((:isf_jet_configure, libisf), Cint,
+                (Ptr{Cvoid}, Ptr{Cfloat}, Cfloat, Ptr{Cfloat}, Cfloat, Cfloat, Clonglong, Culonglong, Cint),
+                isf.h, Float32.(collect(kvec)), Float32(omega), Float32.(collect(nozzle_cen)), Float32(nozzle_len),
+                Float32(nozzle_rad), n_spawn, UInt64(seed), Cint(phase_mode)), "isf_jet_configure")
+
+"One iteration of the simloop body (test/runtests.jl:45-64)."
+step!(isf::ISF, psi, particles::ISFArray, iter::Integer) =
+    check(ccall((:isf_step, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Clonglong, Cint),
+                isf.h, devptr(psi), particles.ptr, length(particles), Cint(iter)), "isf_step")
+
+end # module

@@ -281,7 +281,28 @@ def test_add_particles_ring_bounds_and_wrap():
     assert np.max(np.abs(out32 - out)) < 1e-6
     cyl = O64.fill_cylinder(np.zeros((5000, 3)), 0, 5000, cen, 0.5, 0.5, seed=1)
     assert np.all(np.abs(cyl[:, 0] - cen[0]) <= 0.25)
-    assert np.all(np.hypot(cyl[:, 1] - cen[1], cyl[:, 2] - cen[2]) <= 0.5 + 1e-12)
+    rr = np.hypot(cyl[:, 1] - cen[1], cyl[:, 2] - cen[2])
+    assert np.all(rr <= 0.5 + 1e-12)
+    # uniform in the disk: P(r < 0.25) = 1/4, P(r < 0.5 / sqrt 2) = 1/2
+    assert abs(np.mean(rr < 0.25) - 0.25) < 0.03 and abs(np.mean(rr < 0.5 / np.sqrt(2)) - 0.5) < 0.03
+
+
+def test_spawn_ring_orientation_from_the_draws():
+    """src/particles.jl:37-49: angle rt = 2 pi u0 measured from +y towards +z, x = c_x - 0.1,
+    jitter (u1,u2,u3)*0.1 - 0.05.  The draws come from the Philox KAT-pinned generator."""
+    cen = np.array([0.3, 0.966, 1.066])
+    n, step, seed = 1000, 5, 9
+    out = O64.add_particles(np.full((n, 3), -1.0), 0, n, cen, 0.5, seed, step)
+    u0 = np.array([philox_uniform4(j, step, 0, seed)[0] for j in range(n)], np.float64)
+    ang = np.sort(np.mod(np.arctan2(out[:, 2] - cen[2], out[:, 1] - cen[1]), 2 * np.pi))
+    # the set of ring angles is the set of 2 pi u0 up to the jitter (|jitter| <= 0.05*sqrt2 / 0.4)
+    want = np.sort(2 * np.pi * u0)
+    assert np.mean(np.abs(ang - want)) < 0.05
+    assert np.all(np.abs(out[:, 0] - (cen[0] - 0.1)) <= 0.05 + 1e-12)
+    assert abs(np.mean(out[:, 0]) - (cen[0] - 0.1)) < 0.005
+    # slots are in increasing angle order within the batch (ring order)
+    a_slot = np.mod(np.arctan2(out[:, 2] - cen[2], out[:, 1] - cen[1]), 2 * np.pi)
+    assert np.mean(np.diff(a_slot) >= -0.3) > 0.95
 
 
 def test_fill_cylinder_is_an_ordered_permutation_of_the_draws():
