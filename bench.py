@@ -151,8 +151,8 @@ def main():
             acc.setdefault(k, []).append(v)
     stage_ms = {k: float(np.mean(v)) for k, v in acc.items()}
     sb = stage_bytes(nx, ny, nz, w["n_slots"], w["n_spawn"])
-    dom = max(stage_ms, key=stage_ms.get)
-    achieved = sb[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    dom = max(stage_ms, key=stage_ms.get) if stage_ms else "flow_plane"
+    achieved = sb[dom] / (stage_ms[dom] * 1e-3) / 1e9 if stage_ms else 0.0
     stages = {k: {"ms": round(stage_ms[k], 5), "GB": round(sb[k] / 1e9, 4),
                   "GB_s": round(sb[k] / max(stage_ms[k] * 1e-3, 1e-9) / 1e9, 1)} for k in stage_ms}
     balg = b_alg_step(N, w["n_slots"])

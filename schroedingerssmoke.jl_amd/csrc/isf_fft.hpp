@@ -136,10 +136,28 @@ struct LayStrided {
     __device__ __forceinline__ void st(int e, float2 v) const { p[e * s] = v; }
 };
 
+// ------------------------------------------------------------------ twiddle tables
+// Pass (N, R, NS) needs w(kk, r) = exp(-2 pi i r kk / (NS R)) for kk < NS, r < R.  It is
+// kept in LDS as a [kk][r] table with row pitch R + 1: lanes of one wavefront read
+// different kk with the same r, and the odd pitch puts them in different banks (reading
+// tw_N[r kk N/(NS R)] directly strides by a multiple of the bank count).
+template <int N, int R, int NS>
+struct PassTw {
+    static constexpr int SIZE = NS > 1 ? NS * (R + 1) : 0;
+    // fill from the global table of N-th roots (tw_n = tw_all + N); caller synchronises
+    __device__ __forceinline__ static void fill(float2* t, const float2* __restrict__ tw_n, int tid, int nt) {
+        for (int i = tid; i < SIZE; i += nt) {
+            const int kk = i / (R + 1), r = i % (R + 1);
+            t[i] = r < R ? tw_n[(r * kk * (N / (NS * R))) % N] : make_float2(1.0f, 0.0f);
+        }
+    }
+    __device__ __forceinline__ static float2 get(const float2* t, int kk, int r) { return t[kk * (R + 1) + r]; }
+};
+
 // ------------------------------------------------------------------ Stockham passes
 // One radix-R pass over a line of N held in LDS by T = N/E threads (thread t of the
 // line).  All threads of the block call it (it contains block barriers); threads with
-// active == false only take part in the barriers.
+// active == false only take part in the barriers.  tw is this pass's PassTw table.
 template <class Lay, int N, int R, int NS, int E, int DIR>
 __device__ __forceinline__ void stockham_pass(const Lay& lay, int t, bool active, const float2* tw) {
     constexpr int T = N / E;
@@ -155,7 +173,7 @@ __device__ __forceinline__ void stockham_pass(const Lay& lay, int t, bool active
             if (NS > 1) {
                 const int kk = j % NS;
 #pragma unroll
-                for (int r = 1; r < R; r++) v[b][r] = twmul<DIR>(v[b][r], tw[r * kk * (N / (NS * R))]);
+                for (int r = 1; r < R; r++) v[b][r] = twmul<DIR>(v[b][r], PassTw<N, R, NS>::get(tw, kk, r));
             }
             Dft<R, DIR>::run(v[b]);
         }
@@ -174,25 +192,46 @@ __device__ __forceinline__ void stockham_pass(const Lay& lay, int t, bool active
     __syncthreads();
 }
 
-template <class Lay, int N, int E, int NS, int DIR>
-struct Passes {
+// The passes of a length-N transform with E elements per thread: radix
+// R = min(E, N/NS) at every step.  Their twiddle tables lie back to back.
+template <int N, int E, int NS>
+struct Sched {
+    static constexpr int REM = N / NS;
+    static constexpr int R = E < REM ? E : REM;
+    static constexpr int TW = PassTw<N, R, NS>::SIZE + Sched<N, E, NS * R>::TW;
+    __device__ __forceinline__ static void fill(float2* t, const float2* __restrict__ tw_n, int tid, int nt) {
+        PassTw<N, R, NS>::fill(t, tw_n, tid, nt);
+        Sched<N, E, NS * R>::fill(t + PassTw<N, R, NS>::SIZE, tw_n, tid, nt);
+    }
+    template <class Lay, int DIR>
     __device__ __forceinline__ static void run(const Lay& lay, int t, bool active, const float2* tw) {
-        constexpr int REM = N / NS;
-        constexpr int R = E < REM ? E : REM;
         stockham_pass<Lay, N, R, NS, E, DIR>(lay, t, active, tw);
-        Passes<Lay, N, E, NS * R, DIR>::run(lay, t, active, tw);
+        Sched<N, E, NS * R>::template run<Lay, DIR>(lay, t, active, tw + PassTw<N, R, NS>::SIZE);
     }
 };
-template <class Lay, int N, int E, int DIR>
-struct Passes<Lay, N, E, N, DIR> {
+template <int N, int E>
+struct Sched<N, E, N> {
+    static constexpr int TW = 0;
+    __device__ __forceinline__ static void fill(float2*, const float2* __restrict__, int, int) {}
+    template <class Lay, int DIR>
     __device__ __forceinline__ static void run(const Lay&, int, bool, const float2*) {}
 };
 
+// Twiddle tables of the whole length-N schedule (SIZE entries, at least 1).
+template <int N, int E>
+struct FftTw {
+    static constexpr int SIZE = Sched<N, E, 1>::TW > 0 ? Sched<N, E, 1>::TW : 1;
+    // tw_all: combined table of roots (length-N roots at tw_all + N); caller synchronises
+    __device__ __forceinline__ static void fill(float2* t, const float2* __restrict__ tw_all, int tid, int nt) {
+        Sched<N, E, 1>::fill(t, tw_all + N, tid, nt);
+    }
+};
+
 // Full length-N transform of one LDS line (DIR = -1 forward, +1 inverse, unnormalised).
-// tw is the length-N table exp(-2 pi i m / N), m < N (normally a copy in LDS).
+// tw holds the schedule's tables (FftTw<N, E>).
 template <int N, int E, int DIR, class Lay>
 __device__ __forceinline__ void fft_line(const Lay& lay, int t, bool active, const float2* tw) {
-    Passes<Lay, N, E, 1, DIR>::run(lay, t, active, tw);
+    Sched<N, E, 1>::template run<Lay, DIR>(lay, t, active, tw);
 }
 
 // Copy the length-N twiddle table (at tw_all + N) into LDS; the caller synchronises.
@@ -207,7 +246,8 @@ __device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restr
 // j = t*NB + b (b < NB, blocked so that a thread's elements are adjacent in the first
 // and last passes); butterfly b uses v[b*R .. b*R+R-1], reads elements in(t,b,r) and
 // writes out(t,b,r).  When pass B reads exactly the elements pass A wrote (same t, b, r
-// -> same index), the LDS round trip between them is skipped.
+// -> same index), the LDS round trip between them is skipped.  compute() takes the pass's
+// PassTw<N, R, NS> table.
 template <int N, int R, int NS, int E>
 struct RPass {
     static constexpr int T = N / E;
@@ -225,7 +265,7 @@ struct RPass {
             if (NS > 1) {
                 const int kk = (t * NB + b) % NS;
 #pragma unroll
-                for (int r = 1; r < R; r++) v[b * R + r] = twmul<DIR>(v[b * R + r], tw[r * kk * (N / (NS * R))]);
+                for (int r = 1; r < R; r++) v[b * R + r] = twmul<DIR>(v[b * R + r], PassTw<N, R, NS>::get(tw, kk, r));
             }
             Dft<R, DIR>::run(v + b * R);
         }

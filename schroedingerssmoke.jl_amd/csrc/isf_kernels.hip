@@ -25,7 +25,30 @@ namespace isf {
 
 // ======================================================================== helpers
 
-__device__ __forceinline__ float angle_of(float re, float im) { return atan2f(im, re); }
+// angle(re + i im) = atan2(im, re) (Julia's angle, src/parallel.jl:108).  Range reduction
+// to a = min/max in [0, 1], atan(a) = a P(a^2) with a degree-8 polynomial fitted for this
+// project (weighted least squares driven to equal ripple; |relative error| <= 1.5e-7 on
+// [0, 1] in FP32 Horner evaluation), then the octant is restored.  angle(0) = 0 and the
+// sign of a zero imaginary part is kept, as atan2 does.
+__device__ __forceinline__ float angle_of(float re, float im) {
+    const float ax = fabsf(re), ay = fabsf(im);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float a = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+    const float s = a * a;
+    float p = 2.903499175e-03f;
+    p = fmaf(p, s, -1.628280245e-02f);
+    p = fmaf(p, s, 4.303904623e-02f);
+    p = fmaf(p, s, -7.533650100e-02f);
+    p = fmaf(p, s, 1.065466627e-01f);
+    p = fmaf(p, s, -1.420713067e-01f);
+    p = fmaf(p, s, 1.999305338e-01f);
+    p = fmaf(p, s, -3.333309293e-01f);
+    p = fmaf(p, s, 1.0f);
+    float r = p * a;
+    if (ay > ax) r = 1.57079632679489661923f - r;
+    if (__float_as_uint(re) >> 31) r = 3.14159265358979323846f - r;  // re < 0 or re = -0
+    return copysignf(r, im);
+}
 
 // angle(conj(a1) b1 + conj(a2) b2) for psi cells a, b (src/parallel.jl:108-111)
 __device__ __forceinline__ float edge_angle(float4 a, float4 b) {
@@ -98,10 +121,10 @@ k_flow_x_fwd(const float4* __restrict__ psi, float2* __restrict__ S, int ny, int
     constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
     constexpr int NT = 2 * TY * T;
     __shared__ float2 lds[2 * TY * LS];
-    __shared__ float2 twl[NX];
+    __shared__ float2 twl[FftTw<NX, E>::SIZE];
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * TY, z = blockIdx.y;
-    load_twiddles<NX, NT>(twl, tw);
+    FftTw<NX, E>::fill(twl, tw, tid, NT);
     const float4* src = psi + ((size_t)z * ny + y0) * NX;
     for (int i = tid; i < TY * NX; i += NT) {
         const int l = i / NX, x = i % NX;
@@ -126,10 +149,10 @@ k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, int ny, int
     constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
     constexpr int NT = 2 * TY * T;
     __shared__ float2 lds[2 * TY * LS];
-    __shared__ float2 twl[NX];
+    __shared__ float2 twl[FftTw<NX, E>::SIZE];
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * TY, z = blockIdx.y;
-    load_twiddles<NX, NT>(twl, tw);
+    FftTw<NX, E>::fill(twl, tw, tid, NT);
     for (int i = tid; i < 2 * NX * TY; i += NT) {
         const int ly = i % TY, kx = (i / TY) % NX, c = i / (TY * NX);
         lds[(c * TY + ly) * LS + pad16(kx)] = S[(((size_t)c * NX + kx) * nz + z) * ny + y0 + ly];
@@ -190,13 +213,19 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
     using C = PlaneCfg<NY, NZ>;
     constexpr int E = C::E, NT = C::NT, LSY = C::LSY;
     constexpr int RY2 = C::RY2, RZ2 = C::RZ2;
+    using TY2 = PassTw<NY, RY2, 16>;   // forward y pass 2
+    using TY2I = PassTw<NY, 16, RY2>;  // inverse y pass 2
+    using TZ2 = PassTw<NZ, RZ2, 16>;
+    using TZ2I = PassTw<NZ, 16, RZ2>;
     __shared__ float2 lds[NZ * LSY];
-    __shared__ float2 twy[NY];
-    __shared__ float2 twz[NZ];
+    __shared__ float2 twy[TY2::SIZE + TY2I::SIZE + 1];
+    __shared__ float2 twz[TZ2::SIZE + TZ2I::SIZE + 1];
     __shared__ double mtab[NY + NZ];  // FLOW: float2 mask_y, mask_z ; POISSON: s2y, s2z
     const int tid = threadIdx.x;
-    load_twiddles<NY, NT>(twy, tb.tw);
-    load_twiddles<NZ, NT>(twz, tb.tw);
+    TY2::fill(twy, tb.tw + NY, tid, NT);
+    TY2I::fill(twy + TY2::SIZE, tb.tw + NY, tid, NT);
+    TZ2::fill(twz, tb.tw + NZ, tid, NT);
+    TZ2I::fill(twz + TZ2::SIZE, tb.tw + NZ, tid, NT);
     float2* mt2 = reinterpret_cast<float2*>(mtab);
     for (int i = tid; i < NY + NZ; i += NT) {
         if (MODE == PLANE_FLOW) mt2[i] = i < NY ? tb.mask_y[i] : tb.mask_z[i - NY];
@@ -224,7 +253,7 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
             Y1::load(v, LayGlobal{P + zl * NY}, ty);
         }
 
-        // ---- forward y
+        // ---- forward y (the first pass of each direction has no twiddles)
         Y1::template compute<-1>(v, ty, twy);
         __syncthreads();  // the previous plane's LDS reads are done, tables loaded
         Y1::store(v, ly, ty);
@@ -232,7 +261,7 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
         if constexpr (NY > 16) {
             using Y2 = RPass<NY, RY2, 16, E>;
             Y2::load(v, ly, ty);
-            Y2::template compute<-1>(v, ty, twy);
+            Y2::template compute<-1>(v, ty, twy);  // TY2 table
             __syncthreads();
             Y2::store(v, ly, ty);
             __syncthreads();
@@ -247,7 +276,7 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
             Z1::store(v, lz, tz);
             __syncthreads();
             ZL::load(v, lz, tz);
-            ZL::template compute<-1>(v, tz, twz);
+            ZL::template compute<-1>(v, tz, twz);  // TZ2 table
         }
         constexpr int RL = E / ZL::NB;  // radix of the last forward z pass
         if (MODE == PLANE_FLOW) {
@@ -282,7 +311,7 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
         if constexpr (NZ > 16) {
             using Z2I = RPass<NZ, 16, RZ2, E>;
             Z2I::load(v, lz, tz);
-            Z2I::template compute<1>(v, tz, twz);
+            Z2I::template compute<1>(v, tz, twz + TZ2::SIZE);
             __syncthreads();
             Z2I::store(v, lz, tz);
             __syncthreads();
@@ -297,7 +326,7 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
             Y1I::store(v, ly, ty);
             __syncthreads();
             Y2I::load(v, ly, ty);
-            Y2I::template compute<1>(v, ty, twy);
+            Y2I::template compute<1>(v, ty, twy + TY2::SIZE);
             Y2I::store(v, LayGlobal{P + zl * NY}, ty);
         } else {
             Y1::load(v, ly, ty);
@@ -319,11 +348,11 @@ k_r2c_x(const float* __restrict__ f, float2* __restrict__ P, int ny, int nz, con
     constexpr int T = XCfg<M>::T, E = XCfg<M>::E, LS = XCfg<M>::LS;
     constexpr int NT = TY * T;
     __shared__ float2 lds[TY * LS];
-    __shared__ float2 twm[M];
+    __shared__ float2 twm[FftTw<M, E>::SIZE];
     __shared__ float2 twn[NX];
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * TY, z = blockIdx.y;
-    load_twiddles<M, NT>(twm, tw);
+    FftTw<M, E>::fill(twm, tw, tid, NT);
     load_twiddles<NX, NT>(twn, tw);
     const float2* src = reinterpret_cast<const float2*>(f + ((size_t)z * ny + y0) * NX);
     for (int i = tid; i < TY * M; i += NT) lds[(i / M) * LS + pad16(i % M)] = src[i];
@@ -355,12 +384,12 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
     constexpr int LS = padded_len(M + 1);
     constexpr int NT = TY * T;
     __shared__ float2 lds[TY * LS];
-    __shared__ float2 twm[M];
+    __shared__ float2 twm[FftTw<M, E>::SIZE];
     __shared__ float2 twn[NX];
     const int tid = threadIdx.x;
     const int ny = g.ny, nz = g.nz;
     const int y0 = blockIdx.x * TY, z = blockIdx.y;
-    load_twiddles<M, NT>(twm, tw);
+    FftTw<M, E>::fill(twm, tw, tid, NT);
     load_twiddles<NX, NT>(twn, tw);
     for (int i = tid; i < (M + 1) * TY; i += NT) {
         const int ly = i % TY, k = i / TY;
@@ -444,15 +473,24 @@ __global__ void __launch_bounds__(BX * ST_BY) k_div_psi(const float4* __restrict
     const int yp = y + 1 < ny ? y + 1 : 0, ym = y > 0 ? y - 1 : ny - 1;
     const float ds0 = 1.0f / (g.dx * g.dx), ds1 = 1.0f / (g.dy * g.dy), ds2 = 1.0f / (g.dz * g.dz);
     const int zb = z0 > 0 ? z0 - 1 : nz - 1;
-    float4 pc = psi[x + sy * y + sz * z0];
-    float vzm = edge_angle(psi[x + sy * y + sz * zb], pc);  // v_z(c - e_z) for the first z
+    const size_t cx = x + sy * y, cyp = x + sy * yp;
+    float4 pc = psi[cx + sz * z0];
+    float vzm = edge_angle(psi[cx + sz * zb], pc);  // v_z(c - e_z) for the first z
+    // loads of the next z step are issued one step ahead
+    int zp = z0 + 1 < nz ? z0 + 1 : 0;
+    float4 pz = psi[cx + sz * zp];
+    float4 py = psi[cyp + sz * z0];
 #pragma unroll 1
     for (int k = 0; k < ST_TZ; k++) {
         const int z = z0 + k;
-        const int zp = z + 1 < nz ? z + 1 : 0;
         const size_t row = sy * y + sz * z;
-        const float4 pz = psi[x + sy * y + sz * zp];
-        const float4 py = psi[x + sy * yp + sz * z];
+        float4 pz_n = pz, py_n = py;
+        if (k + 1 < ST_TZ) {
+            const int zn = zp + 1 < nz ? zp + 1 : 0;
+            pz_n = psi[cx + sz * zn];
+            py_n = psi[cyp + sz * zp];
+            zp = zn;
+        }
         float4 px = shfl4(pc, tx + 1 < BX ? lane + 1 : lane);
         if (tx + 1 == BX) px = psi[xp + row];
         const float vx = edge_angle(pc, px), vy = edge_angle(pc, py), vz = edge_angle(pc, pz);
@@ -464,6 +502,8 @@ __global__ void __launch_bounds__(BX * ST_BY) k_div_psi(const float4* __restrict
         f[x + row] = (vx - vxm) * ds0 + (vy - vym) * ds1 + (vz - vzm) * ds2;
         vzm = vz;
         pc = pz;
+        pz = pz_n;
+        py = py_n;
     }
 }
 
@@ -479,14 +519,22 @@ k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, 
     const size_t sy = nx, sz = (size_t)nx * ny;
     const int xp = x + 1 < nx ? x + 1 : 0, yp = y + 1 < ny ? y + 1 : 0;
     const float s0 = sharp ? 1.0f / g.dx : 1.0f, s1 = sharp ? 1.0f / g.dy : 1.0f, s2 = sharp ? 1.0f / g.dz : 1.0f;
-    float4 pc = psi[x + sy * y + sz * z0];
+    const size_t cx = x + sy * y, cyp = x + sy * yp;
+    float4 pc = psi[cx + sz * z0];
+    int zp = z0 + 1 < nz ? z0 + 1 : 0;
+    float4 pz = psi[cx + sz * zp];
+    float4 py = psi[cyp + sz * z0];
 #pragma unroll 1
     for (int k = 0; k < ST_TZ; k++) {
         const int z = z0 + k;
-        const int zp = z + 1 < nz ? z + 1 : 0;
-        const size_t c = x + sy * y + sz * z;
-        const float4 pz = psi[x + sy * y + sz * zp];
-        const float4 py = psi[x + sy * yp + sz * z];
+        const size_t c = cx + sz * z;
+        float4 pz_n = pz, py_n = py;
+        if (k + 1 < ST_TZ) {
+            const int zn = zp + 1 < nz ? zp + 1 : 0;
+            pz_n = psi[cx + sz * zn];
+            py_n = psi[cyp + sz * zp];
+            zp = zn;
+        }
         float4 px = shfl4(pc, tx + 1 < BX ? lane + 1 : lane);
         if (tx + 1 == BX) px = psi[xp + sy * y + sz * z];
         float v0 = edge_angle(pc, px) * hbar, v1 = edge_angle(pc, py) * hbar, v2 = edge_angle(pc, pz) * hbar;
@@ -499,6 +547,8 @@ k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, 
         vel[3 * c + 1] = v1;
         vel[3 * c + 2] = v2;
         pc = pz;
+        pz = pz_n;
+        py = py_n;
     }
 }
 
