@@ -128,6 +128,44 @@ int isf_norm2(isf_t h, const float* psi, double* out);
 /* The same sum accumulated by the last isf_step (read after isf_synchronize). */
 int isf_step_norm2(isf_t h, double* out);
 
+/* ---- batched uneven ensembles (no reference counterpart) ------------------------------- */
+/* One member of an ensemble: the arguments of isf_create and isf_jet_configure. */
+typedef struct {
+    int res[3];
+    int box[3];
+    float hbar, dt;
+    float kvec[3];
+    float omega;
+    float cen[3];
+    float len, rad;
+    long long n_spawn;
+    unsigned long long seed;
+    int phase_mode;
+} isf_member_desc;
+typedef struct isf_batch* isf_batch_t;
+/* Create n independent ISF instances (grids may differ). */
+int isf_batch_create(isf_batch_t* out, int n, const isf_member_desc* members);
+int isf_batch_destroy(isf_batch_t b);
+/* The handle of member i (owned by the batch), for set-up and single operators. */
+int isf_batch_member(isf_batch_t b, int i, isf_t* out);
+/* Device buffers member i steps: psi, particles (nslots slots). */
+int isf_batch_bind(isf_batch_t b, int i, float* psi, float* parts, long long nslots);
+/* One step `iter` (1-based) of every member, the simloop body of each.  Members of the
+ * same grid shape share one launch per kernel stage; shape groups run as parallel
+ * branches of one hipGraph captured on first use and replayed each step (set_graph 0:
+ * launched directly).  Spawned particles are written in slot order. */
+int isf_batch_step(isf_batch_t b, int iter);
+int isf_batch_set_graph(isf_batch_t b, int use_graph);
+/* The same step as n separate isf_step calls, one member after another (baseline). */
+int isf_batch_step_sequential(isf_batch_t b, int iter);
+int isf_batch_synchronize(isf_batch_t b);
+/* Pick the x-pass tile (y lines per block: 8, 16, 32) of every grid shape of the batch by
+ * timing it on scratch data; results are read from / appended to the text file
+ * cache_path when it is not NULL. */
+int isf_batch_tune(isf_batch_t b, const char* cache_path);
+/* The tile chosen for a grid shape (0 when none was tuned). */
+int isf_batch_tile(isf_batch_t b, const int res[3], int* ty);
+
 /* ---- timing helpers (HIP events on the handle's stream) ----------------------------- */
 /* Number and names of the kernel stages of isf_step, in launch order. */
 int isf_step_stage_count(void);

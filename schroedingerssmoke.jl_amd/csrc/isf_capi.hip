@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -55,6 +56,11 @@ struct isf_handle {
     hipEvent_t ev0, ev1;
     hipEvent_t ev_vel, ev_adv;
     bool side_pending;    // an advect on pstream may still read vel / write particles
+    hipEvent_t ext_ev;    // recorded on the handle's stream when a batched step must wait for it
+    hipEvent_t batch_ev;  // (borrowed) end of the last batched step that used this handle's data
+    bool ext_pending;     // the handle's streams must wait for batch_ev before new work
+    unsigned long long seq;  // bumped by every call that enqueues work on the handle's streams
+    const void* owner;       // the batch that owns this handle (isf_destroy refuses it)
     void* table_mem;
     DevTables tb;
     float2* S;          // Schroedinger spectrum, 2N complex; f and P alias it in projections
@@ -95,8 +101,21 @@ static int check_handle(isf_t h) {
         int r_ = check_handle(h);               \
         if (r_) return r_;                      \
     } while (0)
+// Order the handle's streams after a batched step that used this handle's buffers.
+static int wait_ext(isf_t h) {
+    h->seq++;
+    if (h->ext_pending) {
+        hipError_t e = hipStreamWaitEvent(h->stream, h->batch_ev, 0);
+        if (e == hipSuccess) e = hipStreamWaitEvent(h->pstream, h->batch_ev, 0);
+        if (e != hipSuccess) return fail(ISF_EHIP, "hipStreamWaitEvent: %s", hipGetErrorString(e));
+        h->ext_pending = false;
+    }
+    return ISF_OK;
+}
 // Order the handle's stream after a pending particle-side advect of isf_step.
 static int join(isf_t h) {
+    int r = wait_ext(h);
+    if (r) return r;
     if (h->side_pending) {
         hipError_t e = hipStreamWaitEvent(h->stream, h->ev_adv, 0);
         if (e != hipSuccess) return fail(ISF_EHIP, "hipStreamWaitEvent: %s", hipGetErrorString(e));
@@ -272,7 +291,11 @@ int isf_create(isf_t* out, const int res[3], const int box[3], float hbar, float
     // ---- scratch
     if ((e = hipMalloc(&h->S, 16 * h->N)) != hipSuccess) return cleanup(fail(ISF_ENOMEM, "scratch: %s", hipGetErrorString(e)));
     if ((e = hipMalloc(&h->vel, 12 * h->N)) != hipSuccess) return cleanup(fail(ISF_ENOMEM, "velocity: %s", hipGetErrorString(e)));
-    h->norm_nparts = c2r_x_blocks(g) > 1024 ? c2r_x_blocks(g) : 1024;
+    {
+        // partials of the gauge pass for the smallest x tile (8 lines per block)
+        const size_t most = (size_t)(g.ny / (g.ny < 8 ? g.ny : 8)) * g.nz;
+        h->norm_nparts = most > 1024 ? most : 1024;
+    }
     if ((e = hipMalloc(&h->norm_parts, sizeof(double) * (h->norm_nparts + 1))) != hipSuccess)
         return cleanup(fail(ISF_ENOMEM, "norm partials: %s", hipGetErrorString(e)));
     h->norm_out = h->norm_parts + h->norm_nparts;
@@ -287,6 +310,8 @@ int isf_create(isf_t* out, const int res[3], const int box[3], float hbar, float
         return cleanup(hip_fail(e, "hipEventCreate"));
     if ((e = hipEventCreateWithFlags(&h->ev_adv, hipEventDisableTiming)) != hipSuccess)
         return cleanup(hip_fail(e, "hipEventCreate"));
+    if ((e = hipEventCreateWithFlags(&h->ext_ev, hipEventDisableTiming)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipEventCreate"));
     if ((e = hipEventCreate(&h->ev0)) != hipSuccess) return cleanup(hip_fail(e, "hipEventCreate"));
     if ((e = hipEventCreate(&h->ev1)) != hipSuccess) return cleanup(hip_fail(e, "hipEventCreate"));
     *out = h;
@@ -295,10 +320,12 @@ int isf_create(isf_t* out, const int res[3], const int box[3], float hbar, float
 
 int isf_destroy(isf_t h) {
     CHECK_H(h);
+    if (h->owner) return fail(ISF_EINVAL, "this handle belongs to a batch; destroy the batch instead");
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->pstream) (void)hipStreamSynchronize(h->pstream);
     if (h->ev_vel) (void)hipEventDestroy(h->ev_vel);
     if (h->ev_adv) (void)hipEventDestroy(h->ev_adv);
+    if (h->ext_ev) (void)hipEventDestroy(h->ext_ev);
     if (h->pstream) (void)hipStreamDestroy(h->pstream);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -548,6 +575,10 @@ static int step_impl(isf_t h, float* psi, float* parts, long long nslots, int it
     hipStream_t s = h->stream;
     const bool side = ev == nullptr && nslots > 0;
     hipStream_t ps = side ? h->pstream : s;
+    {
+        int r = wait_ext(h);
+        if (r) return r;
+    }
     int k = 0;
     auto mark = [&]() -> int {
         if (ev) {
@@ -753,6 +784,464 @@ int isf_timer_stop(isf_t h, float* ms) {
     HIP_CHECK(hipEventSynchronize(h->ev1), "hipEventSynchronize");
     HIP_CHECK(hipEventElapsedTime(ms, h->ev0, h->ev1), "hipEventElapsedTime");
     return ISF_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batched ensembles
+//
+// A batch owns one ISF handle per member (tables, scratch, velocity) and groups the
+// members by grid shape.  A batched step launches every kernel stage once per group
+// with the group's members in a grid dimension (each block reads its member's
+// descriptor); the groups run as parallel branches forked from and joined back into the
+// batch stream, captured once into a hipGraph and replayed per step.  The step number
+// lives in a device counter set before each replay, so the captured graph stays valid.
+
+static constexpr uint32_t kBatchMagic = 0xBA7C15F0u;
+
+struct BatchGroup {
+    GridInfo g;              // shape shared by the members; g.ty = x tile (0: default)
+    std::vector<int> idx;    // member indices
+    Inst* inst;              // device array of the members' descriptors
+    long long max_spawn;     // largest per-step spawn count of the members with slots
+    long long max_slots;     // largest particle buffer
+    hipStream_t stream;
+    hipEvent_t done;
+};
+
+struct isf_batch {
+    uint32_t magic;
+    std::vector<isf_t> mem;
+    std::vector<float*> psi, parts;
+    std::vector<long long> nslots;
+    std::vector<unsigned long long> seen_seq;  // member seq already ordered before the batch stream
+    std::vector<BatchGroup> groups;
+    hipStream_t stream;
+    hipEvent_t fork, done;
+    int* counter;            // device step counter read by the batched kernels
+    bool use_graph;
+    bool dirty;              // descriptors must be rebuilt (bind, tune)
+    hipGraphExec_t exec;
+};
+
+static int check_batch(isf_batch_t b) {
+    if (!b || b->magic != kBatchMagic) return fail(ISF_EHANDLE, "null or destroyed isf batch");
+    return ISF_OK;
+}
+#define CHECK_B(b)                     \
+    do {                               \
+        int r_ = check_batch(b);       \
+        if (r_) return r_;             \
+    } while (0)
+
+static void batch_free(isf_batch_t b) {
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    for (BatchGroup& gr : b->groups) {
+        if (gr.stream) (void)hipStreamSynchronize(gr.stream);
+        if (gr.inst) (void)hipFree(gr.inst);
+        if (gr.done) (void)hipEventDestroy(gr.done);
+        if (gr.stream) (void)hipStreamDestroy(gr.stream);
+    }
+    for (isf_t h : b->mem) {
+        h->owner = nullptr;
+        h->ext_pending = false;
+        h->batch_ev = nullptr;
+        (void)isf_destroy(h);
+    }
+    if (b->counter) (void)hipFree(b->counter);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    b->magic = 0;
+    delete b;
+}
+
+static Inst inst_of(isf_t h, float* psi, float* parts, long long nslots, int ty) {
+    Inst m{};
+    m.g = h->g;
+    m.g.ty = ty;
+    m.jet = h->jet;
+    m.tb = h->tb;
+    m.psi = reinterpret_cast<float4*>(psi);
+    m.S = h->S;
+    m.vel = h->vel;
+    m.parts = parts;
+    m.nslots = parts ? nslots : 0;
+    m.n_spawn = h->n_spawn;
+    m.seed = h->seed;
+    m.norm_parts = h->norm_parts;
+    m.hbar = h->hbar;
+    m.dt = h->dt;
+    m.omega = h->omega;
+    m.pscale = poisson_scale(h);
+    return m;
+}
+
+static void drop_graph(isf_batch_t b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+}
+
+// Upload the members' descriptors (synchronous: the batch stream is idle afterwards).
+static int batch_prepare(isf_batch_t b) {
+    for (size_t i = 0; i < b->mem.size(); i++)
+        if (!b->psi[i]) return fail(ISF_EINVAL, "batch member %zu has no psi bound (isf_batch_bind)", i);
+    HIP_CHECK(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    drop_graph(b);
+    for (BatchGroup& gr : b->groups) {
+        std::vector<Inst> host;
+        gr.max_spawn = 0;
+        gr.max_slots = 0;
+        for (int i : gr.idx) {
+            host.push_back(inst_of(b->mem[i], b->psi[i], b->parts[i], b->nslots[i], gr.g.ty));
+            const Inst& m = host.back();
+            if (m.nslots > 0) {
+                gr.max_slots = std::max(gr.max_slots, m.nslots);
+                gr.max_spawn = std::max(gr.max_spawn, m.n_spawn);
+            }
+        }
+        HIP_CHECK(hipMemcpy(gr.inst, host.data(), host.size() * sizeof(Inst), hipMemcpyHostToDevice),
+                  "batch descriptors upload");
+    }
+    b->dirty = false;
+    return ISF_OK;
+}
+
+// The kernel stages of one simloop body (test/runtests.jl:45-64) for one shape group,
+// in the order of isf_step.  The pointer arguments are ignored in batch mode.
+static int group_stages(const BatchGroup& gr, const Inst* inst, const int* counter, const DevTables& t0,
+                        double* norm_flag, hipStream_t s) {
+    const Batch bt{inst, counter, (int)gr.idx.size()};
+    const GridInfo& g = gr.g;
+    if (gr.max_spawn > 0)
+        LAUNCH_CHECK(launch_spawn(nullptr, 0, 0, 0, JetParams{}, 0, 0, s, bt, gr.max_spawn), "batch add_particles");
+    LAUNCH_CHECK(launch_flow_x_fwd(g, nullptr, nullptr, t0, s, bt), "batch flow x forward");
+    LAUNCH_CHECK(launch_flow_plane(g, nullptr, t0, s, bt), "batch flow plane");
+    LAUNCH_CHECK(launch_flow_x_inv(g, nullptr, nullptr, true, t0, s, bt), "batch flow x inverse + normalize");
+    for (int pass = 0; pass < 2; pass++) {
+        LAUNCH_CHECK(launch_div_psi(g, nullptr, nullptr, s, bt), "batch div(one_form(psi))");
+        LAUNCH_CHECK(launch_r2c_x(g, nullptr, nullptr, t0, s, bt), "batch poisson x forward");
+        LAUNCH_CHECK(launch_poisson_plane(g, nullptr, 0.0, t0, s, bt), "batch poisson plane");
+        LAUNCH_CHECK(launch_c2r_x(g, nullptr, pass == 0 ? 2 : 1, nullptr, nullptr, nullptr,
+                                  pass == 0 ? nullptr : norm_flag, t0, s, bt),
+                     "batch poisson x inverse + gauge");
+    }
+    LAUNCH_CHECK(launch_velocity(g, nullptr, 0.0f, 1, nullptr, s, bt), "batch velocity_one_form");
+    if (gr.max_slots > 0)
+        LAUNCH_CHECK(launch_advect(g, nullptr, 0, nullptr, nullptr, 0.0f, s, bt, gr.max_slots), "batch staggered_advect");
+    return ISF_OK;
+}
+
+// Fork the groups off the batch stream, enqueue their stages, join them back.
+static int batch_enqueue(isf_batch_t b) {
+    HIP_CHECK(hipEventRecord(b->fork, b->stream), "hipEventRecord");
+    for (BatchGroup& gr : b->groups) {
+        HIP_CHECK(hipStreamWaitEvent(gr.stream, b->fork, 0), "hipStreamWaitEvent");
+        isf_t h0 = b->mem[gr.idx[0]];
+        int r = group_stages(gr, gr.inst, b->counter, h0->tb, h0->norm_parts, gr.stream);
+        if (r) return r;
+        HIP_CHECK(hipEventRecord(gr.done, gr.stream), "hipEventRecord");
+        HIP_CHECK(hipStreamWaitEvent(b->stream, gr.done, 0), "hipStreamWaitEvent");
+    }
+    return ISF_OK;
+}
+
+extern "C" {
+
+int isf_batch_create(isf_batch_t* out, int n, const isf_member_desc* members) {
+    CHECK_PTR(out, "out");
+    CHECK_PTR(members, "members");
+    *out = nullptr;
+    if (n <= 0 || n > 65535) return fail(ISF_EINVAL, "batch size %d out of range (1..65535)", n);
+    isf_batch* b = new isf_batch();
+    b->magic = kBatchMagic;
+    b->use_graph = true;
+    b->dirty = true;
+    auto bail = [&](int code) {
+        batch_free(b);
+        return code;
+    };
+    for (int i = 0; i < n; i++) {
+        const isf_member_desc& d = members[i];
+        isf_t h = nullptr;
+        int r = isf_create(&h, d.res, d.box, d.hbar, d.dt);
+        if (r) return bail(fail(r, "batch member %d: %s", i, g_err.c_str()));
+        h->owner = b;
+        b->mem.push_back(h);
+        r = isf_jet_configure(h, d.kvec, d.omega, d.cen, d.len, d.rad, d.n_spawn, d.seed, d.phase_mode);
+        if (r) return bail(fail(r, "batch member %d: %s", i, g_err.c_str()));
+        b->seen_seq.push_back(h->seq);
+    }
+    b->psi.assign(n, nullptr);
+    b->parts.assign(n, nullptr);
+    b->nslots.assign(n, 0);
+    for (int i = 0; i < n; i++) {
+        const GridInfo& g = b->mem[i]->g;
+        BatchGroup* gr = nullptr;
+        for (BatchGroup& x : b->groups)
+            if (x.g.nx == g.nx && x.g.ny == g.ny && x.g.nz == g.nz) gr = &x;
+        if (!gr) {
+            b->groups.push_back(BatchGroup{});
+            gr = &b->groups.back();
+            gr->g = g;
+            gr->g.ty = 0;
+        }
+        gr->idx.push_back(i);
+    }
+    hipError_t e;
+    for (BatchGroup& gr : b->groups) {
+        if ((e = hipMalloc(&gr.inst, gr.idx.size() * sizeof(Inst))) != hipSuccess)
+            return bail(fail(ISF_ENOMEM, "batch descriptors: %s", hipGetErrorString(e)));
+        if ((e = hipStreamCreateWithFlags(&gr.stream, hipStreamNonBlocking)) != hipSuccess)
+            return bail(hip_fail(e, "hipStreamCreate"));
+        if ((e = hipEventCreateWithFlags(&gr.done, hipEventDisableTiming)) != hipSuccess)
+            return bail(hip_fail(e, "hipEventCreate"));
+    }
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(hip_fail(e, "hipStreamCreate"));
+    if ((e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess)
+        return bail(hip_fail(e, "hipEventCreate"));
+    if ((e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming)) != hipSuccess)
+        return bail(hip_fail(e, "hipEventCreate"));
+    if ((e = hipMalloc(&b->counter, sizeof(int))) != hipSuccess)
+        return bail(fail(ISF_ENOMEM, "batch counter: %s", hipGetErrorString(e)));
+    *out = b;
+    return ISF_OK;
+}
+
+int isf_batch_destroy(isf_batch_t b) {
+    CHECK_B(b);
+    batch_free(b);
+    return ISF_OK;
+}
+
+int isf_batch_member(isf_batch_t b, int i, isf_t* out) {
+    CHECK_B(b);
+    CHECK_PTR(out, "out");
+    if (i < 0 || i >= (int)b->mem.size()) return fail(ISF_EINVAL, "member %d out of range", i);
+    *out = b->mem[i];
+    return ISF_OK;
+}
+
+int isf_batch_bind(isf_batch_t b, int i, float* psi, float* parts, long long nslots) {
+    CHECK_B(b);
+    CHECK_PTR(psi, "psi");
+    if (i < 0 || i >= (int)b->mem.size()) return fail(ISF_EINVAL, "member %d out of range", i);
+    if (nslots < 0 || (nslots > 0 && !parts)) return fail(ISF_EINVAL, "bad particle buffer");
+    b->psi[i] = psi;
+    b->parts[i] = nslots > 0 ? parts : nullptr;
+    b->nslots[i] = nslots;
+    b->dirty = true;
+    return ISF_OK;
+}
+
+int isf_batch_set_graph(isf_batch_t b, int use_graph) {
+    CHECK_B(b);
+    b->use_graph = use_graph != 0;
+    if (!b->use_graph) {
+        HIP_CHECK(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        drop_graph(b);
+    }
+    return ISF_OK;
+}
+
+int isf_batch_step(isf_batch_t b, int iter) {
+    CHECK_B(b);
+    if (iter < 1) return fail(ISF_EINVAL, "iter must be >= 1");
+    if (b->dirty) {
+        int r = batch_prepare(b);
+        if (r) return r;
+    }
+    // order after work the members' own handles enqueued since the last batched step
+    for (size_t i = 0; i < b->mem.size(); i++) {
+        isf_t h = b->mem[i];
+        if (h->seq == b->seen_seq[i] && !h->side_pending) continue;
+        int r = join(h);
+        if (r) return r;
+        HIP_CHECK(hipEventRecord(h->ext_ev, h->stream), "hipEventRecord");
+        HIP_CHECK(hipStreamWaitEvent(b->stream, h->ext_ev, 0), "hipStreamWaitEvent");
+        b->seen_seq[i] = h->seq;
+    }
+    LAUNCH_CHECK(launch_set_counter(b->counter, iter, b->stream), "batch step counter");
+    if (b->use_graph) {
+        if (!b->exec) {
+            HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            const int r = batch_enqueue(b);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+            if (r) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return r;
+            }
+            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+            const hipError_t ei = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) {
+                b->exec = nullptr;
+                return hip_fail(ei, "hipGraphInstantiate");
+            }
+        }
+        HIP_CHECK(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
+    } else {
+        int r = batch_enqueue(b);
+        if (r) return r;
+    }
+    HIP_CHECK(hipEventRecord(b->done, b->stream), "hipEventRecord");
+    for (isf_t h : b->mem) {
+        h->batch_ev = b->done;
+        h->ext_pending = true;
+    }
+    return ISF_OK;
+}
+
+int isf_batch_step_sequential(isf_batch_t b, int iter) {
+    CHECK_B(b);
+    for (size_t i = 0; i < b->mem.size(); i++) {
+        if (!b->psi[i]) return fail(ISF_EINVAL, "batch member %zu has no psi bound (isf_batch_bind)", i);
+        int r = isf_step(b->mem[i], b->psi[i], b->parts[i], b->nslots[i], iter);
+        if (r) return r;
+    }
+    return ISF_OK;
+}
+
+int isf_batch_synchronize(isf_batch_t b) {
+    CHECK_B(b);
+    HIP_CHECK(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    for (size_t i = 0; i < b->mem.size(); i++) {
+        isf_t h = b->mem[i];
+        if (h->seq == b->seen_seq[i] && !h->side_pending) continue;
+        int r = isf_synchronize(h);
+        if (r) return r;
+        b->seen_seq[i] = h->seq;
+    }
+    return ISF_OK;
+}
+
+// Time the four x passes of one step (forward, inverse + normalize, R2C, C2R + gauge)
+// for one group with x tile ty, on scratch psi; returns milliseconds of `reps` rounds.
+static int time_x_tile(isf_batch_t b, const BatchGroup& gr, int ty, const Inst* inst, float* ms) {
+    GridInfo g = gr.g;
+    g.ty = ty;
+    const Batch bt{inst, b->counter, (int)gr.idx.size()};
+    isf_t h0 = b->mem[gr.idx[0]];
+    const DevTables& t0 = h0->tb;
+    auto once = [&]() -> int {
+        LAUNCH_CHECK(launch_flow_x_fwd(g, nullptr, nullptr, t0, b->stream, bt), "tune x forward");
+        LAUNCH_CHECK(launch_flow_x_inv(g, nullptr, nullptr, true, t0, b->stream, bt), "tune x inverse");
+        LAUNCH_CHECK(launch_r2c_x(g, nullptr, nullptr, t0, b->stream, bt), "tune r2c");
+        LAUNCH_CHECK(launch_c2r_x(g, nullptr, 1, nullptr, nullptr, nullptr, nullptr, t0, b->stream, bt), "tune c2r");
+        return ISF_OK;
+    };
+    int r = once();
+    if (r) return r;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_CHECK(hipEventCreate(&e0), "hipEventCreate");
+    hipError_t e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(e, "hipEventCreate");
+    }
+    const int reps = 5;
+    e = hipEventRecord(e0, b->stream);
+    for (int k = 0; !r && e == hipSuccess && k < reps; k++) r = once();
+    if (!r && e == hipSuccess) e = hipEventRecord(e1, b->stream);
+    if (!r && e == hipSuccess) e = hipEventSynchronize(e1);
+    if (!r && e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r) return r;
+    if (e != hipSuccess) return hip_fail(e, "tune timing");
+    return ISF_OK;
+}
+
+int isf_batch_tune(isf_batch_t b, const char* cache_path) {
+    CHECK_B(b);
+    HIP_CHECK(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    struct Entry {
+        int nx, ny, nz, members, ty;
+    };
+    std::vector<Entry> cache;
+    if (cache_path) {
+        if (FILE* f = std::fopen(cache_path, "r")) {
+            Entry en;
+            char line[256];
+            while (std::fgets(line, sizeof line, f))
+                if (std::sscanf(line, "%d %d %d %d %d", &en.nx, &en.ny, &en.nz, &en.members, &en.ty) == 5)
+                    cache.push_back(en);
+            std::fclose(f);
+        }
+    }
+    std::vector<Entry> fresh;
+    int rc = ISF_OK;
+    for (BatchGroup& gr : b->groups) {
+        const int members = (int)gr.idx.size();
+        int best = 0;
+        for (const Entry& en : cache)
+            if (en.nx == gr.g.nx && en.ny == gr.g.ny && en.nz == gr.g.nz && en.members == members &&
+                (en.ty == 8 || en.ty == 16 || en.ty == 32))
+                best = en.ty;
+        if (!best) {
+            std::vector<int> cand;
+            for (int ty : {8, 16, 32})
+                if (ty <= gr.g.ny && (ty == 8 || gr.g.nx <= 256) && gr.g.nx < 512) cand.push_back(ty);
+            if (cand.empty()) cand.push_back(8);
+            if (cand.size() == 1) {
+                best = cand[0];
+            } else {
+                // scratch psi per member; the members' own spectra are scratch already
+                const size_t bytes = (size_t)gr.g.nx * gr.g.ny * gr.g.nz * 16;
+                float* scratch = nullptr;
+                Inst* dinst = nullptr;
+                hipError_t e = hipMalloc(&scratch, bytes * members);
+                if (e == hipSuccess) e = hipMalloc(&dinst, sizeof(Inst) * members);
+                if (e == hipSuccess) e = hipMemset(scratch, 0x3c, bytes * members);  // 0.0115f everywhere
+                std::vector<Inst> host;
+                for (int k = 0; k < members; k++)
+                    host.push_back(inst_of(b->mem[gr.idx[k]], scratch + (bytes / 4) * k, nullptr, 0, 0));
+                if (e == hipSuccess) e = hipMemcpy(dinst, host.data(), sizeof(Inst) * members, hipMemcpyHostToDevice);
+                if (e == hipSuccess && launch_set_counter(b->counter, 1, b->stream) != 0) e = hipErrorLaunchFailure;
+                if (e != hipSuccess) rc = hip_fail(e, "tune scratch");
+                float best_ms = 0.0f;
+                for (int ty : cand) {
+                    if (rc) break;
+                    float ms = 0.0f;
+                    rc = time_x_tile(b, gr, ty, dinst, &ms);
+                    if (!rc && (best == 0 || ms < best_ms)) {
+                        best = ty;
+                        best_ms = ms;
+                    }
+                }
+                (void)hipStreamSynchronize(b->stream);
+                if (scratch) (void)hipFree(scratch);
+                if (dinst) (void)hipFree(dinst);
+                if (rc) return rc;
+            }
+            fresh.push_back(Entry{gr.g.nx, gr.g.ny, gr.g.nz, members, best});
+        }
+        gr.g.ty = best;
+        for (int i : gr.idx) b->mem[i]->g.ty = best;
+    }
+    if (cache_path && !fresh.empty()) {
+        if (FILE* f = std::fopen(cache_path, "a")) {
+            for (const Entry& en : fresh) std::fprintf(f, "%d %d %d %d %d\n", en.nx, en.ny, en.nz, en.members, en.ty);
+            std::fclose(f);
+        }
+    }
+    b->dirty = true;
+    return ISF_OK;
+}
+
+int isf_batch_tile(isf_batch_t b, const int res[3], int* ty) {
+    CHECK_B(b);
+    CHECK_PTR(res, "res");
+    CHECK_PTR(ty, "ty");
+    for (const BatchGroup& gr : b->groups)
+        if (gr.g.nx == res[0] && gr.g.ny == res[1] && gr.g.nz == res[2]) {
+            *ty = gr.g.ty;
+            return ISF_OK;
+        }
+    return fail(ISF_EINVAL, "no member of this batch has grid (%d, %d, %d)", res[0], res[1], res[2]);
 }
 
 }  // extern "C"

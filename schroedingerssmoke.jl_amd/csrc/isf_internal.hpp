@@ -14,6 +14,7 @@ struct GridInfo {
     float Lx, Ly, Lz;   // physical_size (integers in the reference, stored as float here)
     float idx, idy, idz;  // 1/d, used instead of a division only when d is a power of two
     int d_pow2;           // all three d are powers of two (then p * (1/d) == p / d exactly)
+    int ty;               // y lines per block of the x passes (0: default choice)
 };
 
 // Jet constraint parameters (src/particles.jl:12-27, test/runtests.jl:7-12,23-25).
@@ -36,24 +37,58 @@ struct DevTables {
     double* s2z;
 };
 
-// Launch-time choices of the x-direction passes.
+// One member of a batched launch: everything a step's kernels read for that instance.
+// The per-step values (omega t, spawn slots, step number) are derived on the device from
+// the batch's step counter.
+struct Inst {
+    GridInfo g;
+    JetParams jet;           // nozzle and kvec; omgterm is formed per step
+    DevTables tb;
+    float4* psi;
+    float2* S;               // spectrum; f = (float*)S and P = S + 4N bytes alias it
+    float* vel;
+    float* parts;
+    long long nslots;
+    long long n_spawn;
+    unsigned long long seed;
+    double* norm_parts;      // FP64 partial sums of the last gauge pass
+    float hbar, dt, omega;
+    double pscale;           // Poisson multiplier scale, -0.5/N
+};
+
+// Batch argument of every step kernel: inst == nullptr means a single instance whose
+// data come from the kernel's other arguments; otherwise the kernel serves member m of
+// inst (m taken from a grid dimension) and iter points to the batch's step counter.
+struct Batch {
+    const Inst* inst;
+    const int* iter;
+    int members;
+};
+
+// Launch-time choices of the x-direction passes (g.ty, when non-zero, overrides).
 int x_tile_lines(int nx, int ny);
 
 // ---- transform passes (return hipError_t as int)
-int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s);
-int launch_flow_plane(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s);
+// Every launcher takes an optional Batch (default: single instance).  In batch mode the
+// pointer arguments are ignored and g gives the shape shared by all members.
+int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s,
+                      Batch b = Batch{});
+int launch_flow_plane(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s, Batch b = Batch{});
 int launch_flow_x_inv(const GridInfo& g, const float2* S, float4* psi, bool normalize, const DevTables& t,
-                      hipStream_t s);
-int launch_r2c_x(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s);
-int launch_poisson_plane(const GridInfo& g, float2* P, double scale, const DevTables& t, hipStream_t s);
-// mode 0: write q (float per cell) ; mode 1: gauge psi ; mode 2: gauge psi + restrict
+                      hipStream_t s, Batch b = Batch{});
+int launch_r2c_x(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s, Batch b = Batch{});
+int launch_poisson_plane(const GridInfo& g, float2* P, double scale, const DevTables& t, hipStream_t s,
+                         Batch b = Batch{});
+// mode 0: write q (float per cell) ; mode 1: gauge psi ; mode 2: gauge psi + restrict.
+// In batch mode norm_partials != nullptr requests the members' own partial sums.
 int launch_c2r_x(const GridInfo& g, const float2* P, int mode, float* q, float4* psi, const JetParams* jet,
-                 double* norm_partials, const DevTables& t, hipStream_t s);
+                 double* norm_partials, const DevTables& t, hipStream_t s, Batch b = Batch{});
 size_t c2r_x_blocks(const GridInfo& g);
 
 // ---- stencil / pointwise
-int launch_div_psi(const GridInfo& g, const float4* psi, float* f, hipStream_t s);
-int launch_velocity(const GridInfo& g, const float4* psi, float hbar, int sharp, float* vel, hipStream_t s);
+int launch_div_psi(const GridInfo& g, const float4* psi, float* f, hipStream_t s, Batch b = Batch{});
+int launch_velocity(const GridInfo& g, const float4* psi, float hbar, int sharp, float* vel, hipStream_t s,
+                    Batch b = Batch{});
 int launch_div_vel(const GridInfo& g, const float* vel, float* f, hipStream_t s);
 int launch_normalize(const GridInfo& g, float4* psi, hipStream_t s);
 int launch_gauge(const GridInfo& g, float4* psi, const float* q, hipStream_t s);
@@ -64,7 +99,7 @@ int launch_reduce_sum(const double* partials, size_t n, double* out, hipStream_t
 
 // ---- particles
 int launch_spawn(float* parts, long long nslots, long long first, long long n, const JetParams& jet,
-                 unsigned long long seed, unsigned step, hipStream_t s);
+                 unsigned long long seed, unsigned step, hipStream_t s, Batch b = Batch{}, long long max_n = 0);
 // Uniform cloud in the nozzle cylinder in spatially coherent slot order (synchronous).
 int launch_fill_cylinder(float* parts, long long nslots, long long first, long long n, const JetParams& jet,
                          unsigned long long seed, hipStream_t s);
@@ -74,7 +109,10 @@ size_t order_measure_ints(long long nslots);
 int measure_order_jumps(const GridInfo& g, const float* parts, long long nslots, unsigned* dev_partial,
                         unsigned long long* host_count, hipStream_t s);
 int launch_advect(const GridInfo& g, float* parts, long long nslots, const int* order, const float* vel, float dt,
-                  hipStream_t s);
+                  hipStream_t s, Batch b = Batch{}, long long max_slots = 0);
+// Batch step counter: counter[0] = value / counter[0] += 1.
+int launch_set_counter(int* counter, int value, hipStream_t s);
+int launch_bump_counter(int* counter, hipStream_t s);
 // Slot processing order for the advection: slots sorted by bucket of 8 x-cells (stable
 // radix sort of (bucket, slot) pairs).  order_scratch_bytes(g, n) bytes of scratch.
 size_t order_scratch_bytes(const GridInfo& g, long long nslots);

@@ -105,6 +105,10 @@ __device__ __forceinline__ double block_sum(double v) {
     return red[0];
 }
 
+__device__ __forceinline__ float2* poisson_P(const Inst& m) {
+    return reinterpret_cast<float2*>(reinterpret_cast<char*>(m.S) + 4 * (size_t)m.g.nx * m.g.ny * m.g.nz);
+}
+
 template <int NX> struct XCfg {
     static constexpr int E = DefaultE<NX>::value;
     static constexpr int T = NX / E;
@@ -117,8 +121,15 @@ template <int NX> struct XCfg {
 // (c, kx) planes of S with y fastest (src/parallel.jl:179-181, x part of fft!).
 template <int NX, int TY>
 __global__ void __launch_bounds__(2 * TY * XCfg<NX>::T)
-k_flow_x_fwd(const float4* __restrict__ psi, float2* __restrict__ S, int ny, int nz, const float2* __restrict__ tw) {
+k_flow_x_fwd(const float4* __restrict__ psi, float2* __restrict__ S, int ny, int nz, const float2* __restrict__ tw,
+             Batch B) {
     constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
+    if (B.inst) {
+        const Inst& m = B.inst[blockIdx.z];
+        psi = m.psi;
+        S = m.S;
+        tw = m.tb.tw;
+    }
     constexpr int NT = 2 * TY * T;
     __shared__ float2 lds[2 * TY * LS];
     __shared__ float2 twl[FftTw<NX, E>::SIZE];
@@ -145,8 +156,15 @@ k_flow_x_fwd(const float4* __restrict__ psi, float2* __restrict__ S, int ny, int
 // (src/parallel.jl:162-165,184-185).  The 1/N of ifft is folded into the mask.
 template <int NX, int TY, bool NORMALIZE>
 __global__ void __launch_bounds__(2 * TY * XCfg<NX>::T)
-k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, int ny, int nz, const float2* __restrict__ tw) {
+k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, int ny, int nz, const float2* __restrict__ tw,
+             Batch B) {
     constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
+    if (B.inst) {
+        const Inst& m = B.inst[blockIdx.z];
+        psi = m.psi;
+        S = m.S;
+        tw = m.tb.tw;
+    }
     constexpr int NT = 2 * TY * T;
     __shared__ float2 lds[2 * TY * LS];
     __shared__ float2 twl[FftTw<NX, E>::SIZE];
@@ -209,8 +227,14 @@ enum { PLANE_FLOW = 0, PLANE_POISSON = 1 };
 //                  half-length real inverse)
 template <int NY, int NZ, int MODE>
 __global__ void __launch_bounds__((PlaneCfg<NY, NZ>::NT))
-k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables tb) {
+k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables tb, Batch B) {
     using C = PlaneCfg<NY, NZ>;
+    if (B.inst) {
+        const Inst& m = B.inst[blockIdx.y];
+        base = MODE == PLANE_FLOW ? m.S : poisson_P(m);
+        scale = m.pscale;
+        tb = m.tb;
+    }
     constexpr int E = C::E, NT = C::NT, LSY = C::LSY;
     constexpr int RY2 = C::RY2, RZ2 = C::RZ2;
     using TY2 = PassTw<NY, RY2, 16>;   // forward y pass 2
@@ -343,8 +367,14 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
 // F[k] = E[k] + W^k O[k], k = 0..nx/2, written into the kx planes of P.
 template <int NX, int TY>
 __global__ void __launch_bounds__(TY * XCfg<NX / 2>::T)
-k_r2c_x(const float* __restrict__ f, float2* __restrict__ P, int ny, int nz, const float2* __restrict__ tw) {
+k_r2c_x(const float* __restrict__ f, float2* __restrict__ P, int ny, int nz, const float2* __restrict__ tw, Batch B) {
     constexpr int M = NX / 2;
+    if (B.inst) {
+        const Inst& m = B.inst[blockIdx.z];
+        f = reinterpret_cast<const float*>(m.S);
+        P = poisson_P(m);
+        tw = m.tb.tw;
+    }
     constexpr int T = XCfg<M>::T, E = XCfg<M>::E, LS = XCfg<M>::LS;
     constexpr int NT = TY * T;
     __shared__ float2 lds[TY * LS];
@@ -378,8 +408,18 @@ enum { C2R_Q = 0, C2R_GAUGE = 1, C2R_GAUGE_RESTRICT = 2 };
 template <int NX, int TY, int MODE>
 __global__ void __launch_bounds__(TY * XCfg<NX / 2>::T)
 k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restrict__ psi, GridInfo g, JetParams jet,
-        double* __restrict__ partials, const float2* __restrict__ tw) {
+        double* __restrict__ partials, const float2* __restrict__ tw, Batch B) {
     constexpr int M = NX / 2;
+    if (B.inst) {
+        const Inst& m = B.inst[blockIdx.z];
+        P = poisson_P(m);
+        psi = m.psi;
+        g = m.g;
+        jet = m.jet;
+        jet.omgterm = m.omega * ((float)(*B.iter) * m.dt);  // test/runtests.jl:46,57
+        partials = partials ? m.norm_parts : nullptr;
+        tw = m.tb.tw;
+    }
     constexpr int T = XCfg<M>::T, E = XCfg<M>::E;
     constexpr int LS = padded_len(M + 1);
     constexpr int NT = TY * T;
@@ -462,11 +502,21 @@ __device__ __forceinline__ float4 shfl4(float4 v, int src) {
 // Fused velocity_one_form (hbar = 1) + div (src/parallel.jl:100-113,131-143,156-157):
 // f(c) = sum_a (v_a(c) - v_a(c - e_a)) / d_a^2.
 template <int BX>
-__global__ void __launch_bounds__(BX * ST_BY) k_div_psi(const float4* __restrict__ psi, float* __restrict__ f, GridInfo g) {
+__global__ void __launch_bounds__(BX * ST_BY)
+k_div_psi(const float4* __restrict__ psi, float* __restrict__ f, GridInfo g, Batch B) {
     __shared__ float vy_s[2][ST_BY][BX];
+    int zslab = blockIdx.z;
+    if (B.inst) {
+        const int nzs = g.nz / ST_TZ;
+        const Inst& m = B.inst[blockIdx.z / nzs];
+        zslab = blockIdx.z % nzs;
+        psi = m.psi;
+        f = reinterpret_cast<float*>(m.S);
+        g = m.g;
+    }
     const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
     const int lane = threadIdx.x % 64;
-    const int x = blockIdx.x * BX + tx, y = blockIdx.y * ST_BY + ty, z0 = blockIdx.z * ST_TZ;
+    const int x = blockIdx.x * BX + tx, y = blockIdx.y * ST_BY + ty, z0 = zslab * ST_TZ;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const size_t sy = nx, sz = (size_t)nx * ny;
     const int xp = x + 1 < nx ? x + 1 : 0, xm = x > 0 ? x - 1 : nx - 1;
@@ -511,10 +561,20 @@ __global__ void __launch_bounds__(BX * ST_BY) k_div_psi(const float4* __restrict
 // in-place StaggeredSharp v_a *= 1/d_a (test/runtests.jl:61-62).
 template <int BX>
 __global__ void __launch_bounds__(BX * ST_BY)
-k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, float hbar, int sharp) {
+k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, float hbar, int sharp, Batch B) {
+    int zslab = blockIdx.z;
+    if (B.inst) {
+        const int nzs = g.nz / ST_TZ;
+        const Inst& m = B.inst[blockIdx.z / nzs];
+        zslab = blockIdx.z % nzs;
+        psi = m.psi;
+        vel = m.vel;
+        g = m.g;
+        hbar = m.hbar;
+    }
     const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
     const int lane = threadIdx.x % 64;
-    const int x = blockIdx.x * BX + tx, y = blockIdx.y * ST_BY + ty, z0 = blockIdx.z * ST_TZ;
+    const int x = blockIdx.x * BX + tx, y = blockIdx.y * ST_BY + ty, z0 = zslab * ST_TZ;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const size_t sy = nx, sz = (size_t)nx * ny;
     const int xp = x + 1 < nx ? x + 1 : 0, yp = y + 1 < ny ? y + 1 : 0;
@@ -690,7 +750,16 @@ __device__ __forceinline__ void stag_vel(const float* __restrict__ vel, const fl
 // which gathers share cache lines, never a result.
 __global__ void __launch_bounds__(256)
 k_advect(float* __restrict__ parts, long long nslots, const int* __restrict__ order, const float* __restrict__ vel,
-         GridInfo g, float dt) {
+         GridInfo g, float dt, Batch B) {
+    if (B.inst) {
+        const Inst& m = B.inst[blockIdx.y];
+        parts = m.parts;
+        nslots = m.nslots;
+        order = nullptr;
+        vel = m.vel;
+        g = m.g;
+        dt = m.dt;
+    }
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nslots) return;
     const long long s = order ? (long long)order[k] : k;
@@ -736,10 +805,24 @@ __device__ __forceinline__ void spawn_draw(uint32_t j, unsigned step, unsigned l
 
 __global__ void __launch_bounds__(SPAWN_CHUNK)
 k_spawn(float* __restrict__ parts, long long nslots, long long first, long long n, JetParams jt,
-        unsigned long long seed, unsigned step) {
+        unsigned long long seed, unsigned step, Batch B) {
     __shared__ uint32_t key[SPAWN_CHUNK];
+    if (B.inst) {
+        // the ring-buffer slots of test/runtests.jl:52-53 for step *iter
+        const Inst& m = B.inst[blockIdx.y];
+        const int it = *B.iter;
+        parts = m.parts;
+        nslots = m.nslots;
+        n = nslots > 0 ? m.n_spawn : 0;
+        jt = m.jet;
+        seed = m.seed;
+        step = (unsigned)it;
+        const long long start1 = nslots > 0 ? ((long long)(it - 1) * m.n_spawn + 1) % nslots : 0;
+        first = start1 == 0 ? nslots - 1 : start1 - 1;
+    }
     const int t = threadIdx.x;
     const long long base = (long long)blockIdx.x * SPAWN_CHUNK;
+    if (base >= n) return;  // uniform per block (batch members with fewer draws)
     const int m = (int)min((long long)SPAWN_CHUNK, n - base);
     uint32_t raw[4];
     if (t < m) {
@@ -786,9 +869,18 @@ bool x_len_supported(int n) { return n == 16 || n == 32 || n == 64 || n == 128 |
 static bool yz_len_ok(int n) { return n == 16 || n == 32 || n == 64 || n == 128; }
 bool plane_supported(int ny, int nz) { return yz_len_ok(ny) && yz_len_ok(nz) && ny * nz <= 16384; }
 
+// y lines per block of the x passes: g.ty when set (8, 16 or 32, not above ny; 32 only
+// for nx <= 256), else 16 (8 for nx = 512)
 int x_tile_lines(int nx, int ny) {
     const int ty = nx >= 512 ? 8 : 16;
     return ty < ny ? ty : ny;
+}
+static int x_tile(const GridInfo& g) {
+    int ty = g.ty > 0 ? g.ty : x_tile_lines(g.nx, g.ny);
+    if (ty > g.ny) ty = g.ny;
+    if (ty == 32 && g.nx > 256) ty = 16;
+    if (g.nx >= 512) ty = 8;
+    return ty;
 }
 
 #define ISF_LAUNCH(kernel, grid, block, stream, ...)                      \
@@ -797,70 +889,73 @@ int x_tile_lines(int nx, int ny) {
         return (int)hipGetLastError();                                    \
     } while (0)
 
+static inline unsigned members_of(const Batch& b) { return b.inst ? (unsigned)b.members : 1u; }
+
 template <int NX, int TY>
-static int flow_x_fwd_t(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s) {
-    dim3 grid(g.ny / TY, g.nz);
-    ISF_LAUNCH((k_flow_x_fwd<NX, TY>), grid, dim3(2 * TY * XCfg<NX>::T), s, psi, S, g.ny, g.nz, t.tw);
+static int flow_x_fwd_t(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s, Batch b) {
+    dim3 grid(g.ny / TY, g.nz, members_of(b));
+    ISF_LAUNCH((k_flow_x_fwd<NX, TY>), grid, dim3(2 * TY * XCfg<NX>::T), s, psi, S, g.ny, g.nz, t.tw, b);
 }
 template <int NX, int TY>
-static int flow_x_inv_t(const GridInfo& g, const float2* S, float4* psi, bool nrm, const DevTables& t, hipStream_t s) {
-    dim3 grid(g.ny / TY, g.nz);
-    if (nrm) ISF_LAUNCH((k_flow_x_inv<NX, TY, true>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, g.ny, g.nz, t.tw);
-    ISF_LAUNCH((k_flow_x_inv<NX, TY, false>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, g.ny, g.nz, t.tw);
+static int flow_x_inv_t(const GridInfo& g, const float2* S, float4* psi, bool nrm, const DevTables& t, hipStream_t s,
+                        Batch b) {
+    dim3 grid(g.ny / TY, g.nz, members_of(b));
+    if (nrm) ISF_LAUNCH((k_flow_x_inv<NX, TY, true>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, g.ny, g.nz, t.tw, b);
+    ISF_LAUNCH((k_flow_x_inv<NX, TY, false>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, g.ny, g.nz, t.tw, b);
 }
 template <int NX, int TY>
-static int r2c_x_t(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s) {
-    dim3 grid(g.ny / TY, g.nz);
-    ISF_LAUNCH((k_r2c_x<NX, TY>), grid, dim3(TY * XCfg<NX / 2>::T), s, f, P, g.ny, g.nz, t.tw);
+static int r2c_x_t(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s, Batch b) {
+    dim3 grid(g.ny / TY, g.nz, members_of(b));
+    ISF_LAUNCH((k_r2c_x<NX, TY>), grid, dim3(TY * XCfg<NX / 2>::T), s, f, P, g.ny, g.nz, t.tw, b);
 }
 template <int NX, int TY>
 static int c2r_x_t(const GridInfo& g, const float2* P, int mode, float* q, float4* psi, const JetParams* jet,
-                   double* parts, const DevTables& t, hipStream_t s) {
-    dim3 grid(g.ny / TY, g.nz);
+                   double* parts, const DevTables& t, hipStream_t s, Batch b) {
+    dim3 grid(g.ny / TY, g.nz, members_of(b));
     dim3 block(TY * XCfg<NX / 2>::T);
     JetParams j{};
     if (jet) j = *jet;
-    if (mode == C2R_Q) ISF_LAUNCH((k_c2r_x<NX, TY, C2R_Q>), grid, block, s, P, q, psi, g, j, parts, t.tw);
-    if (mode == C2R_GAUGE) ISF_LAUNCH((k_c2r_x<NX, TY, C2R_GAUGE>), grid, block, s, P, q, psi, g, j, parts, t.tw);
-    ISF_LAUNCH((k_c2r_x<NX, TY, C2R_GAUGE_RESTRICT>), grid, block, s, P, q, psi, g, j, parts, t.tw);
+    if (mode == C2R_Q) ISF_LAUNCH((k_c2r_x<NX, TY, C2R_Q>), grid, block, s, P, q, psi, g, j, parts, t.tw, b);
+    if (mode == C2R_GAUGE) ISF_LAUNCH((k_c2r_x<NX, TY, C2R_GAUGE>), grid, block, s, P, q, psi, g, j, parts, t.tw, b);
+    ISF_LAUNCH((k_c2r_x<NX, TY, C2R_GAUGE_RESTRICT>), grid, block, s, P, q, psi, g, j, parts, t.tw, b);
 }
 
 // dispatch on (nx, tile) for the x passes
-#define ISF_X_DISPATCH(FN, g, ...)                                     \
-    do {                                                               \
-        const int ty = x_tile_lines(g.nx, g.ny);                       \
-        switch (g.nx) {                                                \
-            case 16:                                                   \
-                return ty == 16 ? FN<16, 16>(g, __VA_ARGS__) : FN<16, 8>(g, __VA_ARGS__);     \
-            case 32:                                                   \
-                return ty == 16 ? FN<32, 16>(g, __VA_ARGS__) : FN<32, 8>(g, __VA_ARGS__);     \
-            case 64:                                                   \
-                return ty == 16 ? FN<64, 16>(g, __VA_ARGS__) : FN<64, 8>(g, __VA_ARGS__);     \
-            case 128:                                                  \
-                return ty == 16 ? FN<128, 16>(g, __VA_ARGS__) : FN<128, 8>(g, __VA_ARGS__);   \
-            case 256:                                                  \
-                return ty == 16 ? FN<256, 16>(g, __VA_ARGS__) : FN<256, 8>(g, __VA_ARGS__);   \
-            case 512:                                                  \
-                return FN<512, 8>(g, __VA_ARGS__);                     \
-        }                                                              \
-        return (int)hipErrorInvalidValue;                              \
+#define ISF_X_CASE(NX, FN, ...)                                            \
+    case NX:                                                               \
+        if (ty == 32 && NX <= 256) return FN<NX, (NX <= 256 ? 32 : 8)>(__VA_ARGS__); \
+        if (ty == 16 && NX <= 256) return FN<NX, (NX <= 256 ? 16 : 8)>(__VA_ARGS__); \
+        return FN<NX, 8>(__VA_ARGS__);
+#define ISF_X_DISPATCH(FN, g, ...)                                                              \
+    do {                                                                                        \
+        const int ty = x_tile(g);                                                               \
+        switch (g.nx) {                                                                         \
+            ISF_X_CASE(16, FN, g, __VA_ARGS__)                                                  \
+            ISF_X_CASE(32, FN, g, __VA_ARGS__)                                                  \
+            ISF_X_CASE(64, FN, g, __VA_ARGS__)                                                  \
+            ISF_X_CASE(128, FN, g, __VA_ARGS__)                                                 \
+            ISF_X_CASE(256, FN, g, __VA_ARGS__)                                                 \
+            case 512:                                                                           \
+                return FN<512, 8>(g, __VA_ARGS__);                                              \
+        }                                                                                       \
+        return (int)hipErrorInvalidValue;                                                       \
     } while (0)
 
-int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s) {
-    ISF_X_DISPATCH(flow_x_fwd_t, g, psi, S, t, s);
+int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s, Batch b) {
+    ISF_X_DISPATCH(flow_x_fwd_t, g, psi, S, t, s, b);
 }
 int launch_flow_x_inv(const GridInfo& g, const float2* S, float4* psi, bool normalize, const DevTables& t,
-                      hipStream_t s) {
-    ISF_X_DISPATCH(flow_x_inv_t, g, S, psi, normalize, t, s);
+                      hipStream_t s, Batch b) {
+    ISF_X_DISPATCH(flow_x_inv_t, g, S, psi, normalize, t, s, b);
 }
-int launch_r2c_x(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s) {
-    ISF_X_DISPATCH(r2c_x_t, g, f, P, t, s);
+int launch_r2c_x(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s, Batch b) {
+    ISF_X_DISPATCH(r2c_x_t, g, f, P, t, s, b);
 }
 int launch_c2r_x(const GridInfo& g, const float2* P, int mode, float* q, float4* psi, const JetParams* jet,
-                 double* norm_partials, const DevTables& t, hipStream_t s) {
-    ISF_X_DISPATCH(c2r_x_t, g, P, mode, q, psi, jet, norm_partials, t, s);
+                 double* norm_partials, const DevTables& t, hipStream_t s, Batch b) {
+    ISF_X_DISPATCH(c2r_x_t, g, P, mode, q, psi, jet, norm_partials, t, s, b);
 }
-size_t c2r_x_blocks(const GridInfo& g) { return (size_t)(g.ny / x_tile_lines(g.nx, g.ny)) * g.nz; }
+size_t c2r_x_blocks(const GridInfo& g) { return (size_t)(g.ny / x_tile(g)) * g.nz; }
 
 static int cu_count() {
     static int cached[64] = {0};
@@ -875,27 +970,31 @@ static int cu_count() {
 }
 
 template <int NY, int NZ>
-static int plane_t(int mode, float2* base, int nplanes, int nx, double scale, const DevTables& t, hipStream_t s) {
+static int plane_t(int mode, float2* base, int nplanes, int nx, double scale, const DevTables& t, hipStream_t s,
+                   Batch b) {
     using C = PlaneCfg<NY, NZ>;
     const int by_lds = (160 * 1024) / C::LDS_BYTES, by_threads = 2048 / C::NT;
     const int per_cu = std::max(1, std::min(by_lds, by_threads));
-    const int blocks = std::min(nplanes, cu_count() * per_cu);
+    const int mem = (int)members_of(b);
+    // blocks per member: enough to fill the GPU once across all members
+    const int blocks = std::max(1, std::min(nplanes, (cu_count() * per_cu + mem - 1) / mem));
+    const dim3 grid(blocks, mem);
     if (mode == PLANE_FLOW)
-        ISF_LAUNCH((k_plane<NY, NZ, PLANE_FLOW>), dim3(blocks), dim3(C::NT), s, base, nplanes, nx, scale, t);
-    ISF_LAUNCH((k_plane<NY, NZ, PLANE_POISSON>), dim3(blocks), dim3(C::NT), s, base, nplanes, nx, scale, t);
+        ISF_LAUNCH((k_plane<NY, NZ, PLANE_FLOW>), grid, dim3(C::NT), s, base, nplanes, nx, scale, t, b);
+    ISF_LAUNCH((k_plane<NY, NZ, PLANE_POISSON>), grid, dim3(C::NT), s, base, nplanes, nx, scale, t, b);
 }
 
-#define ISF_PLANE_Z(NY)                                                         \
-    switch (nz) {                                                               \
-        case 16: return plane_t<NY, 16>(mode, base, nplanes, nx, scale, t, s);  \
-        case 32: return plane_t<NY, 32>(mode, base, nplanes, nx, scale, t, s);  \
-        case 64: return plane_t<NY, 64>(mode, base, nplanes, nx, scale, t, s);  \
-        case 128: return plane_t<NY, 128>(mode, base, nplanes, nx, scale, t, s); \
-    }                                                                           \
+#define ISF_PLANE_Z(NY)                                                            \
+    switch (nz) {                                                                  \
+        case 16: return plane_t<NY, 16>(mode, base, nplanes, nx, scale, t, s, b);  \
+        case 32: return plane_t<NY, 32>(mode, base, nplanes, nx, scale, t, s, b);  \
+        case 64: return plane_t<NY, 64>(mode, base, nplanes, nx, scale, t, s, b);  \
+        case 128: return plane_t<NY, 128>(mode, base, nplanes, nx, scale, t, s, b); \
+    }                                                                              \
     return (int)hipErrorInvalidValue;
 
 static int plane_dispatch(int mode, int ny, int nz, float2* base, int nplanes, int nx, double scale,
-                          const DevTables& t, hipStream_t s) {
+                          const DevTables& t, hipStream_t s, Batch b) {
     if (!plane_supported(ny, nz)) return (int)hipErrorInvalidValue;
     switch (ny) {
         case 16: { ISF_PLANE_Z(16) }
@@ -906,27 +1005,31 @@ static int plane_dispatch(int mode, int ny, int nz, float2* base, int nplanes, i
     return (int)hipErrorInvalidValue;
 }
 
-int launch_flow_plane(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s) {
-    return plane_dispatch(PLANE_FLOW, g.ny, g.nz, S, 2 * g.nx, g.nx, 0.0, t, s);
+int launch_flow_plane(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s, Batch b) {
+    return plane_dispatch(PLANE_FLOW, g.ny, g.nz, S, 2 * g.nx, g.nx, 0.0, t, s, b);
 }
-int launch_poisson_plane(const GridInfo& g, float2* P, double scale, const DevTables& t, hipStream_t s) {
-    return plane_dispatch(PLANE_POISSON, g.ny, g.nz, P, g.nx / 2 + 1, g.nx, scale, t, s);
+int launch_poisson_plane(const GridInfo& g, float2* P, double scale, const DevTables& t, hipStream_t s, Batch b) {
+    return plane_dispatch(PLANE_POISSON, g.ny, g.nz, P, g.nx / 2 + 1, g.nx, scale, t, s, b);
 }
 
 static inline dim3 grid1d(size_t n, int nt) { return dim3((unsigned)((n + nt - 1) / nt)); }
 static inline size_t ncells(const GridInfo& g) { return (size_t)g.nx * g.ny * g.nz; }
 
-static dim3 stencil_grid(const GridInfo& g, int bx) { return dim3(g.nx / bx, g.ny / ST_BY, g.nz / ST_TZ); }
-
-int launch_div_psi(const GridInfo& g, const float4* psi, float* f, hipStream_t s) {
-    if (g.nx >= 64) ISF_LAUNCH(k_div_psi<64>, stencil_grid(g, 64), dim3(64 * ST_BY), s, psi, f, g);
-    if (g.nx == 32) ISF_LAUNCH(k_div_psi<32>, stencil_grid(g, 32), dim3(32 * ST_BY), s, psi, f, g);
-    ISF_LAUNCH(k_div_psi<16>, stencil_grid(g, 16), dim3(16 * ST_BY), s, psi, f, g);
+static dim3 stencil_grid(const GridInfo& g, int bx, const Batch& b) {
+    return dim3(g.nx / bx, g.ny / ST_BY, (g.nz / ST_TZ) * members_of(b));
 }
-int launch_velocity(const GridInfo& g, const float4* psi, float hbar, int sharp, float* vel, hipStream_t s) {
-    if (g.nx >= 64) ISF_LAUNCH(k_velocity<64>, stencil_grid(g, 64), dim3(64 * ST_BY), s, psi, vel, g, hbar, sharp);
-    if (g.nx == 32) ISF_LAUNCH(k_velocity<32>, stencil_grid(g, 32), dim3(32 * ST_BY), s, psi, vel, g, hbar, sharp);
-    ISF_LAUNCH(k_velocity<16>, stencil_grid(g, 16), dim3(16 * ST_BY), s, psi, vel, g, hbar, sharp);
+
+int launch_div_psi(const GridInfo& g, const float4* psi, float* f, hipStream_t s, Batch b) {
+    if (g.nx >= 64) ISF_LAUNCH(k_div_psi<64>, stencil_grid(g, 64, b), dim3(64 * ST_BY), s, psi, f, g, b);
+    if (g.nx == 32) ISF_LAUNCH(k_div_psi<32>, stencil_grid(g, 32, b), dim3(32 * ST_BY), s, psi, f, g, b);
+    ISF_LAUNCH(k_div_psi<16>, stencil_grid(g, 16, b), dim3(16 * ST_BY), s, psi, f, g, b);
+}
+int launch_velocity(const GridInfo& g, const float4* psi, float hbar, int sharp, float* vel, hipStream_t s, Batch b) {
+    if (g.nx >= 64)
+        ISF_LAUNCH(k_velocity<64>, stencil_grid(g, 64, b), dim3(64 * ST_BY), s, psi, vel, g, hbar, sharp, b);
+    if (g.nx == 32)
+        ISF_LAUNCH(k_velocity<32>, stencil_grid(g, 32, b), dim3(32 * ST_BY), s, psi, vel, g, hbar, sharp, b);
+    ISF_LAUNCH(k_velocity<16>, stencil_grid(g, 16, b), dim3(16 * ST_BY), s, psi, vel, g, hbar, sharp, b);
 }
 int launch_div_vel(const GridInfo& g, const float* vel, float* f, hipStream_t s) {
     ISF_LAUNCH(k_div_vel, grid1d(ncells(g), 256), dim3(256), s, vel, f, g);
@@ -951,15 +1054,28 @@ int launch_norm2(const GridInfo& g, const float4* psi, double* partials, size_t*
 int launch_reduce_sum(const double* partials, size_t n, double* out, hipStream_t s) {
     ISF_LAUNCH(k_reduce_sum, dim3(1), dim3(256), s, partials, n, out);
 }
+// batch mode: max_n = the largest n_spawn of the members (grid size)
 int launch_spawn(float* parts, long long nslots, long long first, long long n, const JetParams& jet,
-                 unsigned long long seed, unsigned step, hipStream_t s) {
-    if (n <= 0) return 0;
-    ISF_LAUNCH(k_spawn, grid1d((size_t)n, SPAWN_CHUNK), dim3(SPAWN_CHUNK), s, parts, nslots, first, n, jet, seed, step);
+                 unsigned long long seed, unsigned step, hipStream_t s, Batch b, long long max_n) {
+    const long long nn = b.inst ? max_n : n;
+    if (nn <= 0) return 0;
+    const dim3 grid((unsigned)((nn + SPAWN_CHUNK - 1) / SPAWN_CHUNK), members_of(b));
+    ISF_LAUNCH(k_spawn, grid, dim3(SPAWN_CHUNK), s, parts, nslots, first, n, jet, seed, step, b);
 }
+// batch mode: max_slots = the largest nslots of the members (grid size)
 int launch_advect(const GridInfo& g, float* parts, long long nslots, const int* order, const float* vel, float dt,
-                  hipStream_t s) {
-    if (nslots <= 0) return 0;
-    ISF_LAUNCH(k_advect, grid1d((size_t)nslots, 256), dim3(256), s, parts, nslots, order, vel, g, dt);
+                  hipStream_t s, Batch b, long long max_slots) {
+    const long long nn = b.inst ? max_slots : nslots;
+    if (nn <= 0) return 0;
+    const dim3 grid((unsigned)((nn + 255) / 256), members_of(b));
+    ISF_LAUNCH(k_advect, grid, dim3(256), s, parts, nslots, order, vel, g, dt, b);
 }
+
+__global__ void k_set_counter(int* c, int v) { c[0] = v; }
+__global__ void k_bump_counter(int* c) { c[0] += 1; }
+int launch_set_counter(int* counter, int value, hipStream_t s) {
+    ISF_LAUNCH(k_set_counter, dim3(1), dim3(1), s, counter, value);
+}
+int launch_bump_counter(int* counter, hipStream_t s) { ISF_LAUNCH(k_bump_counter, dim3(1), dim3(1), s, counter); }
 
 }  // namespace isf

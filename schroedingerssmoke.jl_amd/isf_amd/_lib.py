@@ -17,6 +17,13 @@ c_float_p = C.POINTER(C.c_float)
 c_int3 = C.c_int * 3
 c_float3 = C.c_float * 3
 
+
+class MemberDesc(C.Structure):
+    """isf_member_desc of include/isf.h: one member of a batched ensemble."""
+    _fields_ = [("res", C.c_int * 3), ("box", C.c_int * 3), ("hbar", C.c_float), ("dt", C.c_float),
+                ("kvec", C.c_float * 3), ("omega", C.c_float), ("cen", C.c_float * 3), ("len", C.c_float),
+                ("rad", C.c_float), ("n_spawn", C.c_longlong), ("seed", C.c_ulonglong), ("phase_mode", C.c_int)]
+
 # (name, argtypes) of every exported entry point of include/isf.h
 SIGNATURES = {
     "isf_last_error": ([], C.c_char_p),
@@ -56,6 +63,16 @@ SIGNATURES = {
     "isf_step_stage_count": ([], C.c_int),
     "isf_step_stage_name": ([C.c_int], C.c_char_p),
     "isf_step_profile": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.POINTER(C.c_float)], C.c_int),
+    "isf_batch_create": ([C.POINTER(C.c_void_p), C.c_int, C.POINTER(MemberDesc)], C.c_int),
+    "isf_batch_destroy": ([C.c_void_p], C.c_int),
+    "isf_batch_member": ([C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "isf_batch_bind": ([C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_longlong], C.c_int),
+    "isf_batch_step": ([C.c_void_p, C.c_int], C.c_int),
+    "isf_batch_set_graph": ([C.c_void_p, C.c_int], C.c_int),
+    "isf_batch_step_sequential": ([C.c_void_p, C.c_int], C.c_int),
+    "isf_batch_synchronize": ([C.c_void_p], C.c_int),
+    "isf_batch_tune": ([C.c_void_p, C.c_char_p], C.c_int),
+    "isf_batch_tile": ([C.c_void_p, c_int3, C.POINTER(C.c_int)], C.c_int),
     "isf_timer_start": ([C.c_void_p], C.c_int),
     "isf_timer_stop": ([C.c_void_p, C.POINTER(C.c_float)], C.c_int),
 }

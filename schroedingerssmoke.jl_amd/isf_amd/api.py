@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import ISFError, c_float3, c_int3, check, load
+from ._lib import ISFError, MemberDesc, c_float3, c_int3, check, load
 
 
 class DeviceArray:
@@ -93,11 +93,27 @@ class ISF:
                                   c_int3(*[int(v) for v in physical_size]),
                                   float(np.float32(hbar)), float(np.float32(dt))), "isf_create")
         self.h = h.value
-        self.grid_res = tuple(int(v) for v in dims)
-        self.physical_size = tuple(int(v) for v in physical_size)
+        self.owned = True
+        self._read_grid()
+
+    @classmethod
+    def _borrowed(cls, h):
+        """A view of a handle owned elsewhere (a batch member); never destroyed here."""
+        isf = cls.__new__(cls)
+        isf.lib = load()
+        isf.h = h
+        isf.owned = False
+        isf._read_grid()
+        return isf
+
+    def _read_grid(self):
+        res = c_int3()
+        box = (C.c_float * 3)()
         d = (C.c_float * 3)()
         hb, dt_ = C.c_float(), C.c_float()
-        check(self.lib.isf_get_grid(self.h, None, d, None, C.byref(hb), C.byref(dt_)), "isf_get_grid")
+        check(self.lib.isf_get_grid(self.h, res, d, box, C.byref(hb), C.byref(dt_)), "isf_get_grid")
+        self.grid_res = tuple(int(v) for v in res)
+        self.physical_size = tuple(int(round(v)) for v in box)
         self.d = tuple(np.float32(v) for v in d)
         self.hbar = np.float32(hb.value)
         self.dt = np.float32(dt_.value)
@@ -125,7 +141,8 @@ class ISF:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.isf_destroy(self.h)
+            if getattr(self, "owned", True):
+                self.lib.isf_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -239,6 +256,80 @@ class ISF:
         return ms.value
 
 
+class Batch:
+    """A batched uneven ensemble (isf_batch_*): independent jets of mixed grid sizes stepped
+    together, one hipGraph replay per step.  ``members`` is a list of dicts with the keys
+    of ``isf_member_desc`` (res, box, hbar, dt, kvec, omega, cen, len, rad, n_spawn, seed,
+    phase_mode)."""
+
+    def __init__(self, members):
+        self.lib = load()
+        n = len(members)
+        arr = (MemberDesc * n)()
+        for i, m in enumerate(members):
+            d = arr[i]
+            d.res = c_int3(*[int(v) for v in m["res"]])
+            d.box = c_int3(*[int(v) for v in m["box"]])
+            d.hbar = float(np.float32(m["hbar"]))
+            d.dt = float(np.float32(m["dt"]))
+            d.kvec = c_float3(*[float(np.float32(v)) for v in m["kvec"]])
+            d.omega = float(np.float32(m["omega"]))
+            d.cen = c_float3(*[float(np.float32(v)) for v in m["cen"]])
+            d.len = float(np.float32(m["len"]))
+            d.rad = float(np.float32(m["rad"]))
+            d.n_spawn = int(m.get("n_spawn", 0))
+            d.seed = int(m.get("seed", 0))
+            d.phase_mode = int(m.get("phase_mode", 0))
+        h = C.c_void_p()
+        check(self.lib.isf_batch_create(C.byref(h), n, arr), "isf_batch_create")
+        self.h = h.value
+        self.n = n
+        self.members = []
+        for i in range(n):
+            mh = C.c_void_p()
+            check(self.lib.isf_batch_member(self.h, i, C.byref(mh)), "isf_batch_member")
+            self.members.append(ISF._borrowed(mh.value))
+        self._bound = [None] * n
+
+    def bind(self, i, psi, particles=None):
+        nslots = particles.shape[0] if particles is not None else 0
+        check(self.lib.isf_batch_bind(self.h, int(i), _ptr(psi), _ptr(particles), nslots), "isf_batch_bind")
+        self._bound[i] = (psi, particles)  # keep the buffers alive while bound
+
+    def step_(self, iteration):
+        check(self.lib.isf_batch_step(self.h, int(iteration)), "isf_batch_step")
+
+    def step_sequential_(self, iteration):
+        check(self.lib.isf_batch_step_sequential(self.h, int(iteration)), "isf_batch_step_sequential")
+
+    def set_graph(self, use_graph):
+        check(self.lib.isf_batch_set_graph(self.h, int(bool(use_graph))), "isf_batch_set_graph")
+
+    def synchronize(self):
+        check(self.lib.isf_batch_synchronize(self.h), "isf_batch_synchronize")
+
+    def tune(self, cache_path=None):
+        check(self.lib.isf_batch_tune(self.h, cache_path.encode() if cache_path else None), "isf_batch_tune")
+
+    def tile(self, res):
+        ty = C.c_int()
+        check(self.lib.isf_batch_tile(self.h, c_int3(*[int(v) for v in res]), C.byref(ty)), "isf_batch_tile")
+        return ty.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            for m in self.members:
+                m.h = None
+            self.lib.isf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---- module-level functions with the reference's export names
 def schroedinger_flow_(isf: ISF, psi):
     return isf.schroedinger_flow_(psi)
@@ -274,5 +365,5 @@ def staggered_advect_(particles, isf: ISF):
     return isf.staggered_advect_(particles)
 
 
-__all__ = ["ISF", "DeviceArray", "ISFError", "schroedinger_flow_", "normalize_psi_", "pressure_project_",
+__all__ = ["ISF", "Batch", "DeviceArray", "ISFError", "schroedinger_flow_", "normalize_psi_", "pressure_project_",
            "velocity_one_form_", "isjet", "restrict_velocity_", "add_particles_", "staggered_advect_"]

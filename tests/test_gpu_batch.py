@@ -1,0 +1,151 @@
+"""Batched uneven ensembles (isf_batch_*): every member of a batched step equals the same
+member stepped on its own (isf_step), with and without hipGraph replay, for every x tile
+the autotuner can choose; one member is also checked against the FP64 oracle.
+
+Pins: batched == sequential per member (bit-identical: both run the same kernels on the
+same data, only the grid mapping differs; the contract allows 1e-6 relative L2), graph
+replay == direct launches on every replay, and every tile gives the same result within
+FP32 rounding (tiles are separate kernel instantiations whose floating-point contraction
+can differ: 1e-6 relative L2 on psi, 1e-5 on particle positions).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import JetConfig
+from _harness import oracle_run, particle_report, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SMALL_GRIDS = ((32, 32, 32), (64, 32, 32), (16, 32, 16))
+
+
+@pytest.fixture(scope="module")
+def ens():
+    from isf_amd import ensemble
+    return ensemble
+
+
+def members(ens, count, seed, grids=SMALL_GRIDS, n_slots=3000, n_spawn=60):
+    return ens.draw_jets(count=count, seed=seed, grids=grids, n_slots=n_slots, n_spawn=n_spawn)
+
+
+def state(b):
+    b.synchronize()
+    out = []
+    for psi, parts in b._bound:
+        out.append((psi.to_host(), parts.to_host() if parts is not None else None))
+    return out
+
+
+def assert_same(sa, sb, what):
+    for i, ((pa, qa), (pb, qb)) in enumerate(zip(sa, sb)):
+        e = rel_l2(pa, pb)
+        assert e <= 1e-6, f"{what}: member {i} psi rel L2 {e:.3e}"
+        assert np.array_equal(pa, pb), f"{what}: member {i} psi not bit-identical (rel L2 {e:.3e})"
+        if qa is not None:
+            assert np.array_equal(qa, qb), f"{what}: member {i} particles differ"
+
+
+def assert_close(sa, sb, what):
+    for i, ((pa, qa), (pb, qb)) in enumerate(zip(sa, sb)):
+        e = rel_l2(pa, pb)
+        assert e <= 1e-6, f"{what}: member {i} psi rel L2 {e:.3e}"
+        if qa is not None:
+            alive = np.all(qa > 0, axis=1) & np.all(qb > 0, axis=1)
+            assert int((np.all(qa > 0, axis=1) != np.all(qb > 0, axis=1)).sum()) <= 1, f"{what}: member {i}"
+            assert float(np.max(np.abs(qa[alive] - qb[alive]), initial=0.0)) < 1e-5, f"{what}: member {i}"
+
+
+def test_batched_equals_sequential(ens):
+    ms = members(ens, 7, seed=11)
+    print("[batch] shapes", [m["res"] for m in ms], flush=True)
+    a = ens.build_batch(ms, n_init=2000)
+    s = ens.build_batch(ms, n_init=2000)
+    assert_same(state(a), state(s), "after set-up")
+    for it in range(1, 6):
+        a.step_(it)
+        s.step_sequential_(it)
+    assert_same(state(a), state(s), "5 steps")
+    # the FP64 norm partials of the last projection belong to each member
+    for i in range(len(ms)):
+        na, ns = a.members[i].step_norm2(), s.members[i].step_norm2()
+        assert na == ns, (i, na, ns)
+        assert abs(na - np.prod(ms[i]["res"])) < 1e-5 * np.prod(ms[i]["res"])
+    a.close()
+    s.close()
+
+
+def test_graph_replays_match_direct_launches(ens):
+    ms = members(ens, 5, seed=12)
+    g = ens.build_batch(ms, n_init=1000)
+    d = ens.build_batch(ms, n_init=1000)
+    d.set_graph(False)
+    for it in range(1, 5):
+        g.step_(it)
+        d.step_(it)
+        # eager work on the member handles between replays, then compare every replay
+        for i in range(len(ms)):
+            g.members[i].norm2(g._bound[i][0])
+        assert_same(state(g), state(d), f"replay {it}")
+    # interleave a sequential step between replays: ordering across handles
+    g.step_sequential_(5)
+    d.step_sequential_(5)
+    g.step_(6)
+    d.step_(6)
+    assert_same(state(g), state(d), "mixed")
+    g.close()
+    d.close()
+
+
+def test_batched_member_matches_oracle(ens):
+    """The batched path against the FP64 oracle on the CPU-runnable jet (64x32x32)."""
+    cfg = JetConfig(n_spawn=50, n_slots=2000, n_init=0, constraint_iters=10)
+    f32 = np.float32
+    v = np.array(cfg.jet_velocity, f32)
+    hb = f32(cfg.hbar)
+    m = dict(res=cfg.dims, box=cfg.vol_size, hbar=hb, dt=f32(cfg.dt), kvec=tuple(v / hb),
+             omega=f32(np.sum(v ** f32(2)) / (f32(2) * hb)), cen=tuple(np.array(cfg.nozzle_cen).astype(f32)),
+             len=f32(cfg.nozzle_len), rad=f32(cfg.nozzle_rad), n_spawn=cfg.n_spawn, seed=cfg.seed,
+             phase_mode=cfg.phase_mode, n_slots=cfg.n_slots)
+    other = members(ens, 3, seed=13)
+    b = ens.build_batch([m] + other, n_init=0)
+    for it in range(1, 11):
+        b.step_(it)
+    psi, parts = state(b)[0]
+    o = oracle_run(cfg, np.float64)
+    o.step(10)
+    err = rel_l2(psi, o.psi)
+    rep = particle_report(parts, o.parts, cfg.vol_size)
+    print(f"  batched member vs FP64 oracle: psi rel L2 {err:.3e}; particles {rep}")
+    assert err < 1e-4
+    assert rep["status_mismatch"] <= 2 and rep["p99"] < 1e-4
+    b.close()
+
+
+def test_autotuned_tiles_give_identical_results(ens, tmp_path):
+    ms = members(ens, 4, seed=14, grids=((64, 32, 32),))
+    cache = str(tmp_path / "tiles.txt")
+    t = ens.build_batch(ms, n_init=500)
+    t.tune(cache)
+    ty = t.tile((64, 32, 32))
+    assert ty in (8, 16, 32)
+    lines = open(cache).read().split()
+    assert lines == ["64", "32", "32", "4", str(ty)]
+    t.close()
+    results = {}
+    for forced in (8, 16, 32):
+        path = str(tmp_path / f"forced{forced}.txt")
+        with open(path, "w") as f:
+            f.write(f"64 32 32 4 {forced}\n")
+        b = ens.build_batch(ms, n_init=500)
+        b.tune(path)
+        assert b.tile((64, 32, 32)) == forced
+        for it in range(1, 4):
+            b.step_(it)
+        results[forced] = state(b)
+        b.close()
+    assert_close(results[8], results[16], "tile 8 vs 16")
+    assert_close(results[8], results[32], "tile 8 vs 32")
+    assert os.path.getsize(cache) > 0
