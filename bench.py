@@ -105,6 +105,66 @@ def cpu_baseline(w, budget_s=12.0):
                       f"{n} steps in {el:.1f} s"}
 
 
+def bench_ensemble(steps, warmup, count=48, seed=20261203):
+    """Batched uneven ensemble: `count` jets on grids drawn from {32^3, 64x32x32, 64^3,
+    128x64x64} (box = grid/16), hbar in [0.05, 0.2], |v| in [0.5, 1.5] within 20 degrees of
+    +x, 10k particle slots each filled in the nozzle, 100 spawned per step.  The same
+    ensemble is stepped (a) batched: one hipGraph replay per ensemble step, and (b)
+    sequentially: one isf_step per member, both after the tile autotuner; wall-clock
+    time over `steps` ensemble steps after `warmup`, synchronised on both sides."""
+    from isf_amd.ensemble import build_batch, draw_jets
+    ms = draw_jets(count=count, seed=seed)
+    cells = sum(int(np.prod(m["res"])) for m in ms)
+    slots = sum(m["n_slots"] for m in ms)
+    t0 = time.perf_counter()
+    b = build_batch(ms)
+    b.tune(None)
+    b.synchronize()
+    setup_s = time.perf_counter() - t0
+    shapes = sorted({m["res"] for m in ms})
+    tiles = {"x".join(map(str, r)): b.tile(r) for r in shapes}
+    counts = {"x".join(map(str, r)): sum(1 for m in ms if m["res"] == r) for r in shapes}
+
+    def run(fn, it0):
+        it = it0
+        for _ in range(warmup):
+            it += 1
+            fn(it)
+        b.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            it += 1
+            fn(it)
+        b.synchronize()
+        return (time.perf_counter() - t) / steps, it
+
+    t_seq, it = run(b.step_sequential_, 0)
+    t_bat, it = run(b.step_, it)
+    norms = [b.members[i].step_norm2() / np.prod(ms[i]["res"]) for i in range(count)]
+    b.close()
+    balg = 248 * cells + 24 * slots
+    print(f"[bench] ensemble of {count}: batched {t_bat * 1e3:.3f} ms, sequential {t_seq * 1e3:.3f} ms per "
+          f"ensemble step (setup {setup_s:.1f} s)", file=sys.stderr, flush=True)
+    return {
+        "workload": f"{count} independent jets, grids {counts}, 10k particle slots each, 100 spawned per step",
+        "ensemble_steps_s": round(1.0 / t_bat, 2),
+        "ensemble_steps_s_sequential": round(1.0 / t_seq, 2),
+        "speedup_vs_sequential": round(t_seq / t_bat, 2),
+        "member_steps_s": round(count / t_bat, 1),
+        "ms_per_step": round(t_bat * 1e3, 4),
+        "ms_per_step_sequential": round(t_seq * 1e3, 4),
+        "cells": cells,
+        "B_alg_per_step": balg,
+        "achieved_GB_s": round(balg / t_bat / 1e9, 1),
+        "frac": round(balg / t_bat / 1e9 / HBM_PEAK_GBS, 4),
+        "x_tiles": tiles,
+        "norm2_over_N_range": [round(float(min(norms)), 9), round(float(max(norms)), 9)],
+        "steps": steps,
+        "warmup": warmup,
+        "timing": "wall clock around the steps, device synchronised on both sides",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +172,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--batch-steps", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -179,6 +241,9 @@ def main():
                           "frac": round(balg * steps_s / 1e9 / HBM_PEAK_GBS, 4)},
         "stages": stages,
     }
+    if not args.no_batch:
+        print("[bench] batched ensemble leg", file=sys.stderr, flush=True)
+        out["batched"] = bench_ensemble(args.batch_steps, 5)
     if not args.no_cpu:
         print("[bench] CPU baseline leg", file=sys.stderr, flush=True)
         cb = cpu_baseline(w)
