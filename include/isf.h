@@ -166,6 +166,72 @@ int isf_batch_tune(isf_batch_t b, const char* cache_path);
 /* The tile chosen for a grid shape (0 when none was tuned). */
 int isf_batch_tile(isf_batch_t b, const int res[3], int* ty);
 
+/* ---- vector-Jacobian products of the operators (no reference counterpart) ------------- */
+/* Each maps the cotangent of an operator's output to the cotangent of its input, under
+ * the real inner product <a, b> = sum Re(conj(a) b) over cells and components; the
+ * primal input is passed where the Jacobian depends on it.  In place on the cotangent
+ * unless stated. */
+int isf_vjp_schroedinger_flow(isf_t h, float* psibar);                        /* src/parallel.jl:176-187 */
+int isf_vjp_normalize(isf_t h, const float* psi_in, float* psibar);          /* src/parallel.jl:162-165 */
+int isf_vjp_pressure_project(isf_t h, const float* psi_in, float* psibar);   /* src/parallel.jl:155-160 */
+/* restrict_velocity with the configured nozzle (src/particles.jl:12-27); dphase (host,
+ * may be NULL) receives sum(phibar) and sum(phibar * pos_a), phibar the cotangent of the
+ * jet phase of each jet cell: dL/dk_a = sum(phibar pos_a) (phase_mode 0) or sum(phibar)
+ * (1), dL/d omgterm = -sum(phibar). */
+int isf_vjp_restrict_velocity(isf_t h, const float* psi_in, float* psibar, const float kvec[3], float omgterm,
+                              int phase_mode, double dphase[4]);
+/* velocity_one_form (src/parallel.jl:115-122, with 1/d if sharp): psibar += J^T velbar */
+int isf_vjp_velocity_one_form(isf_t h, const float* psi, float hbar, int sharp, const float* velbar, float* psibar);
+/* staggered_advect (src/parallel.jl:190-261) at positions parts0: pbar (cotangent of the
+ * advected positions) becomes that of parts0; velbar += the velocity cotangent. */
+int isf_vjp_staggered_advect(isf_t h, const float* parts0, long long nslots, const float* vel, float* pbar,
+                             float* velbar);
+
+/* ---- differentiable rollout (no reference counterpart) -------------------------------- */
+/* A rollout is the driver's set-up and simloop (test/runtests.jl:16-66) on a handle whose
+ * jet is configured (isf_jet_configure: nozzle, spawn count, seed, phase mode), made a
+ * function of differentiable inputs:
+ *   velocity  the nozzle velocity v (kvec = v / hbar, omega = |v|^2 / (2 hbar),
+ *             test/runtests.jl:23-24),
+ *   theta     a coarse initial phase field: psi0 = psi_init * exp(i Phi), Phi its periodic
+ *             trilinear upsampling to the grid (cell x at theta coordinate x pr / n),
+ *   psi_init  the initial wavefunction (default: (1, 0.01) normalized, :19-21),
+ *   particles the initial slot positions (n_init slots filled uniformly in the nozzle).
+ * Set-up: constraint_iters x (restrict with omega t = 0, pressure_project), :27-30; then
+ * `steps` simloop bodies; loss = mean over the splat grid of (rho - target)^2, rho(y) =
+ * (1 / nslots) sum over live slots of exp(-|y - p|^2 / (2 sigma^2)) for |y - p| <=
+ * cutoff sigma, y the centres of splat_res cells over the box.
+ * The backward pass runs the hand-written adjoint of every operator in reverse, with psi
+ * and particles checkpointed every checkpoint_every steps and each segment recomputed. */
+typedef struct {
+    int steps;                 /* simloop bodies per rollout, >= 1 */
+    int checkpoint_every;      /* K >= 1; K >= steps keeps the whole tape of one segment */
+    int constraint_iters;      /* >= 0 */
+    int phase_res[3];          /* theta grid (powers of two, each <= the grid); 0,0,0: none */
+    int splat_res[3];          /* density grid, each 1..1024 */
+    float splat_sigma;         /* physical units, > 0 */
+    float splat_cutoff;        /* in units of sigma, > 0 */
+    long long nslots;          /* particle slots, >= 0 */
+    long long n_init;          /* slots filled in the nozzle cylinder at set-up, <= nslots */
+    unsigned long long fill_seed;
+} isf_rollout_desc;
+typedef struct isf_rollout* isf_rollout_t;
+int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc);
+int isf_rollout_destroy(isf_rollout_t r);
+/* target density (device, splat_res cells, x fastest); copied */
+int isf_rollout_set_target(isf_rollout_t r, const float* target);
+/* velocity: host float[3]; theta: device (phase_res cells) or NULL (zero); psi_init:
+ * device psi or NULL (the default).  Writes the loss (FP64) after synchronising. */
+int isf_rollout_forward(isf_rollout_t r, const float velocity[3], const float* theta, const float* psi_init,
+                        double* loss);
+/* Gradients of the last forward's loss; any output may be NULL.  grad_velocity: host
+ * double[3]; grad_theta: device (phase_res cells); grad_psi_init: device psi layout;
+ * grad_parts_init: device (nslots x 3).  Synchronous. */
+int isf_rollout_backward(isf_rollout_t r, double grad_velocity[3], float* grad_theta, float* grad_psi_init,
+                         float* grad_parts_init);
+/* Device views of the last forward's final psi, particles and density (rho, FP64). */
+int isf_rollout_state(isf_rollout_t r, const float** psi, const float** parts, const double** rho);
+
 /* ---- timing helpers (HIP events on the handle's stream) ----------------------------- */
 /* Number and names of the kernel stages of isf_step, in launch order. */
 int isf_step_stage_count(void);

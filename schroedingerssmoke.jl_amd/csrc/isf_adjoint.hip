@@ -1,0 +1,627 @@
+// isf_adjoint.hip -- hand-written adjoints (transposed Jacobians) of the ISF step
+// operators, the set-up operators of a differentiable rollout and its particle-density
+// loss.  Complex cotangents follow the real inner product <a, b> = Re(conj(a) b) summed
+// over cells and components, so for an operator y = A(x) the adjoint maps ybar to
+// xbar = J^T ybar with <J dx, ybar> = <dx, xbar>.
+//
+// Forward operator -> adjoint used here:
+//   schroedinger_flow  F^-1 M F        -> F^-1 conj(M) F (plane pass PLANE_FLOW_ADJ)
+//   normalize  n = u / |u| (u in R^4)  -> ubar = (nbar - n (n . nbar)) / |u|
+//   gauge      psi' = psi exp(-i q)    -> psibar = psi'bar exp(+i q),
+//                                         qbar = sum_comp Im(conj(psi'bar) psi')
+//   Poisson    q = P f (P real, even)  -> fbar = P qbar (self-adjoint)
+//   div        f = sum_a (v_a(c) - v_a(c - e_a)) / d_a^2
+//                                      -> vbar_a(c) = (fbar(c) - fbar(c + e_a)) / d_a^2
+//   one-form   v_a(c) = s angle(z_a(c)), z_a(c) = sum_comp conj(psi(c)) psi(c + e_a)
+//                                      -> zbar = vbar s i z / |z|^2,
+//                                         psibar(c) += conj(zbar) psi(c + e_a),
+//                                         psibar(c + e_a) += zbar psi(c)
+//   restrict   psi_a <- |psi_a| e^{i phi} in the jet
+//                                      -> psibar_a = Re(e^{-i phi} psi'bar_a) psi_a / |psi_a|,
+//                                         phibar = sum_a |psi_a| Im(e^{-i phi} psi'bar_a)
+//   RK4 staggered advection            -> reverse sweep over the four stages with the
+//                                         trilinear weights scattered into velbar (atomics)
+#include <hip/hip_runtime.h>
+
+#include "isf_device.hpp"
+#include "isf_internal.hpp"
+
+namespace isf {
+
+#define ISF_LAUNCH(kernel, grid, block, stream, ...)                      \
+    do {                                                                  \
+        hipLaunchKernelGGL(kernel, grid, block, 0, stream, __VA_ARGS__);  \
+        return (int)hipGetLastError();                                    \
+    } while (0)
+
+static inline dim3 grid1d(size_t n, int nt) { return dim3((unsigned)((n + nt - 1) / nt)); }
+__host__ __device__ static inline size_t ncells(const GridInfo& g) { return (size_t)g.nx * g.ny * g.nz; }
+
+__device__ __forceinline__ size_t cidx(int x, int y, int z, const GridInfo& g) {
+    return (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
+}
+
+// ======================================================================== set-up
+
+__global__ void __launch_bounds__(256) k_fill_psi(float4* __restrict__ psi, float4 v, size_t N) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < N) psi[c] = v;
+}
+
+__global__ void __launch_bounds__(256) k_fill_f32(float* __restrict__ p, float v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_fill_f64(double* __restrict__ p, double v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Periodic trilinear upsampling of a coarse field theta (pr[0] x pr[1] x pr[2], x
+// fastest) at grid cell (x, y, z): theta coordinate s_a = x_a pr_a / n_a.
+struct PhaseGrid {
+    int pr[3];
+};
+__device__ __forceinline__ float phase_at(const float* __restrict__ th, int x, int y, int z, const GridInfo& g,
+                                          const PhaseGrid& P) {
+    const int c3[3] = {x, y, z}, n3[3] = {g.nx, g.ny, g.nz};
+    int i0[3], i1[3];
+    float f[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int r = n3[a] / P.pr[a];  // cells per coarse interval (powers of two)
+        i0[a] = c3[a] / r;
+        i1[a] = i0[a] + 1 == P.pr[a] ? 0 : i0[a] + 1;
+        f[a] = (float)(c3[a] - i0[a] * r) / (float)r;
+    }
+    auto T = [&](int a, int b, int c) { return th[a + P.pr[0] * (b + P.pr[1] * c)]; };
+    const float z0 = (1.0f - f[0]) * ((1.0f - f[1]) * T(i0[0], i0[1], i0[2]) + f[1] * T(i0[0], i1[1], i0[2])) +
+                     f[0] * ((1.0f - f[1]) * T(i1[0], i0[1], i0[2]) + f[1] * T(i1[0], i1[1], i0[2]));
+    const float z1 = (1.0f - f[0]) * ((1.0f - f[1]) * T(i0[0], i0[1], i1[2]) + f[1] * T(i0[0], i1[1], i1[2])) +
+                     f[0] * ((1.0f - f[1]) * T(i1[0], i0[1], i1[2]) + f[1] * T(i1[0], i1[1], i1[2]));
+    return (1.0f - f[2]) * z0 + f[2] * z1;
+}
+
+// psi <- psi exp(i Phi), Phi = upsampled theta
+__global__ void __launch_bounds__(256)
+k_apply_phase(float4* __restrict__ psi, const float* __restrict__ th, GridInfo g, PhaseGrid P) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells(g)) return;
+    int x, y, z;
+    cell_xyz(c, g, x, y, z);
+    float s, co;
+    sincosf(phase_at(th, x, y, z, g, P), &s, &co);
+    const float4 p = psi[c];
+    psi[c] = make_float4(p.x * co - p.y * s, p.x * s + p.y * co, p.z * co - p.w * s, p.z * s + p.w * co);
+}
+
+// Adjoint of k_apply_phase at psi' (the phased field): phibar = -sum Im(conj(psi'bar) psi'),
+// psibar <- psi'bar exp(-i Phi).
+__global__ void __launch_bounds__(256)
+k_apply_phase_adj(float4* __restrict__ psibar, const float4* __restrict__ psi_out, const float* __restrict__ th,
+                  float* __restrict__ phibar, GridInfo g, PhaseGrid P) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells(g)) return;
+    int x, y, z;
+    cell_xyz(c, g, x, y, z);
+    const float4 b = psibar[c], p = psi_out[c];
+    phibar[c] = -((b.x * p.y - b.y * p.x) + (b.z * p.w - b.w * p.z));
+    float s, co;
+    sincosf(phase_at(th, x, y, z, g, P), &s, &co);
+    psibar[c] = make_float4(b.x * co + b.y * s, b.y * co - b.x * s, b.z * co + b.w * s, b.w * co - b.z * s);
+}
+
+// thetabar = U^T phibar: each coarse node gathers the cells of its support (weight
+// (1 - |s - node|) per axis), a fixed summation order.
+__global__ void __launch_bounds__(64)
+k_phase_gather(const float* __restrict__ phibar, float* __restrict__ thbar, GridInfo g, PhaseGrid P) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    const int M = P.pr[0] * P.pr[1] * P.pr[2];
+    if (node >= M) return;
+    const int m3[3] = {node % P.pr[0], (node / P.pr[0]) % P.pr[1], node / (P.pr[0] * P.pr[1])};
+    const int n3[3] = {g.nx, g.ny, g.nz};
+    int r[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) r[a] = n3[a] / P.pr[a];
+    double acc = 0.0;
+    for (int dz = -r[2] + 1; dz < r[2]; dz++) {
+        const int z = (m3[2] * r[2] + dz + n3[2]) % n3[2];
+        const float wz = 1.0f - (float)abs(dz) / (float)r[2];
+        for (int dy = -r[1] + 1; dy < r[1]; dy++) {
+            const int y = (m3[1] * r[1] + dy + n3[1]) % n3[1];
+            const float wy = 1.0f - (float)abs(dy) / (float)r[1];
+            for (int dx = -r[0] + 1; dx < r[0]; dx++) {
+                const int x = (m3[0] * r[0] + dx + n3[0]) % n3[0];
+                const float wx = 1.0f - (float)abs(dx) / (float)r[0];
+                acc += (double)(wx * wy * wz) * (double)phibar[cidx(x, y, z, g)];
+            }
+        }
+    }
+    thbar[node] = (float)acc;
+}
+
+// ======================================================================== out-of-place forms
+
+__global__ void __launch_bounds__(256) k_normalize_to(const float4* __restrict__ in, float4* __restrict__ out, size_t N) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < N) out[c] = normalized(in[c]);
+}
+
+__global__ void __launch_bounds__(256)
+k_restrict_to(const float4* __restrict__ in, float4* __restrict__ out, GridInfo g, JetParams j) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells(g)) return;
+    int x, y, z;
+    cell_xyz(c, g, x, y, z);
+    const float4 p = in[c];
+    out[c] = is_jet_cell(x, y, z, g, j) ? jet_phase(p, x, y, z, g, j) : p;
+}
+
+// ======================================================================== adjoints
+
+// qbar(c) = sum_comp Im(conj(psibar) psi_out): the cotangent of q of a gauge transform
+// psi_out = psi exp(-i q).
+__global__ void __launch_bounds__(256)
+k_gauge_qbar(const float4* __restrict__ psibar, const float4* __restrict__ psi_out, float* __restrict__ qbar, size_t N) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const float4 b = psibar[c], p = psi_out[c];
+    qbar[c] = (b.x * p.y - b.y * p.x) + (b.z * p.w - b.w * p.z);
+}
+
+enum { OF_DIV = 0, OF_VEL = 1 };
+
+// zbar = w i z / |z|^2 for edge weight w (the cotangent of angle(z) times its scale)
+__device__ __forceinline__ float2 angle_zbar(float re, float im, float w) {
+    const float m2 = re * re + im * im;
+    const float s = m2 > 0.0f ? w / m2 : 0.0f;
+    return make_float2(-im * s, re * s);
+}
+__device__ __forceinline__ void edge_z(float4 a, float4 b, float& re, float& im) {
+    re = (a.x * b.x + a.y * b.y) + (a.z * b.z + a.w * b.w);
+    im = (a.x * b.y - a.y * b.x) + (a.z * b.w - a.w * b.z);
+}
+// conj(zb) * b added to acc (componentwise over the two wavefunctions)
+__device__ __forceinline__ void add_conj_mul(float4& acc, float2 zb, float4 b) {
+    acc.x += zb.x * b.x + zb.y * b.y;
+    acc.y += zb.x * b.y - zb.y * b.x;
+    acc.z += zb.x * b.z + zb.y * b.w;
+    acc.w += zb.x * b.w - zb.y * b.z;
+}
+// zb * a added to acc
+__device__ __forceinline__ void add_mul(float4& acc, float2 zb, float4 a) {
+    acc.x += zb.x * a.x - zb.y * a.y;
+    acc.y += zb.x * a.y + zb.y * a.x;
+    acc.z += zb.x * a.z - zb.y * a.w;
+    acc.w += zb.x * a.w + zb.y * a.z;
+}
+
+// Adjoint of a one-form evaluated at psi, fused with the adjoint of what consumed it:
+//   OF_DIV: div then a Poisson solve produced q; w = fbar (Poisson-solved qbar),
+//           edge weight (fbar(c) - fbar(c + e_a)) / d_a^2, hbar = 1 (src/parallel.jl:156)
+//   OF_VEL: velocity_one_form(hbar), with 1/d when sharp (test/runtests.jl:59-62); w =
+//           velbar (3 per cell), edge weight velbar_a(c) hbar (/ d_a)
+// psibar(c) <- rot(psibar(c), +q(c)) (the gauge's own adjoint, when q != nullptr) plus the
+// one-form contributions of the six edges that touch c.
+template <int MODE>
+__global__ void __launch_bounds__(256)
+k_oneform_adj(float4* __restrict__ psibar, const float4* __restrict__ psi, const float* __restrict__ q,
+              const float* __restrict__ w, GridInfo g, float hbar, int sharp) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells(g)) return;
+    int x, y, z;
+    cell_xyz(c, g, x, y, z);
+    const int n3[3] = {g.nx, g.ny, g.nz};
+    const float d[3] = {g.dx, g.dy, g.dz};
+    float4 acc = psibar[c];
+    if (q) {
+        float s, co;
+        sincosf(q[c], &s, &co);
+        acc = make_float4(acc.x * co - acc.y * s, acc.x * s + acc.y * co, acc.z * co - acc.w * s, acc.z * s + acc.w * co);
+    }
+    const float4 pc = psi[c];
+    const float wc = MODE == OF_DIV ? w[c] : 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int cp[3] = {x, y, z}, cm[3] = {x, y, z};
+        cp[a] = cp[a] + 1 == n3[a] ? 0 : cp[a] + 1;
+        cm[a] = cm[a] == 0 ? n3[a] - 1 : cm[a] - 1;
+        const size_t ip = cidx(cp[0], cp[1], cp[2], g), im = cidx(cm[0], cm[1], cm[2], g);
+        const float4 pp = psi[ip], pm = psi[im];
+        float wf, wb;  // weights of the forward edge (c, c+e_a) and the backward edge (c-e_a, c)
+        if (MODE == OF_DIV) {
+            const float id2 = 1.0f / (d[a] * d[a]);
+            wf = (wc - w[ip]) * id2;
+            wb = (w[im] - wc) * id2;
+        } else {
+            const float sc = sharp ? hbar / d[a] : hbar;
+            wf = w[3 * c + a] * sc;
+            wb = w[3 * im + a] * sc;
+        }
+        float re, imz;
+        edge_z(pc, pp, re, imz);
+        add_conj_mul(acc, angle_zbar(re, imz, wf), pp);  // c is the first end of (c, c+e_a)
+        edge_z(pm, pc, re, imz);
+        add_mul(acc, angle_zbar(re, imz, wb), pm);       // c is the second end of (c-e_a, c)
+    }
+    psibar[c] = acc;
+}
+
+// Adjoint of restrict_velocity (jet cells) at its input psi_in, in place on psibar; per
+// block FP64 partial sums of phibar and phibar * pos_a (4 per block) for the gradient with
+// respect to kvec and omega t.
+__global__ void __launch_bounds__(256)
+k_restrict_adj(float4* __restrict__ psibar, const float4* __restrict__ psi_in, GridInfo g, JetParams j,
+               double* __restrict__ partials) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (c < ncells(g)) {
+        int x, y, z;
+        cell_xyz(c, g, x, y, z);
+        if (is_jet_cell(x, y, z, g, j)) {
+            const float4 p = psi_in[c], b = psibar[c];
+            const float4 o = jet_phase(p, x, y, z, g, j);
+            const float a1 = hypotf(p.x, p.y), a2 = hypotf(p.z, p.w);
+            // e^{i phi} recovered from the output (a_k > 0 in practice; guarded)
+            const float e1r = a1 > 0.0f ? o.x / a1 : 0.0f, e1i = a1 > 0.0f ? o.y / a1 : 0.0f;
+            const float e2r = a2 > 0.0f ? o.z / a2 : 0.0f, e2i = a2 > 0.0f ? o.w / a2 : 0.0f;
+            const float r1 = b.x * e1r + b.y * e1i, r2 = b.z * e2r + b.w * e2i;  // Re(conj(e) bar)
+            const float phib = a1 * (b.y * e1r - b.x * e1i) + a2 * (b.w * e2r - b.z * e2i);
+            const float i1 = a1 > 0.0f ? 1.0f / a1 : 0.0f, i2 = a2 > 0.0f ? 1.0f / a2 : 0.0f;
+            psibar[c] = make_float4(r1 * p.x * i1, r1 * p.y * i1, r2 * p.z * i2, r2 * p.w * i2);
+            const float px = __fmul_rn((float)x, g.dx), py = __fmul_rn((float)y, g.dy), pz = __fmul_rn((float)z, g.dz);
+            s0 = phib;
+            s1 = (double)phib * px;
+            s2 = (double)phib * py;
+            s3 = (double)phib * pz;
+        }
+    }
+    s0 = block_sum<256>(s0);
+    s1 = block_sum<256>(s1);
+    s2 = block_sum<256>(s2);
+    s3 = block_sum<256>(s3);
+    if (threadIdx.x == 0) {
+        double* o = partials + 4 * (size_t)blockIdx.x;
+        o[0] = s0;
+        o[1] = s1;
+        o[2] = s2;
+        o[3] = s3;
+    }
+}
+
+// out[k] += sum over blocks of partials[4 b + k], k < 4 (one block, fixed order)
+__global__ void __launch_bounds__(256) k_reduce4(const double* __restrict__ partials, size_t nblocks, double* __restrict__ out) {
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (size_t b = threadIdx.x; b < nblocks; b += 256)
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[k] += partials[4 * b + k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double s = block_sum<256>(a[k]);
+        if (threadIdx.x == 0) out[k] += s;
+    }
+}
+
+// Adjoint of normalize at its input u: ubar = (nbar - n (n . nbar)) / |u|, in place.
+__global__ void __launch_bounds__(256)
+k_normalize_adj(float4* __restrict__ psibar, const float4* __restrict__ psi_in, size_t N) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const float4 u = psi_in[c], b = psibar[c];
+    const float inv = 1.0f / sqrtf((u.x * u.x + u.y * u.y) + (u.z * u.z + u.w * u.w));
+    const float4 n = make_float4(u.x * inv, u.y * inv, u.z * inv, u.w * inv);
+    const float dn = (n.x * b.x + n.y * b.y) + (n.z * b.z + n.w * b.w);
+    psibar[c] = make_float4((b.x - n.x * dn) * inv, (b.y - n.y * dn) * inv, (b.z - n.z * dn) * inv, (b.w - n.w * dn) * inv);
+}
+
+// ---- particles
+
+// Staggered interpolation at x: u (3) and, for the adjoint, the position Jacobian terms and
+// the 12 (cell, component) weights.
+struct StagEval {
+    int i[3], ip[3];
+    float w[3];
+    float v0[3], vn[3], pp[3], pp2[3];
+};
+__device__ __forceinline__ void stag_eval(const float* __restrict__ vel, const float x[3], const GridInfo& g, StagEval& e,
+                                          float u[3]) {
+    stag_index(x, g, e.i, e.ip, e.w);
+    const int *i = e.i, *ip = e.ip;
+    auto V = [&](int ix, int iy, int iz, int comp) { return vel[3 * cidx(ix, iy, iz, g) + comp]; };
+    e.v0[0] = V(i[0], i[1], i[2], 0);
+    e.v0[1] = V(i[0], i[1], i[2], 1);
+    e.v0[2] = V(i[0], i[1], i[2], 2);
+    e.vn[0] = V(i[0], ip[1], ip[2], 0);
+    e.vn[1] = V(ip[0], i[1], ip[2], 1);
+    e.vn[2] = V(ip[0], ip[1], i[2], 2);
+    e.pp[0] = V(i[0], ip[1], i[2], 0);
+    e.pp[1] = V(ip[0], i[1], i[2], 1);
+    e.pp[2] = V(ip[0], i[1], i[2], 2);
+    e.pp2[0] = V(i[0], i[1], ip[2], 0);
+    e.pp2[1] = V(i[0], i[1], ip[2], 1);
+    e.pp2[2] = V(i[0], ip[1], i[2], 2);
+    const float w1[3] = {e.w[2], e.w[2], e.w[1]}, w2[3] = {e.w[1], e.w[0], e.w[0]};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        u[a] = (1.0f - w1[a]) * ((1.0f - w2[a]) * e.v0[a] + w2[a] * e.pp[a]) +
+               w1[a] * ((1.0f - w2[a]) * e.pp2[a] + w2[a] * e.vn[a]);
+}
+
+// ubar at the evaluation point: position cotangent xbar (3) and scatter into velbar.
+__device__ __forceinline__ void stag_adj(const StagEval& e, const float ub[3], const GridInfo& g, float* __restrict__ velbar,
+                                         float xb[3]) {
+    const float w1[3] = {e.w[2], e.w[2], e.w[1]}, w2[3] = {e.w[1], e.w[0], e.w[0]};
+    float dw1[3], dw2[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        dw1[a] = -((1.0f - w2[a]) * e.v0[a] + w2[a] * e.pp[a]) + ((1.0f - w2[a]) * e.pp2[a] + w2[a] * e.vn[a]);
+        dw2[a] = (1.0f - w1[a]) * (e.pp[a] - e.v0[a]) + w1[a] * (e.vn[a] - e.pp2[a]);
+    }
+    // w1 = (w_z, w_z, w_y), w2 = (w_y, w_x, w_x)
+    xb[0] = ub[1] * dw2[1] + ub[2] * dw2[2];
+    xb[1] = ub[0] * dw2[0] + ub[2] * dw1[2];
+    xb[2] = ub[0] * dw1[0] + ub[1] * dw1[1];
+    const int *i = e.i, *ip = e.ip;
+    const size_t c000 = cidx(i[0], i[1], i[2], g);
+    const size_t cv[3][3] = {// (pp, pp2, vn) cells per component
+                             {cidx(i[0], ip[1], i[2], g), cidx(i[0], i[1], ip[2], g), cidx(i[0], ip[1], ip[2], g)},
+                             {cidx(ip[0], i[1], i[2], g), cidx(i[0], i[1], ip[2], g), cidx(ip[0], i[1], ip[2], g)},
+                             {cidx(ip[0], i[1], i[2], g), cidx(i[0], ip[1], i[2], g), cidx(ip[0], ip[1], i[2], g)}};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float u = ub[a];
+        unsafeAtomicAdd(velbar + 3 * c000 + a, u * (1.0f - w1[a]) * (1.0f - w2[a]));
+        unsafeAtomicAdd(velbar + 3 * cv[a][0] + a, u * (1.0f - w1[a]) * w2[a]);
+        unsafeAtomicAdd(velbar + 3 * cv[a][1] + a, u * w1[a] * (1.0f - w2[a]));
+        unsafeAtomicAdd(velbar + 3 * cv[a][2] + a, u * w1[a] * w2[a]);
+    }
+}
+
+// Adjoint of staggered_advect at the slot positions before the step (parts0): pbar (the
+// cotangent of the new positions) becomes the cotangent of parts0; velbar accumulates.
+__global__ void __launch_bounds__(256)
+k_advect_adj(const float* __restrict__ parts0, long long nslots, const float* __restrict__ vel, GridInfo g, float dt,
+             float* __restrict__ pbar, float* __restrict__ velbar) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const float p[3] = {parts0[3 * s], parts0[3 * s + 1], parts0[3 * s + 2]};
+    const bool in = (p[0] > 0.0f) & (p[0] < g.Lx) & (p[1] > 0.0f) & (p[1] < g.Ly) & (p[2] > 0.0f) & (p[2] < g.Lz);
+    if (!in) {  // the dead marker does not depend on anything
+        pbar[3 * s] = 0.0f;
+        pbar[3 * s + 1] = 0.0f;
+        pbar[3 * s + 2] = 0.0f;
+        return;
+    }
+    StagEval e1, e2, e3, e4;
+    float k1[3], k2[3], k3[3], k4[3], t[3];
+    stag_eval(vel, p, g, e1, k1);
+#pragma unroll
+    for (int a = 0; a < 3; a++) t[a] = p[a] + k1[a] * dt * 0.5f;
+    stag_eval(vel, t, g, e2, k2);
+#pragma unroll
+    for (int a = 0; a < 3; a++) t[a] = p[a] + k2[a] * dt * 0.5f;
+    stag_eval(vel, t, g, e3, k3);
+#pragma unroll
+    for (int a = 0; a < 3; a++) t[a] = p[a] + k3[a] * dt;
+    stag_eval(vel, t, g, e4, k4);
+    const float pb[3] = {pbar[3 * s], pbar[3 * s + 1], pbar[3 * s + 2]};
+    const float c6 = dt / 6.0f;
+    float acc[3], kb1[3], kb2[3], kb3[3], kb4[3], xb[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        acc[a] = pb[a];
+        kb1[a] = c6 * pb[a];
+        kb2[a] = 2.0f * c6 * pb[a];
+        kb3[a] = 2.0f * c6 * pb[a];
+        kb4[a] = c6 * pb[a];
+    }
+    stag_adj(e4, kb4, g, velbar, xb);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        acc[a] += xb[a];
+        kb3[a] += dt * xb[a];
+    }
+    stag_adj(e3, kb3, g, velbar, xb);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        acc[a] += xb[a];
+        kb2[a] += dt * 0.5f * xb[a];
+    }
+    stag_adj(e2, kb2, g, velbar, xb);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        acc[a] += xb[a];
+        kb1[a] += dt * 0.5f * xb[a];
+    }
+    stag_adj(e1, kb1, g, velbar, xb);
+#pragma unroll
+    for (int a = 0; a < 3; a++) pbar[3 * s + a] = acc[a] + xb[a];
+}
+
+// Slots overwritten by add_particles! (test/runtests.jl:52-54) carry no cotangent back.
+__global__ void __launch_bounds__(256)
+k_zero_slots(float* __restrict__ pbar, long long nslots, long long first, long long n) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const long long s = (first + k) % nslots;
+    pbar[3 * s] = 0.0f;
+    pbar[3 * s + 1] = 0.0f;
+    pbar[3 * s + 2] = 0.0f;
+}
+
+// ======================================================================== splat loss
+
+// Particle density on a grid of cell centres y = (j + 1/2) h over the box:
+// rho(y) = inv_n * sum over live slots of exp(-|y - p|^2 / (2 sigma^2)) for |y - p| <= R.
+__device__ __forceinline__ bool splat_live(const float p[3], const SplatCfg& s) {
+    return (p[0] > 0.0f) & (p[0] < s.L[0]) & (p[1] > 0.0f) & (p[1] < s.L[1]) & (p[2] > 0.0f) & (p[2] < s.L[2]);
+}
+__device__ __forceinline__ void splat_range(float p, float h, float R, int n, int& lo, int& hi) {
+    lo = (int)ceilf((p - R) / h - 0.5f);
+    hi = (int)floorf((p + R) / h - 0.5f);
+    lo = lo < 0 ? 0 : lo;
+    hi = hi > n - 1 ? n - 1 : hi;
+}
+
+__global__ void __launch_bounds__(256)
+k_splat(const float* __restrict__ parts, long long nslots, SplatCfg sc, double* __restrict__ rho) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const float p[3] = {parts[3 * s], parts[3 * s + 1], parts[3 * s + 2]};
+    if (!splat_live(p, sc)) return;
+    int lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) splat_range(p[a], sc.h[a], sc.R, sc.res[a], lo[a], hi[a]);
+    const float R2 = sc.R * sc.R, k = -0.5f / (sc.sigma * sc.sigma);
+    for (int jz = lo[2]; jz <= hi[2]; jz++) {
+        const float dz = ((float)jz + 0.5f) * sc.h[2] - p[2];
+        for (int jy = lo[1]; jy <= hi[1]; jy++) {
+            const float dy = ((float)jy + 0.5f) * sc.h[1] - p[1];
+            const float r2yz = dy * dy + dz * dz;
+            if (r2yz > R2) continue;
+            for (int jx = lo[0]; jx <= hi[0]; jx++) {
+                const float dx = ((float)jx + 0.5f) * sc.h[0] - p[0];
+                const float r2 = dx * dx + r2yz;
+                if (r2 > R2) continue;
+                const size_t j = (size_t)jx + (size_t)sc.res[0] * ((size_t)jy + (size_t)sc.res[1] * jz);
+                unsafeAtomicAdd(rho + j, (double)(__expf(r2 * k) * sc.inv_n));
+            }
+        }
+    }
+}
+
+// resid = 2/M (rho - target) (the loss cotangent of rho); per-block FP64 sums of (rho - target)^2
+__global__ void __launch_bounds__(256)
+k_splat_loss(const double* __restrict__ rho, const float* __restrict__ target, int M, float* __restrict__ resid,
+             double* __restrict__ partials) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double e2 = 0.0;
+    if (j < M) {
+        const double e = rho[j] - (double)target[j];
+        resid[j] = (float)(2.0 * e / (double)M);
+        e2 = e * e;
+    }
+    const double s = block_sum<256>(e2);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// pbar = d loss / d p = sum_y resid(y) inv_n G(y - p) (y - p) / sigma^2 over the splat support
+__global__ void __launch_bounds__(256)
+k_splat_adj(const float* __restrict__ parts, long long nslots, SplatCfg sc, const float* __restrict__ resid,
+            float* __restrict__ pbar) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const float p[3] = {parts[3 * s], parts[3 * s + 1], parts[3 * s + 2]};
+    float g[3] = {0.0f, 0.0f, 0.0f};
+    if (splat_live(p, sc)) {
+        int lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) splat_range(p[a], sc.h[a], sc.R, sc.res[a], lo[a], hi[a]);
+        const float R2 = sc.R * sc.R, k = -0.5f / (sc.sigma * sc.sigma), is2 = 1.0f / (sc.sigma * sc.sigma);
+        for (int jz = lo[2]; jz <= hi[2]; jz++) {
+            const float dz = ((float)jz + 0.5f) * sc.h[2] - p[2];
+            for (int jy = lo[1]; jy <= hi[1]; jy++) {
+                const float dy = ((float)jy + 0.5f) * sc.h[1] - p[1];
+                const float r2yz = dy * dy + dz * dz;
+                if (r2yz > R2) continue;
+                for (int jx = lo[0]; jx <= hi[0]; jx++) {
+                    const float dx = ((float)jx + 0.5f) * sc.h[0] - p[0];
+                    const float r2 = dx * dx + r2yz;
+                    if (r2 > R2) continue;
+                    const size_t j = (size_t)jx + (size_t)sc.res[0] * ((size_t)jy + (size_t)sc.res[1] * jz);
+                    const float f = resid[j] * sc.inv_n * __expf(r2 * k) * is2;
+                    g[0] += f * dx;
+                    g[1] += f * dy;
+                    g[2] += f * dz;
+                }
+            }
+        }
+    }
+    pbar[3 * s] = g[0];
+    pbar[3 * s + 1] = g[1];
+    pbar[3 * s + 2] = g[2];
+}
+
+// ======================================================================== launchers
+
+int launch_fill_psi(float4* psi, float4 v, size_t n, hipStream_t s) {
+    ISF_LAUNCH(k_fill_psi, grid1d(n, 256), dim3(256), s, psi, v, n);
+}
+int launch_fill_f32(float* p, float v, size_t n, hipStream_t s) {
+    if (n == 0) return 0;
+    ISF_LAUNCH(k_fill_f32, grid1d(n, 256), dim3(256), s, p, v, n);
+}
+int launch_fill_f64(double* p, double v, size_t n, hipStream_t s) {
+    if (n == 0) return 0;
+    ISF_LAUNCH(k_fill_f64, grid1d(n, 256), dim3(256), s, p, v, n);
+}
+int launch_apply_phase(const GridInfo& g, float4* psi, const float* theta, const int pr[3], hipStream_t s) {
+    const PhaseGrid P{{pr[0], pr[1], pr[2]}};
+    ISF_LAUNCH(k_apply_phase, grid1d(ncells(g), 256), dim3(256), s, psi, theta, g, P);
+}
+int launch_apply_phase_adj(const GridInfo& g, float4* psibar, const float4* psi_out, const float* theta, float* phibar,
+                           const int pr[3], hipStream_t s) {
+    const PhaseGrid P{{pr[0], pr[1], pr[2]}};
+    ISF_LAUNCH(k_apply_phase_adj, grid1d(ncells(g), 256), dim3(256), s, psibar, psi_out, theta, phibar, g, P);
+}
+int launch_phase_gather(const GridInfo& g, const float* phibar, float* thbar, const int pr[3], hipStream_t s) {
+    const PhaseGrid P{{pr[0], pr[1], pr[2]}};
+    ISF_LAUNCH(k_phase_gather, grid1d((size_t)pr[0] * pr[1] * pr[2], 64), dim3(64), s, phibar, thbar, g, P);
+}
+int launch_normalize_to(const GridInfo& g, const float4* in, float4* out, hipStream_t s) {
+    ISF_LAUNCH(k_normalize_to, grid1d(ncells(g), 256), dim3(256), s, in, out, ncells(g));
+}
+int launch_restrict_to(const GridInfo& g, const float4* in, float4* out, const JetParams& j, hipStream_t s) {
+    ISF_LAUNCH(k_restrict_to, grid1d(ncells(g), 256), dim3(256), s, in, out, g, j);
+}
+int launch_gauge_qbar(const GridInfo& g, const float4* psibar, const float4* psi_out, float* qbar, hipStream_t s) {
+    ISF_LAUNCH(k_gauge_qbar, grid1d(ncells(g), 256), dim3(256), s, psibar, psi_out, qbar, ncells(g));
+}
+int launch_oneform_adj_div(const GridInfo& g, float4* psibar, const float4* psi, const float* q, const float* fbar,
+                           hipStream_t s) {
+    ISF_LAUNCH(k_oneform_adj<OF_DIV>, grid1d(ncells(g), 256), dim3(256), s, psibar, psi, q, fbar, g, 1.0f, 0);
+}
+int launch_oneform_adj_vel(const GridInfo& g, float4* psibar, const float4* psi, const float* velbar, float hbar,
+                           int sharp, hipStream_t s) {
+    ISF_LAUNCH(k_oneform_adj<OF_VEL>, grid1d(ncells(g), 256), dim3(256), s, psibar, psi, nullptr, velbar, g, hbar, sharp);
+}
+size_t restrict_adj_blocks(const GridInfo& g) { return (ncells(g) + 255) / 256; }
+int launch_restrict_adj(const GridInfo& g, float4* psibar, const float4* psi_in, const JetParams& j, double* partials,
+                        double* out4, hipStream_t s) {
+    hipLaunchKernelGGL(k_restrict_adj, grid1d(ncells(g), 256), dim3(256), 0, s, psibar, psi_in, g, j, partials);
+    int e = (int)hipGetLastError();
+    if (e) return e;
+    ISF_LAUNCH(k_reduce4, dim3(1), dim3(256), s, partials, restrict_adj_blocks(g), out4);
+}
+int launch_normalize_adj(const GridInfo& g, float4* psibar, const float4* psi_in, hipStream_t s) {
+    ISF_LAUNCH(k_normalize_adj, grid1d(ncells(g), 256), dim3(256), s, psibar, psi_in, ncells(g));
+}
+int launch_advect_adj(const GridInfo& g, const float* parts0, long long nslots, const float* vel, float dt, float* pbar,
+                      float* velbar, hipStream_t s) {
+    if (nslots <= 0) return 0;
+    ISF_LAUNCH(k_advect_adj, grid1d((size_t)nslots, 256), dim3(256), s, parts0, nslots, vel, g, dt, pbar, velbar);
+}
+int launch_zero_slots(float* pbar, long long nslots, long long first, long long n, hipStream_t s) {
+    if (n <= 0 || nslots <= 0) return 0;
+    ISF_LAUNCH(k_zero_slots, grid1d((size_t)n, 256), dim3(256), s, pbar, nslots, first, n);
+}
+int launch_splat(const float* parts, long long nslots, const SplatCfg& sc, double* rho, hipStream_t s) {
+    if (nslots <= 0) return 0;
+    ISF_LAUNCH(k_splat, grid1d((size_t)nslots, 256), dim3(256), s, parts, nslots, sc, rho);
+}
+size_t splat_loss_blocks(int M) { return ((size_t)M + 255) / 256; }
+int launch_splat_loss(const double* rho, const float* target, int M, float* resid, double* partials, double* out,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_splat_loss, grid1d((size_t)M, 256), dim3(256), 0, s, rho, target, M, resid, partials);
+    int e = (int)hipGetLastError();
+    if (e) return e;
+    return launch_reduce_sum(partials, splat_loss_blocks(M), out, s);
+}
+int launch_splat_adj(const float* parts, long long nslots, const SplatCfg& sc, const float* resid, float* pbar,
+                     hipStream_t s) {
+    if (nslots <= 0) return 0;
+    ISF_LAUNCH(k_splat_adj, grid1d((size_t)nslots, 256), dim3(256), s, parts, nslots, sc, resid, pbar);
+}
+
+}  // namespace isf

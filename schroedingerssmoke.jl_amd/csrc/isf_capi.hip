@@ -14,13 +14,15 @@
 #include <vector>
 
 #include "../../include/isf.h"
+#include "isf_handle.hpp"
 #include "isf_internal.hpp"
 
 using namespace isf;
 
+namespace isf {
 static thread_local std::string g_err;
 
-static int fail(int code, const char* fmt, ...) {
+int fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -29,70 +31,16 @@ static int fail(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
-static int hip_fail(hipError_t e, const char* what) {
+int hip_fail(hipError_t e, const char* what) {
     return fail(ISF_EHIP, "%s: %s", what, hipGetErrorString(e));
 }
-#define HIP_CHECK(expr, what)                          \
-    do {                                               \
-        hipError_t e_ = (expr);                        \
-        if (e_ != hipSuccess) return hip_fail(e_, what); \
-    } while (0)
-#define LAUNCH_CHECK(expr, what)                                      \
-    do {                                                              \
-        int e_ = (expr);                                              \
-        if (e_ != 0) return hip_fail((hipError_t)e_, what);           \
-    } while (0)
-
-static constexpr uint32_t kMagic = 0x15F0CAFEu;
-
-struct isf_handle {
-    uint32_t magic;
-    int device;
-    GridInfo g;
-    float hbar, dt;
-    size_t N;
-    hipStream_t stream;   // the handle's stream: every operator, the grid part of isf_step
-    hipStream_t pstream;  // particle side of isf_step (spawn, order, advect), joined by events
-    hipEvent_t ev0, ev1;
-    hipEvent_t ev_vel, ev_adv;
-    bool side_pending;    // an advect on pstream may still read vel / write particles
-    hipEvent_t ext_ev;    // recorded on the handle's stream when a batched step must wait for it
-    hipEvent_t batch_ev;  // (borrowed) end of the last batched step that used this handle's data
-    bool ext_pending;     // the handle's streams must wait for batch_ev before new work
-    unsigned long long seq;  // bumped by every call that enqueues work on the handle's streams
-    const void* owner;       // the batch that owns this handle (isf_destroy refuses it)
-    void* table_mem;
-    DevTables tb;
-    float2* S;          // Schroedinger spectrum, 2N complex; f and P alias it in projections
-    float* vel;         // velocity field, 3N floats
-    double* norm_parts; // per-block FP64 partial sums
-    size_t norm_nparts;
-    double* norm_out;
-    // jet (isf_jet_configure)
-    bool jet_set;
-    JetParams jet;
-    float omega;
-    long long n_spawn;
-    unsigned long long seed;
-    // spatial processing order of particle slots for the advection
-    int* order;
-    void* order_scratch;
-    size_t order_scratch_bytes;
-    long long order_cap;
-    const float* order_parts;
-    long long order_n;
-    int order_iter;
-    int sort_interval;
-    int order_check_iter;
-    bool order_sorted;               // slots are advected through `order` (else slot order)
-    unsigned* jump_partial;          // per-block counts of the order measure
-    long long jump_cap;
-};
+const char* last_error() { return g_err.c_str(); }
+}  // namespace isf
 
 static float* f_of(isf_t h) { return reinterpret_cast<float*>(h->S); }
 static float2* P_of(isf_t h) { return reinterpret_cast<float2*>(reinterpret_cast<char*>(h->S) + 4 * h->N); }
 
-static int check_handle(isf_t h) {
+int isf::check_handle(isf_t h) {
     if (!h || h->magic != kMagic) return fail(ISF_EHANDLE, "null or destroyed isf handle");
     return ISF_OK;
 }
@@ -113,7 +61,7 @@ static int wait_ext(isf_t h) {
     return ISF_OK;
 }
 // Order the handle's stream after a pending particle-side advect of isf_step.
-static int join(isf_t h) {
+int isf::join(isf_t h) {
     int r = wait_ext(h);
     if (r) return r;
     if (h->side_pending) {
@@ -129,14 +77,10 @@ static int join(isf_t h) {
         if (!r_) r_ = join(h);                  \
         if (r_) return r_;                      \
     } while (0)
-#define CHECK_PTR(p, name)                                                   \
-    do {                                                                     \
-        if (!(p)) return fail(ISF_EINVAL, "%s must not be NULL", name);      \
-    } while (0)
 
 extern "C" {
 
-const char* isf_last_error(void) { return g_err.c_str(); }
+const char* isf_last_error(void) { return isf::last_error(); }
 const char* isf_version(void) { return "libisf 0.1 (gfx950)"; }
 
 int isf_device_count(int* out) {
@@ -336,6 +280,7 @@ int isf_destroy(isf_t h) {
     if (h->order) (void)hipFree(h->order);
     if (h->order_scratch) (void)hipFree(h->order_scratch);
     if (h->jump_partial) (void)hipFree(h->jump_partial);
+    if (h->vjp_mem) (void)hipFree(h->vjp_mem);
     if (h->table_mem) (void)hipFree(h->table_mem);
     h->magic = 0;
     delete h;
@@ -966,11 +911,11 @@ int isf_batch_create(isf_batch_t* out, int n, const isf_member_desc* members) {
         const isf_member_desc& d = members[i];
         isf_t h = nullptr;
         int r = isf_create(&h, d.res, d.box, d.hbar, d.dt);
-        if (r) return bail(fail(r, "batch member %d: %s", i, g_err.c_str()));
+        if (r) return bail(fail(r, "batch member %d: %s", i, std::string(isf::last_error()).c_str()));
         h->owner = b;
         b->mem.push_back(h);
         r = isf_jet_configure(h, d.kvec, d.omega, d.cen, d.len, d.rad, d.n_spawn, d.seed, d.phase_mode);
-        if (r) return bail(fail(r, "batch member %d: %s", i, g_err.c_str()));
+        if (r) return bail(fail(r, "batch member %d: %s", i, std::string(isf::last_error()).c_str()));
         b->seen_seq.push_back(h->seq);
     }
     b->psi.assign(n, nullptr);

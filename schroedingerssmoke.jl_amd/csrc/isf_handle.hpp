@@ -1,0 +1,90 @@
+// isf_handle.hpp -- the ISF handle (struct isf_handle behind isf_t) and the error
+// helpers shared by the C-ABI sources (isf_capi.hip, isf_rollout.hip).  Not part of
+// the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/isf.h"
+#include "isf_internal.hpp"
+
+namespace isf {
+// set the thread-local message returned by isf_last_error and return code
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+const char* last_error();
+}  // namespace isf
+
+#define HIP_CHECK(expr, what)                                         \
+    do {                                                              \
+        hipError_t e_ = (expr);                                       \
+        if (e_ != hipSuccess) return isf::hip_fail(e_, what);         \
+    } while (0)
+#define LAUNCH_CHECK(expr, what)                                      \
+    do {                                                              \
+        int e_ = (expr);                                              \
+        if (e_ != 0) return isf::hip_fail((hipError_t)e_, what);      \
+    } while (0)
+#define CHECK_PTR(p, name)                                                         \
+    do {                                                                           \
+        if (!(p)) return isf::fail(ISF_EINVAL, "%s must not be NULL", name);       \
+    } while (0)
+
+using isf::DevTables;
+using isf::GridInfo;
+using isf::JetParams;
+
+constexpr uint32_t kMagic = 0x15F0CAFEu;
+
+struct isf_handle {
+    uint32_t magic;
+    int device;
+    GridInfo g;
+    float hbar, dt;
+    size_t N;
+    hipStream_t stream;   // the handle's stream: every operator, the grid part of isf_step
+    hipStream_t pstream;  // particle side of isf_step (spawn, order, advect), joined by events
+    hipEvent_t ev0, ev1;
+    hipEvent_t ev_vel, ev_adv;
+    bool side_pending;    // an advect on pstream may still read vel / write particles
+    hipEvent_t ext_ev;    // recorded on the handle's stream when a batched step must wait for it
+    hipEvent_t batch_ev;  // (borrowed) end of the last batched step that used this handle's data
+    bool ext_pending;     // the handle's streams must wait for batch_ev before new work
+    unsigned long long seq;  // bumped by every call that enqueues work on the handle's streams
+    const void* owner;       // the batch that owns this handle (isf_destroy refuses it)
+    void* table_mem;
+    DevTables tb;
+    float2* S;          // Schroedinger spectrum, 2N complex; f and P alias it in projections
+    float* vel;         // velocity field, 3N floats
+    double* norm_parts; // per-block FP64 partial sums
+    size_t norm_nparts;
+    double* norm_out;
+    // jet (isf_jet_configure)
+    bool jet_set;
+    JetParams jet;
+    float omega;
+    long long n_spawn;
+    unsigned long long seed;
+    // spatial processing order of particle slots for the advection
+    int* order;
+    void* order_scratch;
+    size_t order_scratch_bytes;
+    long long order_cap;
+    const float* order_parts;
+    long long order_n;
+    int order_iter;
+    int sort_interval;
+    int order_check_iter;
+    bool order_sorted;               // slots are advected through `order` (else slot order)
+    unsigned* jump_partial;          // per-block counts of the order measure
+    void* vjp_mem;                   // scratch of the operator VJPs (allocated on first use)
+    long long jump_cap;
+};
+
+
+namespace isf {
+int check_handle(isf_t h);
+// order the handle's streams after its pending side-stream and batched work
+int join(isf_t h);
+}  // namespace isf

@@ -119,6 +119,46 @@ size_t order_scratch_bytes(const GridInfo& g, long long nslots);
 int launch_particle_order(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
                           size_t scratch_bytes, hipStream_t s);
 
+// ---- adjoint of the step and the differentiable rollout (isf_adjoint.hip)
+// Particle-density grid: cells of size h over the box L, Gaussian width sigma truncated
+// at radius R, density scale inv_n (1 / number of slots).
+struct SplatCfg {
+    int res[3];
+    float h[3];
+    float L[3];
+    float sigma, R, inv_n;
+};
+int launch_flow_plane_adj(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s);
+int launch_fill_psi(float4* psi, float4 v, size_t n, hipStream_t s);
+int launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
+int launch_fill_f64(double* p, double v, size_t n, hipStream_t s);
+int launch_apply_phase(const GridInfo& g, float4* psi, const float* theta, const int pr[3], hipStream_t s);
+int launch_apply_phase_adj(const GridInfo& g, float4* psibar, const float4* psi_out, const float* theta, float* phibar,
+                           const int pr[3], hipStream_t s);
+int launch_phase_gather(const GridInfo& g, const float* phibar, float* thbar, const int pr[3], hipStream_t s);
+int launch_normalize_to(const GridInfo& g, const float4* in, float4* out, hipStream_t s);
+int launch_restrict_to(const GridInfo& g, const float4* in, float4* out, const JetParams& j, hipStream_t s);
+int launch_gauge_qbar(const GridInfo& g, const float4* psibar, const float4* psi_out, float* qbar, hipStream_t s);
+int launch_oneform_adj_div(const GridInfo& g, float4* psibar, const float4* psi, const float* q, const float* fbar,
+                           hipStream_t s);
+int launch_oneform_adj_vel(const GridInfo& g, float4* psibar, const float4* psi, const float* velbar, float hbar,
+                           int sharp, hipStream_t s);
+// partials: restrict_adj_blocks(g) x 4 doubles; out4[k] += the block sums (sum phibar,
+// sum phibar x, sum phibar y, sum phibar z)
+size_t restrict_adj_blocks(const GridInfo& g);
+int launch_restrict_adj(const GridInfo& g, float4* psibar, const float4* psi_in, const JetParams& j, double* partials,
+                        double* out4, hipStream_t s);
+int launch_normalize_adj(const GridInfo& g, float4* psibar, const float4* psi_in, hipStream_t s);
+int launch_advect_adj(const GridInfo& g, const float* parts0, long long nslots, const float* vel, float dt, float* pbar,
+                      float* velbar, hipStream_t s);
+int launch_zero_slots(float* pbar, long long nslots, long long first, long long n, hipStream_t s);
+int launch_splat(const float* parts, long long nslots, const SplatCfg& sc, double* rho, hipStream_t s);
+size_t splat_loss_blocks(int M);
+int launch_splat_loss(const double* rho, const float* target, int M, float* resid, double* partials, double* out,
+                      hipStream_t s);
+int launch_splat_adj(const float* parts, long long nslots, const SplatCfg& sc, const float* resid, float* pbar,
+                     hipStream_t s);
+
 // Supported axis lengths.
 bool x_len_supported(int n);
 bool plane_supported(int ny, int nz);

@@ -17,97 +17,12 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "isf_device.hpp"
 #include "isf_fft.hpp"
 #include "isf_internal.hpp"
 #include "isf_philox.hpp"
 
 namespace isf {
-
-// ======================================================================== helpers
-
-// angle(re + i im) = atan2(im, re) (Julia's angle, src/parallel.jl:108).  Range reduction
-// to a = min/max in [0, 1], atan(a) = a P(a^2) with a degree-8 polynomial fitted for this
-// project (weighted least squares driven to equal ripple; |relative error| <= 1.5e-7 on
-// [0, 1] in FP32 Horner evaluation), then the octant is restored.  angle(0) = 0 and the
-// sign of a zero imaginary part is kept, as atan2 does.
-__device__ __forceinline__ float angle_of(float re, float im) {
-    const float ax = fabsf(re), ay = fabsf(im);
-    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
-    const float a = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
-    const float s = a * a;
-    float p = 2.903499175e-03f;
-    p = fmaf(p, s, -1.628280245e-02f);
-    p = fmaf(p, s, 4.303904623e-02f);
-    p = fmaf(p, s, -7.533650100e-02f);
-    p = fmaf(p, s, 1.065466627e-01f);
-    p = fmaf(p, s, -1.420713067e-01f);
-    p = fmaf(p, s, 1.999305338e-01f);
-    p = fmaf(p, s, -3.333309293e-01f);
-    p = fmaf(p, s, 1.0f);
-    float r = p * a;
-    if (ay > ax) r = 1.57079632679489661923f - r;
-    if (__float_as_uint(re) >> 31) r = 3.14159265358979323846f - r;  // re < 0 or re = -0
-    return copysignf(r, im);
-}
-
-// angle(conj(a1) b1 + conj(a2) b2) for psi cells a, b (src/parallel.jl:108-111)
-__device__ __forceinline__ float edge_angle(float4 a, float4 b) {
-    const float re = (a.x * b.x + a.y * b.y) + (a.z * b.z + a.w * b.w);
-    const float im = (a.x * b.y - a.y * b.x) + (a.z * b.w - a.w * b.z);
-    return angle_of(re, im);
-}
-
-// isjet, src/particles.jl:6-10, at position (i-1)*d (src/parallel.jl:75-77).  Rounded
-// operations (no contraction) so the predicate is the one the Float32 reference evaluates.
-__device__ __forceinline__ bool is_jet_cell(int x, int y, int z, const GridInfo& g, const JetParams& j) {
-    const float px = __fmul_rn((float)x, g.dx), py = __fmul_rn((float)y, g.dy), pz = __fmul_rn((float)z, g.dz);
-    const float ax = fabsf(__fsub_rn(px, j.cx));
-    const float dy = __fsub_rn(py, j.cy), dz = __fsub_rn(pz, j.cz);
-    const float r2 = __fadd_rn(__fmul_rn(dy, dy), __fmul_rn(dz, dz));
-    return (ax <= j.len * 0.5f) & (r2 <= __fmul_rn(j.rad, j.rad));
-}
-
-// restrict_kernel inside the jet, src/particles.jl:12-20: psi_a <- |psi_a| exp(i phase)
-__device__ __forceinline__ float4 jet_phase(float4 p, int x, int y, int z, const GridInfo& g, const JetParams& j) {
-    const float px = __fmul_rn((float)x, g.dx), py = __fmul_rn((float)y, g.dy), pz = __fmul_rn((float)z, g.dz);
-    float ph;
-    if (j.phase_mode == 1)
-        ph = __fadd_rn(__fadd_rn(__fadd_rn(j.kx, px), __fadd_rn(j.ky, py)), __fadd_rn(j.kz, pz));
-    else
-        ph = __fadd_rn(__fadd_rn(__fmul_rn(j.kx, px), __fmul_rn(j.ky, py)), __fmul_rn(j.kz, pz));
-    ph = __fsub_rn(ph, j.omgterm);
-    float s, c;
-    sincosf(ph, &s, &c);
-    const float a1 = hypotf(p.x, p.y), a2 = hypotf(p.z, p.w);
-    return make_float4(a1 * c, a1 * s, a2 * c, a2 * s);
-}
-
-// normalize_psi (src/parallel.jl:162-165): psi_a / hypot(|psi1|, |psi2|).  |psi| is O(1)
-// here, so the norm is formed as sqrt(sum of squares) (the reference uses @fastmath
-// hypot) and applied as one reciprocal.
-__device__ __forceinline__ float4 normalized(float4 p) {
-    const float inv = 1.0f / sqrtf((p.x * p.x + p.y * p.y) + (p.z * p.z + p.w * p.w));
-    return make_float4(p.x * inv, p.y * inv, p.z * inv, p.w * inv);
-}
-
-// block-wide sum of a double (NT a power of two); result valid in thread 0.
-// Fixed tree order, so the sum is deterministic.
-template <int NT>
-__device__ __forceinline__ double block_sum(double v) {
-    __shared__ double red[NT];
-    red[threadIdx.x] = v;
-    __syncthreads();
-#pragma unroll
-    for (int s = NT / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    return red[0];
-}
-
-__device__ __forceinline__ float2* poisson_P(const Inst& m) {
-    return reinterpret_cast<float2*>(reinterpret_cast<char*>(m.S) + 4 * (size_t)m.g.nx * m.g.ny * m.g.nz);
-}
 
 template <int NX> struct XCfg {
     static constexpr int E = DefaultE<NX>::value;
@@ -212,7 +127,9 @@ template <int NY, int NZ> struct PlaneCfg {
     static_assert(NY >= 16 && NZ >= 16 && NT <= 1024, "plane too large for one workgroup");
 };
 
-enum { PLANE_FLOW = 0, PLANE_POISSON = 1 };
+// PLANE_FLOW_ADJ: the adjoint of PLANE_FLOW, conj(mask) (the adjoint of F^-1 M F with an
+// unnormalised forward transform F is F^-1 conj(M) F).
+enum { PLANE_FLOW = 0, PLANE_POISSON = 1, PLANE_FLOW_ADJ = 2 };
 
 // y and z transforms of one kx plane, spectral multiplier, inverse z and y; the block
 // walks planes blockIdx.x, blockIdx.x + gridDim.x, ... and prefetches the next plane into
@@ -231,7 +148,7 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
     using C = PlaneCfg<NY, NZ>;
     if (B.inst) {
         const Inst& m = B.inst[blockIdx.y];
-        base = MODE == PLANE_FLOW ? m.S : poisson_P(m);
+        base = MODE == PLANE_POISSON ? poisson_P(m) : m.S;
         scale = m.pscale;
         tb = m.tb;
     }
@@ -252,7 +169,7 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
     TZ2I::fill(twz + TZ2::SIZE, tb.tw + NZ, tid, NT);
     float2* mt2 = reinterpret_cast<float2*>(mtab);
     for (int i = tid; i < NY + NZ; i += NT) {
-        if (MODE == PLANE_FLOW) mt2[i] = i < NY ? tb.mask_y[i] : tb.mask_z[i - NY];
+        if (MODE != PLANE_POISSON) mt2[i] = i < NY ? tb.mask_y[i] : tb.mask_z[i - NY];
         else mtab[i] = i < NY ? tb.s2y[i] : tb.s2z[i - NY];
     }
     const int zl = tid / C::TY, ty = tid % C::TY;  // y transforms: line z = zl
@@ -303,14 +220,16 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
             ZL::template compute<-1>(v, tz, twz);  // TZ2 table
         }
         constexpr int RL = E / ZL::NB;  // radix of the last forward z pass
-        if (MODE == PLANE_FLOW) {
+        if (MODE != PLANE_POISSON) {
             const int kx = plane % nx;
             const float2 wxy = cmul(tb.mask_x[kx], mt2[yl]);
 #pragma unroll
             for (int b = 0; b < ZL::NB; b++)
 #pragma unroll
-                for (int r = 0; r < RL; r++)
-                    v[b * RL + r] = cmul(v[b * RL + r], cmul(wxy, mt2[NY + ZL::out(tz, b, r)]));
+                for (int r = 0; r < RL; r++) {
+                    const float2 m = cmul(wxy, mt2[NY + ZL::out(tz, b, r)]);
+                    v[b * RL + r] = cmul(v[b * RL + r], MODE == PLANE_FLOW_ADJ ? cconj(m) : m);
+                }
         } else {
             // eigenvalue sum in FP64 (the per-axis tables are FP64), rounded once to FP32;
             // scale = -0.5/N is a power of two, so scale * rcp(den) is within ~1.5 ulp of the
@@ -466,6 +385,7 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
             float4 p = psi[row0 + i];
             float s, c;
             sincosf(q, &s, &c);  // exp(-i q) = (c, -s)
+            if (qout) qout[row0 + i] = q;  // kept for the adjoint
             p = make_float4(p.x * c + p.y * s, p.y * c - p.x * s, p.z * c + p.w * s, p.w * c - p.z * s);
             if (MODE == C2R_GAUGE_RESTRICT && is_jet_cell(x, y0 + l, z, g, jet)) p = jet_phase(p, x, y0 + l, z, g, jet);
             if (partials) acc += (double)p.x * p.x + (double)p.y * p.y + (double)p.z * p.z + (double)p.w * p.w;
@@ -480,12 +400,6 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
 
 // ======================================================================== stencils
 
-__device__ __forceinline__ void cell_xyz(size_t c, const GridInfo& g, int& x, int& y, int& z) {
-    x = (int)(c % g.nx);
-    const size_t r = c / g.nx;
-    y = (int)(r % g.ny);
-    z = (int)(r / g.ny);
-}
 
 // ---- z-marching stencil tiles
 // A block of BX x 4 threads covers BX consecutive x and 4 consecutive y and walks TZ
@@ -687,42 +601,12 @@ __global__ void __launch_bounds__(256) k_reduce_sum(const double* __restrict__ p
 
 // ======================================================================== particles
 
-// Julia mod(x, y) for floats (result has the sign of y), src/parallel.jl:220:
-// r = rem(x, y); r != 0 && sign(r) != sign(y) ? r + y : r.  For y > 0 and the ranges
-// a particle position takes, rem is exact and simple: x in [0, y) -> x;
-// x in (-y, 0) -> x + y; x in [y, 2y) -> x - y (exact by Sterbenz); otherwise fmodf.
-__device__ __forceinline__ float jl_mod(float x, float y) {
-    if (x >= 0.0f && x < y) return x;
-    if (x < 0.0f && x > -y) return x + y;
-    if (x >= y && x < 2.0f * y) return x - y;
-    float r = fmodf(x, y);
-    if (r != 0.0f && ((r < 0.0f) != (y < 0.0f))) r += y;
-    return r;
-}
-
-struct F3 {
-    float x, y, z;
-};
-// three consecutive floats (the compiler merges them into one dwordx3 load)
-__device__ __forceinline__ F3 ld3(const float* __restrict__ p) { return F3{p[0], p[1], p[2]}; }
-
 // staggered_velocity, src/parallel.jl:219-246.  Weights are not divided by d (:238).
 // An index that mod() rounds onto the upper face wraps to the first cell.
 __device__ __forceinline__ void stag_vel(const float* __restrict__ vel, const float p_in[3], const GridInfo& g, float u[3]) {
-    const float d[3] = {g.dx, g.dy, g.dz};
-    const float id[3] = {g.idx, g.idy, g.idz};
-    const float L[3] = {g.Lx, g.Ly, g.Lz};
-    const int n[3] = {g.nx, g.ny, g.nz};
-    float p[3];
     int i[3], ip[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        p[a] = jl_mod(p_in[a], L[a]);
-        int ia = (int)floorf(g.d_pow2 ? p[a] * id[a] : p[a] / d[a]);
-        if (ia >= n[a]) ia -= n[a];
-        i[a] = ia;
-        ip[a] = ia + 1 == n[a] ? 0 : ia + 1;
-    }
+    float w[3];
+    stag_index(p_in, g, i, ip, w);
     // the 7 cells as 4 rows: (j,k) and (j+1,k), (j,k+1) at x = i and i+1, (j+1,k+1) at i
     const unsigned sy = (unsigned)g.nx, sz = (unsigned)g.nx * (unsigned)g.ny;
     const unsigned rjk = sy * i[1] + sz * i[2], rjpk = sy * ip[1] + sz * i[2];
@@ -735,9 +619,6 @@ __device__ __forceinline__ void stag_vel(const float* __restrict__ vel, const fl
     const float vn[3] = {d0x, c1.y, b1.z};    // (v[i,j+1,k+1].x, v[i+1,j,k+1].y, v[i+1,j+1,k].z)
     const float pp[3] = {b0.x, a1.y, a1.z};   // (pyp.x, pxp.y, pxp.z)
     const float pp2[3] = {c0.x, c0.y, b0.z};  // (pzp.x, pzp.y, pyp.z)
-    float w[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) w[a] = p[a] - (float)i[a] * d[a];
     const float w1[3] = {w[2], w[2], w[1]}, w2[3] = {w[1], w[0], w[0]};
 #pragma unroll
     for (int a = 0; a < 3; a++)
@@ -981,6 +862,8 @@ static int plane_t(int mode, float2* base, int nplanes, int nx, double scale, co
     const dim3 grid(blocks, mem);
     if (mode == PLANE_FLOW)
         ISF_LAUNCH((k_plane<NY, NZ, PLANE_FLOW>), grid, dim3(C::NT), s, base, nplanes, nx, scale, t, b);
+    if (mode == PLANE_FLOW_ADJ)
+        ISF_LAUNCH((k_plane<NY, NZ, PLANE_FLOW_ADJ>), grid, dim3(C::NT), s, base, nplanes, nx, scale, t, b);
     ISF_LAUNCH((k_plane<NY, NZ, PLANE_POISSON>), grid, dim3(C::NT), s, base, nplanes, nx, scale, t, b);
 }
 
@@ -1007,6 +890,9 @@ static int plane_dispatch(int mode, int ny, int nz, float2* base, int nplanes, i
 
 int launch_flow_plane(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s, Batch b) {
     return plane_dispatch(PLANE_FLOW, g.ny, g.nz, S, 2 * g.nx, g.nx, 0.0, t, s, b);
+}
+int launch_flow_plane_adj(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s) {
+    return plane_dispatch(PLANE_FLOW_ADJ, g.ny, g.nz, S, 2 * g.nx, g.nx, 0.0, t, s, Batch{});
 }
 int launch_poisson_plane(const GridInfo& g, float2* P, double scale, const DevTables& t, hipStream_t s, Batch b) {
     return plane_dispatch(PLANE_POISSON, g.ny, g.nz, P, g.nx / 2 + 1, g.nx, scale, t, s, b);

@@ -1,0 +1,601 @@
+// isf_rollout.hip -- C ABI of the differentiable rollout (isf_rollout_*): the driver's
+// set-up and simloop (test/runtests.jl:16-66) as a function of the nozzle velocity, an
+// initial phase field, the initial wavefunction and the initial particle positions, a
+// particle-density loss, and the reverse sweep through the hand-written adjoints of
+// isf_adjoint.hip.
+//
+// Forward: the step is run operator by operator (the same kernels and arithmetic as
+// isf_step, with the fused normalize and restrict done as separate passes so that their
+// inputs can be kept) and psi / particles are checkpointed every K steps.  Backward:
+// segments are processed last to first; each is recomputed from its checkpoint keeping
+// the per-step tape (step input psi0, flow output psi1, first gauge output psi3, the two
+// gauge phases q1 q2, particles after the spawn) and then reversed step by step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "isf_handle.hpp"
+#include "isf_internal.hpp"
+
+using namespace isf;
+
+static constexpr uint32_t kRolloutMagic = 0x2011A0D7u;
+
+struct isf_rollout {
+    uint32_t magic;
+    isf_t h;
+    isf_rollout_desc d;
+    GridInfo g;
+    size_t N;
+    long long P;
+    int T, K, C, nseg, M;
+    SplatCfg sc;
+    std::vector<void*> allocs;
+    // set-up tape: cpsi[j] = input of constraint iteration j (cpsi[C] = step-1 input)
+    std::vector<float4*> cpsi;
+    std::vector<float*> cq;
+    float* theta;
+    bool use_theta;
+    float* parts_init;
+    // checkpoints at steps 0, K, 2K, ... (ck_psi[0] = cpsi[C], ck_parts[0] = parts_init)
+    std::vector<float4*> ck_psi;
+    std::vector<float*> ck_parts;
+    // segment tape
+    std::vector<float4*> t_mid, t_psi1, t_psi3;
+    std::vector<float*> t_q1, t_q2, t_p0;
+    // work
+    float4 *W, *psibar;
+    float *f, *fbar, *vel, *velbar, *parts, *parts_final, *pbar, *phibar, *target, *resid;
+    double *rho, *partials, *loss_dev, *vacc;
+    // parameters of the last forward
+    float v[3];
+    float omega;
+    JetParams jet;  // kvec from v, omgterm set per use
+    bool forward_done;
+};
+
+static int check_rollout(isf_rollout_t r) {
+    if (!r || r->magic != kRolloutMagic) return fail(ISF_EHANDLE, "null or destroyed isf rollout");
+    return ISF_OK;
+}
+#define CHECK_R(r)                      \
+    do {                                \
+        int r_ = check_rollout(r);      \
+        if (r_) return r_;              \
+    } while (0)
+
+static void rollout_free(isf_rollout_t r) {
+    if (r->h && r->h->stream) (void)hipStreamSynchronize(r->h->stream);
+    for (void* p : r->allocs) (void)hipFree(p);
+    r->magic = 0;
+    delete r;
+}
+
+template <class T>
+static int ralloc(isf_rollout_t r, T** p, size_t count) {
+    *p = nullptr;
+    void* q = nullptr;
+    const hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) return fail(ISF_ENOMEM, "rollout buffers (%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    r->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return ISF_OK;
+}
+
+static bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// omega = sum(v.^2) / (2 hbar) with Float32 roundings and no contraction, as the driver
+// computes it (test/runtests.jl:24)
+static float driver_omega(const float v[3], float hbar) {
+#pragma clang fp contract(off)
+    const float s = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    return s / (2.0f * hbar);
+}
+
+// ------------------------------------------------------------------ forward pieces
+
+static float2* P_of(isf_rollout_t r) {
+    return reinterpret_cast<float2*>(reinterpret_cast<char*>(r->h->S) + 4 * r->N);
+}
+static double poisson_scale(isf_rollout_t r) { return -0.25 * 2.0 / (double)r->N; }
+
+// fout = Poisson(fin) (src/parallel.jl:168-174)
+static int poisson(isf_rollout_t r, const float* fin, float* fout) {
+    isf_t h = r->h;
+    LAUNCH_CHECK(launch_r2c_x(r->g, fin, P_of(r), h->tb, h->stream), "poisson x forward");
+    LAUNCH_CHECK(launch_poisson_plane(r->g, P_of(r), poisson_scale(r), h->tb, h->stream), "poisson plane");
+    LAUNCH_CHECK(launch_c2r_x(r->g, P_of(r), 0, fout, nullptr, nullptr, nullptr, h->tb, h->stream), "poisson x inverse");
+    return ISF_OK;
+}
+// pressure_project! in place keeping the gauge phase q (src/parallel.jl:155-160)
+static int project_q(isf_rollout_t r, float4* psi, float* q) {
+    isf_t h = r->h;
+    LAUNCH_CHECK(launch_div_psi(r->g, psi, r->f, h->stream), "div(one_form(psi))");
+    LAUNCH_CHECK(launch_r2c_x(r->g, r->f, P_of(r), h->tb, h->stream), "poisson x forward");
+    LAUNCH_CHECK(launch_poisson_plane(r->g, P_of(r), poisson_scale(r), h->tb, h->stream), "poisson plane");
+    LAUNCH_CHECK(launch_c2r_x(r->g, P_of(r), 1, q, psi, nullptr, nullptr, h->tb, h->stream), "poisson x inverse + gauge");
+    return ISF_OK;
+}
+
+static int seg_steps(isf_rollout_t r, int s) { return std::min(r->K, r->T - s * r->K); }
+static float4* psi0_of(isf_rollout_t r, int s, int k) {
+    if (k == 0) return r->ck_psi[s];
+    if (k == seg_steps(r, s)) return r->ck_psi[s + 1];
+    return r->t_mid[k - 1];
+}
+static JetParams jet_at(isf_rollout_t r, int it) {
+    JetParams j = r->jet;
+    j.omgterm = it > 0 ? r->omega * ((float)it * r->h->dt) : 0.0f;  // test/runtests.jl:46,57
+    return j;
+}
+static void spawn_slots(isf_rollout_t r, int it, long long* first0, long long* n) {
+    *first0 = 0;
+    *n = 0;
+    if (r->P <= 0 || r->h->n_spawn <= 0) return;
+    const long long start1 = ((long long)(it - 1) * r->h->n_spawn + 1) % r->P;  // test/runtests.jl:52
+    *first0 = start1 == 0 ? r->P - 1 : start1 - 1;
+    *n = r->h->n_spawn;
+}
+
+// Steps of segment s from its checkpoint, keeping the tape.
+static int seg_forward(isf_rollout_t r, int s) {
+    isf_t h = r->h;
+    hipStream_t st = h->stream;
+    const size_t pb = sizeof(float) * 3 * (size_t)r->P;
+    if (r->P > 0)
+        HIP_CHECK(hipMemcpyAsync(r->parts, r->ck_parts[s], pb, hipMemcpyDeviceToDevice, st), "particles checkpoint");
+    const int ks = seg_steps(r, s);
+    for (int k = 0; k < ks; k++) {
+        const int it = s * r->K + k + 1;
+        const float4* in = psi0_of(r, s, k);
+        float4* out = psi0_of(r, s, k + 1);
+        long long first0, nsp;
+        spawn_slots(r, it, &first0, &nsp);
+        if (nsp > 0) {
+            LAUNCH_CHECK(launch_spawn(r->parts, r->P, first0, nsp, h->jet, h->seed, (unsigned)it, st), "add_particles");
+        }
+        if (r->P > 0)
+            HIP_CHECK(hipMemcpyAsync(r->t_p0[k], r->parts, pb, hipMemcpyDeviceToDevice, st), "particles tape");
+        LAUNCH_CHECK(launch_flow_x_fwd(r->g, in, h->S, h->tb, st), "flow x forward");
+        LAUNCH_CHECK(launch_flow_plane(r->g, h->S, h->tb, st), "flow plane");
+        LAUNCH_CHECK(launch_flow_x_inv(r->g, h->S, r->t_psi1[k], false, h->tb, st), "flow x inverse");
+        LAUNCH_CHECK(launch_normalize_to(r->g, r->t_psi1[k], r->t_psi3[k], st), "normalize");
+        int e = project_q(r, r->t_psi3[k], r->t_q1[k]);
+        if (e) return e;
+        LAUNCH_CHECK(launch_restrict_to(r->g, r->t_psi3[k], out, jet_at(r, it), st), "restrict_velocity");
+        if ((e = project_q(r, out, r->t_q2[k]))) return e;
+        LAUNCH_CHECK(launch_velocity(r->g, out, h->hbar, 1, r->vel, st), "velocity_one_form");
+        if (r->P > 0) LAUNCH_CHECK(launch_advect(r->g, r->parts, r->P, nullptr, r->vel, h->dt, st), "staggered_advect");
+    }
+    if (s + 1 < r->nseg && r->P > 0)
+        HIP_CHECK(hipMemcpyAsync(r->ck_parts[s + 1], r->parts, pb, hipMemcpyDeviceToDevice, st), "particles checkpoint");
+    return ISF_OK;
+}
+
+// Reverse of step k of segment s (the tape of segment s is current).
+static int step_adjoint(isf_rollout_t r, int s, int k) {
+    isf_t h = r->h;
+    hipStream_t st = h->stream;
+    const GridInfo& g = r->g;
+    const int it = s * r->K + k + 1;
+    const float4* psi5 = psi0_of(r, s, k + 1);
+    const JetParams j = jet_at(r, it);
+    // :59-63 velocity and advection
+    LAUNCH_CHECK(launch_velocity(g, psi5, h->hbar, 1, r->vel, st), "velocity_one_form");
+    LAUNCH_CHECK(launch_fill_f32(r->velbar, 0.0f, 3 * r->N, st), "fill");
+    if (r->P > 0) {
+        LAUNCH_CHECK(launch_advect_adj(g, r->t_p0[k], r->P, r->vel, h->dt, r->pbar, r->velbar, st), "advect adjoint");
+        long long first0, nsp;
+        spawn_slots(r, it, &first0, &nsp);
+        LAUNCH_CHECK(launch_zero_slots(r->pbar, r->P, first0, nsp, st), "spawn adjoint");
+    }
+    LAUNCH_CHECK(launch_oneform_adj_vel(g, r->psibar, psi5, r->velbar, h->hbar, 1, st), "velocity adjoint");
+    // :58 second projection (gauge output psi5, one-form at psi4 = restrict(psi3))
+    LAUNCH_CHECK(launch_gauge_qbar(g, r->psibar, psi5, r->f, st), "gauge adjoint");
+    int e = poisson(r, r->f, r->fbar);
+    if (e) return e;
+    LAUNCH_CHECK(launch_restrict_to(g, r->t_psi3[k], r->W, j, st), "restrict_velocity");
+    LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, r->W, r->t_q2[k], r->fbar, st), "projection adjoint");
+    // :57 jet constraint
+    LAUNCH_CHECK(launch_restrict_adj(g, r->psibar, r->t_psi3[k], j, r->partials, r->vacc + 4 * (size_t)(it - 1), st),
+                 "restrict adjoint");
+    // :50 first projection (gauge output psi3, one-form at psi2 = normalize(psi1))
+    LAUNCH_CHECK(launch_gauge_qbar(g, r->psibar, r->t_psi3[k], r->f, st), "gauge adjoint");
+    if ((e = poisson(r, r->f, r->fbar))) return e;
+    LAUNCH_CHECK(launch_normalize_to(g, r->t_psi1[k], r->W, st), "normalize");
+    LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, r->W, r->t_q1[k], r->fbar, st), "projection adjoint");
+    // :49 normalize, :48 schroedinger_flow
+    LAUNCH_CHECK(launch_normalize_adj(g, r->psibar, r->t_psi1[k], st), "normalize adjoint");
+    LAUNCH_CHECK(launch_flow_x_fwd(g, r->psibar, h->S, h->tb, st), "flow adjoint x forward");
+    LAUNCH_CHECK(launch_flow_plane_adj(g, h->S, h->tb, st), "flow adjoint plane");
+    LAUNCH_CHECK(launch_flow_x_inv(g, h->S, r->psibar, false, h->tb, st), "flow adjoint x inverse");
+    return ISF_OK;
+}
+
+extern "C" {
+
+int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc) {
+    CHECK_PTR(out, "out");
+    CHECK_PTR(desc, "desc");
+    *out = nullptr;
+    {
+        int e = check_handle(h);
+        if (e) return e;
+    }
+    if (!h->jet_set) return fail(ISF_EINVAL, "isf_jet_configure must be called first");
+    const isf_rollout_desc& d = *desc;
+    if (d.steps < 1) return fail(ISF_EINVAL, "steps must be >= 1");
+    if (d.checkpoint_every < 1) return fail(ISF_EINVAL, "checkpoint_every must be >= 1");
+    if (d.constraint_iters < 0) return fail(ISF_EINVAL, "constraint_iters must be >= 0");
+    const int n3[3] = {h->g.nx, h->g.ny, h->g.nz};
+    const bool no_phase = d.phase_res[0] == 0 && d.phase_res[1] == 0 && d.phase_res[2] == 0;
+    for (int a = 0; a < 3 && !no_phase; a++)
+        if (!pow2(d.phase_res[a]) || d.phase_res[a] > n3[a])
+            return fail(ISF_EINVAL, "phase_res[%d] = %d must be a power of two <= %d", a, d.phase_res[a], n3[a]);
+    for (int a = 0; a < 3; a++)
+        if (d.splat_res[a] < 1 || d.splat_res[a] > 1024) return fail(ISF_EINVAL, "splat_res[%d] out of range", a);
+    if (!(d.splat_sigma > 0.0f) || !(d.splat_cutoff > 0.0f)) return fail(ISF_EINVAL, "splat sigma and cutoff must be > 0");
+    if (d.nslots < 0 || d.n_init < 0 || d.n_init > d.nslots) return fail(ISF_EINVAL, "bad particle counts");
+
+    isf_rollout* r = new isf_rollout();
+    r->magic = kRolloutMagic;
+    r->h = h;
+    r->d = d;
+    r->g = h->g;
+    r->N = h->N;
+    r->P = d.nslots;
+    r->T = d.steps;
+    r->K = std::min(d.checkpoint_every, d.steps);
+    r->C = d.constraint_iters;
+    r->nseg = (r->T + r->K - 1) / r->K;
+    r->M = d.splat_res[0] * d.splat_res[1] * d.splat_res[2];
+    for (int a = 0; a < 3; a++) {
+        r->sc.res[a] = d.splat_res[a];
+        r->sc.L[a] = a == 0 ? h->g.Lx : a == 1 ? h->g.Ly : h->g.Lz;
+        r->sc.h[a] = r->sc.L[a] / (float)d.splat_res[a];
+    }
+    r->sc.sigma = d.splat_sigma;
+    r->sc.R = d.splat_cutoff * d.splat_sigma;
+    r->sc.inv_n = d.nslots > 0 ? 1.0f / (float)d.nslots : 0.0f;
+    r->use_theta = false;
+    auto bail = [&](int code) {
+        rollout_free(r);
+        return code;
+    };
+    const size_t N = r->N, P3 = 3 * (size_t)r->P;
+    int e = 0;
+#define RA(ptr, count)                      \
+    do {                                    \
+        if ((e = ralloc(r, ptr, count))) return bail(e); \
+    } while (0)
+    r->cpsi.resize(r->C + 1);
+    r->cq.resize(r->C);
+    for (auto& p : r->cpsi) RA(&p, N);
+    for (auto& p : r->cq) RA(&p, N);
+    RA(&r->theta, no_phase ? 1 : (size_t)d.phase_res[0] * d.phase_res[1] * d.phase_res[2]);
+    RA(&r->parts_init, P3);
+    r->ck_psi.resize(r->nseg + 1);
+    r->ck_parts.resize(r->nseg + 1);
+    r->ck_psi[0] = r->cpsi[r->C];
+    r->ck_parts[0] = r->parts_init;
+    for (int s = 1; s <= r->nseg; s++) RA(&r->ck_psi[s], N);
+    for (int s = 1; s < r->nseg; s++) RA(&r->ck_parts[s], P3);
+    r->ck_parts[r->nseg] = nullptr;
+    r->t_mid.resize(r->K - 1);
+    for (auto& p : r->t_mid) RA(&p, N);
+    r->t_psi1.resize(r->K);
+    r->t_psi3.resize(r->K);
+    r->t_q1.resize(r->K);
+    r->t_q2.resize(r->K);
+    r->t_p0.resize(r->K);
+    for (int k = 0; k < r->K; k++) {
+        RA(&r->t_psi1[k], N);
+        RA(&r->t_psi3[k], N);
+        RA(&r->t_q1[k], N);
+        RA(&r->t_q2[k], N);
+        RA(&r->t_p0[k], P3);
+    }
+    RA(&r->W, N);
+    RA(&r->psibar, N);
+    RA(&r->f, N);
+    RA(&r->fbar, N);
+    RA(&r->vel, 3 * N);
+    RA(&r->velbar, 3 * N);
+    RA(&r->parts, P3);
+    RA(&r->parts_final, P3);
+    RA(&r->pbar, P3);
+    RA(&r->phibar, N);
+    RA(&r->target, (size_t)r->M);
+    RA(&r->resid, (size_t)r->M);
+    RA(&r->rho, (size_t)r->M);
+    RA(&r->partials, std::max(4 * restrict_adj_blocks(r->g), splat_loss_blocks(r->M)));
+    RA(&r->loss_dev, 1);
+    RA(&r->vacc, 4 * (size_t)(r->T + r->C));
+#undef RA
+    hipStream_t st = h->stream;
+    if ((e = join(h))) return bail(e);
+    if (launch_fill_f32(r->target, 0.0f, (size_t)r->M, st)) return bail(fail(ISF_EHIP, "fill"));
+    if (r->P > 0) {
+        if (launch_fill_f32(r->parts_init, -1.0f, P3, st)) return bail(fail(ISF_EHIP, "fill"));
+        if (d.n_init > 0 && launch_fill_cylinder(r->parts_init, r->P, 0, d.n_init, h->jet, d.fill_seed, st))
+            return bail(fail(ISF_EHIP, "fill_nozzle"));
+    }
+    const hipError_t he = hipStreamSynchronize(st);
+    if (he != hipSuccess) return bail(hip_fail(he, "rollout set-up"));
+    *out = r;
+    return ISF_OK;
+}
+
+int isf_rollout_destroy(isf_rollout_t r) {
+    CHECK_R(r);
+    rollout_free(r);
+    return ISF_OK;
+}
+
+int isf_rollout_set_target(isf_rollout_t r, const float* target) {
+    CHECK_R(r);
+    CHECK_PTR(target, "target");
+    int e = join(r->h);
+    if (e) return e;
+    HIP_CHECK(hipMemcpyAsync(r->target, target, sizeof(float) * r->M, hipMemcpyDeviceToDevice, r->h->stream),
+              "target copy");
+    r->forward_done = false;
+    return ISF_OK;
+}
+
+int isf_rollout_forward(isf_rollout_t r, const float velocity[3], const float* theta, const float* psi_init,
+                        double* loss) {
+    CHECK_R(r);
+    CHECK_PTR(velocity, "velocity");
+    CHECK_PTR(loss, "loss");
+    const bool no_phase = r->d.phase_res[0] == 0;
+    if (theta && no_phase) return fail(ISF_EINVAL, "this rollout has no phase field (phase_res = 0)");
+    isf_t h = r->h;
+    int e = join(h);
+    if (e) return e;
+    hipStream_t st = h->stream;
+    r->forward_done = false;
+    // kvec = v / hbar, omega = sum(v.^2) / (2 hbar) in Float32 (test/runtests.jl:23-24)
+    for (int a = 0; a < 3; a++) r->v[a] = velocity[a];
+    r->jet = h->jet;
+    r->jet.kx = velocity[0] / h->hbar;
+    r->jet.ky = velocity[1] / h->hbar;
+    r->jet.kz = velocity[2] / h->hbar;
+    r->omega = driver_omega(velocity, h->hbar);
+    // set-up: psi_init (default (1, 0.01) normalized, test/runtests.jl:19-21), phase field
+    if (psi_init) {
+        HIP_CHECK(hipMemcpyAsync(r->cpsi[0], psi_init, sizeof(float4) * r->N, hipMemcpyDeviceToDevice, st), "psi_init copy");
+    } else {
+        LAUNCH_CHECK(launch_fill_psi(r->cpsi[0], make_float4(1.0f, 0.0f, 0.01f, 0.0f), r->N, st), "fill");
+        LAUNCH_CHECK(launch_normalize(r->g, r->cpsi[0], st), "normalize");
+    }
+    r->use_theta = theta != nullptr;
+    if (theta) {
+        const size_t nt = (size_t)r->d.phase_res[0] * r->d.phase_res[1] * r->d.phase_res[2];
+        HIP_CHECK(hipMemcpyAsync(r->theta, theta, sizeof(float) * nt, hipMemcpyDeviceToDevice, st), "theta copy");
+        LAUNCH_CHECK(launch_apply_phase(r->g, r->cpsi[0], r->theta, r->d.phase_res, st), "apply phase");
+    }
+    // test/runtests.jl:27-30
+    for (int j = 0; j < r->C; j++) {
+        LAUNCH_CHECK(launch_restrict_to(r->g, r->cpsi[j], r->cpsi[j + 1], jet_at(r, 0), st), "restrict_velocity");
+        if ((e = project_q(r, r->cpsi[j + 1], r->cq[j]))) return e;
+    }
+    for (int s = 0; s < r->nseg; s++)
+        if ((e = seg_forward(r, s))) return e;
+    if (r->P > 0)
+        HIP_CHECK(hipMemcpyAsync(r->parts_final, r->parts, sizeof(float) * 3 * r->P, hipMemcpyDeviceToDevice, st),
+                  "copy");
+    LAUNCH_CHECK(launch_fill_f64(r->rho, 0.0, (size_t)r->M, st), "fill");
+    LAUNCH_CHECK(launch_splat(r->parts_final, r->P, r->sc, r->rho, st), "splat");
+    LAUNCH_CHECK(launch_splat_loss(r->rho, r->target, r->M, r->resid, r->partials, r->loss_dev, st), "loss");
+    double sum = 0.0;
+    HIP_CHECK(hipMemcpyAsync(&sum, r->loss_dev, sizeof(double), hipMemcpyDeviceToHost, st), "loss copy");
+    HIP_CHECK(hipStreamSynchronize(st), "hipStreamSynchronize");
+    *loss = sum / (double)r->M;  // mean over the splat cells
+    r->forward_done = true;
+    return ISF_OK;
+}
+
+int isf_rollout_backward(isf_rollout_t r, double grad_velocity[3], float* grad_theta, float* grad_psi_init,
+                         float* grad_parts_init) {
+    CHECK_R(r);
+    if (!r->forward_done) return fail(ISF_EINVAL, "isf_rollout_forward must run first");
+    if (grad_theta && !r->use_theta) return fail(ISF_EINVAL, "the last forward had no phase field");
+    isf_t h = r->h;
+    int e = join(h);
+    if (e) return e;
+    hipStream_t st = h->stream;
+    const GridInfo& g = r->g;
+    LAUNCH_CHECK(launch_fill_psi(r->psibar, make_float4(0.0f, 0.0f, 0.0f, 0.0f), r->N, st), "fill");
+    LAUNCH_CHECK(launch_fill_f64(r->vacc, 0.0, 4 * (size_t)(r->T + r->C), st), "fill");
+    LAUNCH_CHECK(launch_splat_adj(r->parts_final, r->P, r->sc, r->resid, r->pbar, st), "loss adjoint");
+    for (int s = r->nseg - 1; s >= 0; s--) {
+        if ((e = seg_forward(r, s))) return e;
+        for (int k = seg_steps(r, s) - 1; k >= 0; k--)
+            if ((e = step_adjoint(r, s, k))) return e;
+    }
+    // set-up constraint iterations in reverse
+    for (int j = r->C - 1; j >= 0; j--) {
+        const JetParams j0 = jet_at(r, 0);
+        LAUNCH_CHECK(launch_gauge_qbar(g, r->psibar, r->cpsi[j + 1], r->f, st), "gauge adjoint");
+        if ((e = poisson(r, r->f, r->fbar))) return e;
+        LAUNCH_CHECK(launch_restrict_to(g, r->cpsi[j], r->W, j0, st), "restrict_velocity");
+        LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, r->W, r->cq[j], r->fbar, st), "projection adjoint");
+        LAUNCH_CHECK(launch_restrict_adj(g, r->psibar, r->cpsi[j], j0, r->partials,
+                                         r->vacc + 4 * (size_t)(r->T + j), st),
+                     "restrict adjoint");
+    }
+    if (r->use_theta) {
+        LAUNCH_CHECK(launch_apply_phase_adj(g, r->psibar, r->cpsi[0], r->theta, r->phibar, r->d.phase_res, st),
+                     "phase adjoint");
+        if (grad_theta) LAUNCH_CHECK(launch_phase_gather(g, r->phibar, grad_theta, r->d.phase_res, st), "phase gather");
+    }
+    if (grad_psi_init)
+        HIP_CHECK(hipMemcpyAsync(grad_psi_init, r->psibar, sizeof(float4) * r->N, hipMemcpyDeviceToDevice, st), "copy");
+    if (grad_parts_init && r->P > 0)
+        HIP_CHECK(hipMemcpyAsync(grad_parts_init, r->pbar, sizeof(float) * 3 * r->P, hipMemcpyDeviceToDevice, st), "copy");
+    std::vector<double> acc(4 * (size_t)(r->T + r->C));
+    HIP_CHECK(hipMemcpyAsync(acc.data(), r->vacc, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, st), "copy");
+    HIP_CHECK(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (grad_velocity) {
+        // phi = k . pos - omega t (mode 0) or sum(k .+ pos) - omega t (mode 1), k = v / hbar,
+        // omega = |v|^2 / (2 hbar):  d phi / d v_a = (pos_a or 1 - t v_a) / hbar
+        const double hb = (double)h->hbar;
+        double gv[3] = {0.0, 0.0, 0.0};
+        for (int slot = 0; slot < r->T + r->C; slot++) {
+            const double* S = acc.data() + 4 * slot;
+            const double t = slot < r->T ? (double)((float)(slot + 1) * h->dt) : 0.0;
+            for (int a = 0; a < 3; a++) {
+                const double lin = r->jet.phase_mode == 1 ? S[0] : S[1 + a];
+                gv[a] += (lin - t * (double)r->v[a] * S[0]) / hb;
+            }
+        }
+        for (int a = 0; a < 3; a++) grad_velocity[a] = gv[a];
+    }
+    return ISF_OK;
+}
+
+int isf_rollout_state(isf_rollout_t r, const float** psi, const float** parts, const double** rho) {
+    CHECK_R(r);
+    if (!r->forward_done) return fail(ISF_EINVAL, "isf_rollout_forward must run first");
+    if (psi) *psi = reinterpret_cast<const float*>(r->ck_psi[r->nseg]);
+    if (parts) *parts = r->parts_final;
+    if (rho) *rho = r->rho;
+    return ISF_OK;
+}
+
+
+// ------------------------------------------------------------------ operator VJPs
+
+}  // extern "C"
+
+// Scratch of the operator VJPs (psi copy, q, f, fbar), allocated on first use.
+static int vjp_scratch(isf_t h) {
+    if (h->vjp_mem) return ISF_OK;
+    const hipError_t e = hipMalloc(&h->vjp_mem, 16 * h->N + 12 * h->N + 4 * 4 * restrict_adj_blocks(h->g) * 8);
+    if (e != hipSuccess) return fail(ISF_ENOMEM, "vjp scratch: %s", hipGetErrorString(e));
+    return ISF_OK;
+}
+static float4* vjp_psi(isf_t h) { return static_cast<float4*>(h->vjp_mem); }
+static float* vjp_q(isf_t h) { return reinterpret_cast<float*>(static_cast<char*>(h->vjp_mem) + 16 * h->N); }
+static float* vjp_f(isf_t h) { return vjp_q(h) + h->N; }
+static float* vjp_fbar(isf_t h) { return vjp_q(h) + 2 * h->N; }
+static double* vjp_partials(isf_t h) {
+    return reinterpret_cast<double*>(static_cast<char*>(h->vjp_mem) + 28 * h->N);
+}
+static float2* hP(isf_t h) { return reinterpret_cast<float2*>(reinterpret_cast<char*>(h->S) + 4 * h->N); }
+
+extern "C" {
+
+int isf_vjp_schroedinger_flow(isf_t h, float* psibar) {
+    int e = check_handle(h);
+    if (!e) e = join(h);
+    if (e) return e;
+    CHECK_PTR(psibar, "psibar");
+    float4* b = reinterpret_cast<float4*>(psibar);
+    LAUNCH_CHECK(launch_flow_x_fwd(h->g, b, h->S, h->tb, h->stream), "flow adjoint x forward");
+    LAUNCH_CHECK(launch_flow_plane_adj(h->g, h->S, h->tb, h->stream), "flow adjoint plane");
+    LAUNCH_CHECK(launch_flow_x_inv(h->g, h->S, b, false, h->tb, h->stream), "flow adjoint x inverse");
+    return ISF_OK;
+}
+
+int isf_vjp_normalize(isf_t h, const float* psi_in, float* psibar) {
+    int e = check_handle(h);
+    if (!e) e = join(h);
+    if (e) return e;
+    CHECK_PTR(psi_in, "psi_in");
+    CHECK_PTR(psibar, "psibar");
+    LAUNCH_CHECK(launch_normalize_adj(h->g, reinterpret_cast<float4*>(psibar), reinterpret_cast<const float4*>(psi_in),
+                                      h->stream),
+                 "normalize adjoint");
+    return ISF_OK;
+}
+
+int isf_vjp_pressure_project(isf_t h, const float* psi_in, float* psibar) {
+    int e = check_handle(h);
+    if (!e) e = join(h);
+    if (!e) e = vjp_scratch(h);
+    if (e) return e;
+    CHECK_PTR(psi_in, "psi_in");
+    CHECK_PTR(psibar, "psibar");
+    hipStream_t st = h->stream;
+    const double sc = -0.25 * 2.0 / (double)h->N;
+    const float4* in = reinterpret_cast<const float4*>(psi_in);
+    float4* b = reinterpret_cast<float4*>(psibar);
+    float4* out = vjp_psi(h);
+    // forward again, keeping q and the output
+    HIP_CHECK(hipMemcpyAsync(out, in, 16 * h->N, hipMemcpyDeviceToDevice, st), "copy");
+    LAUNCH_CHECK(launch_div_psi(h->g, out, vjp_f(h), st), "div(one_form(psi))");
+    LAUNCH_CHECK(launch_r2c_x(h->g, vjp_f(h), hP(h), h->tb, st), "poisson x forward");
+    LAUNCH_CHECK(launch_poisson_plane(h->g, hP(h), sc, h->tb, st), "poisson plane");
+    LAUNCH_CHECK(launch_c2r_x(h->g, hP(h), 1, vjp_q(h), out, nullptr, nullptr, h->tb, st), "poisson x inverse + gauge");
+    // reverse
+    LAUNCH_CHECK(launch_gauge_qbar(h->g, b, out, vjp_f(h), st), "gauge adjoint");
+    LAUNCH_CHECK(launch_r2c_x(h->g, vjp_f(h), hP(h), h->tb, st), "poisson x forward");
+    LAUNCH_CHECK(launch_poisson_plane(h->g, hP(h), sc, h->tb, st), "poisson plane");
+    LAUNCH_CHECK(launch_c2r_x(h->g, hP(h), 0, vjp_fbar(h), nullptr, nullptr, nullptr, h->tb, st), "poisson x inverse");
+    LAUNCH_CHECK(launch_oneform_adj_div(h->g, b, in, vjp_q(h), vjp_fbar(h), st), "projection adjoint");
+    return ISF_OK;
+}
+
+int isf_vjp_restrict_velocity(isf_t h, const float* psi_in, float* psibar, const float kvec[3], float omgterm,
+                              int phase_mode, double dphase[4]) {
+    int e = check_handle(h);
+    if (!e) e = join(h);
+    if (!e) e = vjp_scratch(h);
+    if (e) return e;
+    CHECK_PTR(psi_in, "psi_in");
+    CHECK_PTR(psibar, "psibar");
+    CHECK_PTR(kvec, "kvec");
+    if (!h->jet_set) return fail(ISF_EINVAL, "isf_jet_configure must be called first (nozzle)");
+    if (phase_mode != ISF_PHASE_DOT && phase_mode != ISF_PHASE_LEGACY) return fail(ISF_EINVAL, "bad phase_mode %d", phase_mode);
+    JetParams j = h->jet;
+    j.kx = kvec[0];
+    j.ky = kvec[1];
+    j.kz = kvec[2];
+    j.omgterm = omgterm;
+    j.phase_mode = phase_mode;
+    double* acc = vjp_partials(h);
+    double* parts = acc + 4;
+    LAUNCH_CHECK(launch_fill_f64(acc, 0.0, 4, h->stream), "fill");
+    LAUNCH_CHECK(launch_restrict_adj(h->g, reinterpret_cast<float4*>(psibar), reinterpret_cast<const float4*>(psi_in), j,
+                                     parts, acc, h->stream),
+                 "restrict adjoint");
+    if (dphase) {
+        HIP_CHECK(hipMemcpyAsync(dphase, acc, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream), "copy");
+        HIP_CHECK(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+    }
+    return ISF_OK;
+}
+
+int isf_vjp_velocity_one_form(isf_t h, const float* psi, float hbar, int sharp, const float* velbar, float* psibar) {
+    int e = check_handle(h);
+    if (!e) e = join(h);
+    if (e) return e;
+    CHECK_PTR(psi, "psi");
+    CHECK_PTR(velbar, "velbar");
+    CHECK_PTR(psibar, "psibar");
+    LAUNCH_CHECK(launch_oneform_adj_vel(h->g, reinterpret_cast<float4*>(psibar), reinterpret_cast<const float4*>(psi),
+                                        velbar, hbar, sharp, h->stream),
+                 "velocity adjoint");
+    return ISF_OK;
+}
+
+int isf_vjp_staggered_advect(isf_t h, const float* parts0, long long nslots, const float* vel, float* pbar,
+                             float* velbar) {
+    int e = check_handle(h);
+    if (!e) e = join(h);
+    if (e) return e;
+    CHECK_PTR(parts0, "parts0");
+    CHECK_PTR(pbar, "pbar");
+    CHECK_PTR(velbar, "velbar");
+    if (nslots < 0) return fail(ISF_EINVAL, "bad nslots");
+    LAUNCH_CHECK(launch_advect_adj(h->g, parts0, nslots, vel ? vel : h->vel, h->dt, pbar, velbar, h->stream),
+                 "advect adjoint");
+    return ISF_OK;
+}
+
+}  // extern "C"

@@ -24,6 +24,14 @@ class MemberDesc(C.Structure):
                 ("kvec", C.c_float * 3), ("omega", C.c_float), ("cen", C.c_float * 3), ("len", C.c_float),
                 ("rad", C.c_float), ("n_spawn", C.c_longlong), ("seed", C.c_ulonglong), ("phase_mode", C.c_int)]
 
+class RolloutDesc(C.Structure):
+    """isf_rollout_desc of include/isf.h."""
+    _fields_ = [("steps", C.c_int), ("checkpoint_every", C.c_int), ("constraint_iters", C.c_int),
+                ("phase_res", C.c_int * 3), ("splat_res", C.c_int * 3), ("splat_sigma", C.c_float),
+                ("splat_cutoff", C.c_float), ("nslots", C.c_longlong), ("n_init", C.c_longlong),
+                ("fill_seed", C.c_ulonglong)]
+
+
 # (name, argtypes) of every exported entry point of include/isf.h
 SIGNATURES = {
     "isf_last_error": ([], C.c_char_p),
@@ -73,6 +81,19 @@ SIGNATURES = {
     "isf_batch_synchronize": ([C.c_void_p], C.c_int),
     "isf_batch_tune": ([C.c_void_p, C.c_char_p], C.c_int),
     "isf_batch_tile": ([C.c_void_p, c_int3, C.POINTER(C.c_int)], C.c_int),
+    "isf_vjp_schroedinger_flow": ([C.c_void_p, C.c_void_p], C.c_int),
+    "isf_vjp_normalize": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_vjp_pressure_project": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_vjp_restrict_velocity": ([C.c_void_p, C.c_void_p, C.c_void_p, c_float3, C.c_float, C.c_int,
+                                   C.POINTER(C.c_double)], C.c_int),
+    "isf_vjp_velocity_one_form": ([C.c_void_p, C.c_void_p, C.c_float, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_vjp_staggered_advect": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_rollout_create": ([C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(RolloutDesc)], C.c_int),
+    "isf_rollout_destroy": ([C.c_void_p], C.c_int),
+    "isf_rollout_set_target": ([C.c_void_p, C.c_void_p], C.c_int),
+    "isf_rollout_forward": ([C.c_void_p, c_float3, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "isf_rollout_backward": ([C.c_void_p, C.POINTER(C.c_double), C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_rollout_state": ([C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)], C.c_int),
     "isf_timer_start": ([C.c_void_p], C.c_int),
     "isf_timer_stop": ([C.c_void_p, C.POINTER(C.c_float)], C.c_int),
 }

@@ -247,6 +247,36 @@ class ISF:
         check(self.lib.isf_step_norm2(self.h, C.byref(out)), "isf_step_norm2")
         return out.value
 
+    # ---- vector-Jacobian products (cotangents in the layouts of the primal arrays)
+    def vjp_schroedinger_flow_(self, psibar):
+        check(self.lib.isf_vjp_schroedinger_flow(self.h, _ptr(psibar)), "isf_vjp_schroedinger_flow")
+        return psibar
+
+    def vjp_normalize_(self, psi_in, psibar):
+        check(self.lib.isf_vjp_normalize(self.h, _ptr(psi_in), _ptr(psibar)), "isf_vjp_normalize")
+        return psibar
+
+    def vjp_pressure_project_(self, psi_in, psibar):
+        check(self.lib.isf_vjp_pressure_project(self.h, _ptr(psi_in), _ptr(psibar)), "isf_vjp_pressure_project")
+        return psibar
+
+    def vjp_restrict_velocity_(self, psi_in, psibar, kvec, omgterm=0.0, phase_mode=0):
+        """psibar in place; returns (sum phibar, sum phibar x, sum phibar y, sum phibar z)."""
+        out = (C.c_double * 4)()
+        check(self.lib.isf_vjp_restrict_velocity(self.h, _ptr(psi_in), _ptr(psibar), c_float3(*[float(v) for v in kvec]),
+                                                 float(omgterm), int(phase_mode), out), "isf_vjp_restrict_velocity")
+        return tuple(out)
+
+    def vjp_velocity_one_form_(self, psi, velbar, psibar, hbar=1.0, sharp=False):
+        check(self.lib.isf_vjp_velocity_one_form(self.h, _ptr(psi), float(hbar), int(bool(sharp)), _ptr(velbar),
+                                                 _ptr(psibar)), "isf_vjp_velocity_one_form")
+        return psibar
+
+    def vjp_staggered_advect_(self, parts0, vel, pbar, velbar):
+        check(self.lib.isf_vjp_staggered_advect(self.h, _ptr(parts0), parts0.shape[0], _ptr(vel), _ptr(pbar),
+                                                _ptr(velbar)), "isf_vjp_staggered_advect")
+        return pbar, velbar
+
     def timer_start(self):
         check(self.lib.isf_timer_start(self.h), "isf_timer_start")
 
