@@ -165,6 +165,56 @@ def bench_ensemble(steps, warmup, count=48, seed=20261203):
     }
 
 
+def bench_fwd_adjoint(reps):
+    """Differentiable rollout: 128x64x64 jet (box 8x4x4, hbar 0.1, dt 1/48), 32 steps with
+    checkpoints every 4, 100k particle slots filled in the nozzle and 1000 spawned per step,
+    16^3 seeded initial phase field, loss = MSE of the Gaussian-splatted particle density
+    (32x16x16 grid, sigma 0.25, cutoff 4 sigma) against a sum of 3 seeded Gaussians.
+    Timed: `reps` forward + backward passes (set-up constraint iterations included),
+    wall clock, synchronised; steps/s counts the 32 simloop steps of each pass."""
+    from isf_amd import ISF
+    from isf_amd.rollout import Rollout, gaussian_target
+    dims, box, steps = (128, 64, 64), (8, 4, 4), 32
+    hb = np.float32(0.1)
+    v = np.array([1.0, 0.0, 0.0], np.float32)
+    isf = ISF(box, dims, hb, 1.0 / 48.0)
+    isf.jet_configure(v / hb, np.float32(np.sum(v ** np.float32(2)) / (np.float32(2) * hb)),
+                      np.array([0.3, 2 - 0.034, 2 + 0.066], np.float32), 0.5, 0.5, 1000, 20261203, 0)
+    r = Rollout(isf, steps, checkpoint_every=4, constraint_iters=10, phase_res=(16, 16, 16), splat_res=(32, 16, 16),
+                splat_sigma=0.25, splat_cutoff=4.0, nslots=100_000, n_init=100_000, fill_seed=20261203)
+    r.set_target(gaussian_target(box, (32, 16, 16), 20261203, centre_lo=(0.3, 1.5, 1.5), centre_hi=(2.0, 2.5, 2.5)))
+    th = (0.2 * np.random.default_rng(20261203).standard_normal((16, 16, 16))).astype(np.float32)
+    r.forward(v, theta=th)
+    r.backward(want_psi=False, want_parts=False)  # warm-up
+    tf = tb = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        loss = r.forward(v, theta=th)
+        t1 = time.perf_counter()
+        g = r.backward(want_psi=False, want_parts=False)
+        t2 = time.perf_counter()
+        tf += t1 - t0
+        tb += t2 - t1
+    r.close()
+    isf.close()
+    tf /= reps
+    tb /= reps
+    print(f"[bench] rollout 128x64x64 x 32 steps: forward {tf * 1e3:.2f} ms, backward {tb * 1e3:.2f} ms",
+          file=sys.stderr, flush=True)
+    return {
+        "workload": "128x64x64 jet, 32 steps, checkpoint every 4, 100k particles, 16^3 phase field, splat MSE loss",
+        "fwd_adjoint_steps_s": round(steps / (tf + tb), 2),
+        "fwd_steps_s": round(steps / tf, 2),
+        "ms_forward": round(tf * 1e3, 3),
+        "ms_backward": round(tb * 1e3, 3),
+        "loss": loss,
+        "grad_velocity": [float(x) for x in g["velocity"]],
+        "grad_theta_norm": float(np.linalg.norm(g["theta"])),
+        "reps": reps,
+        "timing": "wall clock per pass (set-up and constraint iterations included), synchronised",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +224,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--batch-steps", type=int, default=60)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint rollout leg")
+    ap.add_argument("--adjoint-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -244,6 +296,9 @@ def main():
     if not args.no_batch:
         print("[bench] batched ensemble leg", file=sys.stderr, flush=True)
         out["batched"] = bench_ensemble(args.batch_steps, 5)
+    if not args.no_adjoint:
+        print("[bench] forward+adjoint rollout leg", file=sys.stderr, flush=True)
+        out["fwd_adjoint"] = bench_fwd_adjoint(args.adjoint_reps)
     if not args.no_cpu:
         print("[bench] CPU baseline leg", file=sys.stderr, flush=True)
         cb = cpu_baseline(w)

@@ -188,28 +188,34 @@ class TorchRollout:
         out = ty[:, :, x0] * (1 - fx)[None, None, :] + ty[:, :, x1] * fx[None, None, :]
         return out
 
-    def splat(self, parts):
+    def splat(self, parts, chunk=4096):
         h = [self.L[a] / self.splat_res[a] for a in range(3)]
         Lt = torch.tensor(self.L, dtype=torch.float64)
+        ht = torch.tensor(h, dtype=torch.float64)
         live = ((parts > 0) & (parts < Lt)).all(1)
-        p = parts[live]
+        P = parts[live]
         R = self.cutoff * self.sigma
-        K = [int(math.ceil(R / h[a])) + 1 for a in range(3)]
-        base = torch.floor(p.detach() / torch.tensor(h, dtype=torch.float64) - 0.5).long()
+        # cells j with |(j + 1/2) h - p| <= R lie within base - K .. base + K + 1,
+        # base = floor(p / h - 1/2), K = ceil(R / h)
+        K = [int(math.ceil(R / h[a])) for a in range(3)]
         offs = torch.stack(torch.meshgrid(*[torch.arange(-K[a], K[a] + 2) for a in range(3)], indexing="ij"), -1)
         offs = offs.reshape(-1, 3)
-        idx = base[:, None, :] + offs[None, :, :]                     # (P, M, 3)
         res = torch.tensor(self.splat_res)
-        inside = ((idx >= 0) & (idx < res)).all(-1)
-        y = (idx.to(torch.float64) + 0.5) * torch.tensor(h, dtype=torch.float64)
-        r2 = ((y - p[:, None, :]) ** 2).sum(-1)
-        ok = inside & (r2 <= R * R)
-        g = torch.exp(-r2 / (2 * self.sigma ** 2)) * ok
-        flat = idx[..., 0] + self.splat_res[0] * (idx[..., 1] + self.splat_res[1] * idx[..., 2])
-        flat = torch.where(ok, flat, torch.zeros_like(flat))
         M = self.splat_res[0] * self.splat_res[1] * self.splat_res[2]
-        rho = torch.zeros(M, dtype=torch.float64).index_add(0, flat.reshape(-1), g.reshape(-1)) / self.nslots
-        return rho.reshape(self.splat_res[::-1])
+        rho = torch.zeros(M, dtype=torch.float64)
+        for c0 in range(0, P.shape[0], chunk):
+            p = P[c0:c0 + chunk]
+            base = torch.floor(p.detach() / ht - 0.5).long()
+            idx = base[:, None, :] + offs[None, :, :]                 # (P, M, 3)
+            inside = ((idx >= 0) & (idx < res)).all(-1)
+            y = (idx.to(torch.float64) + 0.5) * ht
+            r2 = ((y - p[:, None, :]) ** 2).sum(-1)
+            ok = inside & (r2 <= R * R)
+            g = torch.exp(-r2 / (2 * self.sigma ** 2)) * ok
+            flat = idx[..., 0] + self.splat_res[0] * (idx[..., 1] + self.splat_res[1] * idx[..., 2])
+            flat = torch.where(ok, flat, torch.zeros_like(flat))
+            rho = rho.index_add(0, flat.reshape(-1), g.reshape(-1))
+        return (rho / self.nslots).reshape(self.splat_res[::-1])
 
     # ---- rollout
     def initial_psi(self, kvec, theta, psi_init=None):
