@@ -12,6 +12,7 @@ module SchroedingersSmokeAMD
 
 export ISF, ISFArray, normalize_psi!, pressure_project!, velocity_one_form!, schroedinger_flow!
 export staggered_advect!, add_particles!, restrict_velocity!, isjet, synchronize
+export ISFBatch, ISFRollout, rollout_loss, rollout_gradient
 
 const libisf = joinpath(@__DIR__, "..", "lib", "libisf.so")
 
@@ -155,5 +156,144 @@ jet_configure!(isf::ISF, kvec, omega, nozzle_cen, nozzle_len, nozzle_rad, n_spaw
 step!(isf::ISF, psi, particles::ISFArray, iter::Integer) =
     check(ccall((:isf_step, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Clonglong, Cint),
                 isf.h, devptr(psi), particles.ptr, length(particles), Cint(iter)), "isf_step")
+
+# ---------------------------------------------------------------- batched uneven ensembles
+"Mirror of `isf_member_desc` (include/isf.h)."
+struct MemberDesc
+    res::NTuple{3,Cint}
+    box::NTuple{3,Cint}
+    hbar::Cfloat
+    dt::Cfloat
+    kvec::NTuple{3,Cfloat}
+    omega::Cfloat
+    cen::NTuple{3,Cfloat}
+    len::Cfloat
+    rad::Cfloat
+    n_spawn::Clonglong
+    seed::Culonglong
+    phase_mode::Cint
+end
+
+"""
+    ISFBatch(members::Vector{MemberDesc})
+
+Independent jets of mixed grid sizes stepped together (`isf_batch_*`): one hipGraph
+replay per ensemble step.  `member(b, i)` gives the i-th member's handle pointer for
+set-up; `bind!` attaches its psi and particle arrays.
+"""
+mutable struct ISFBatch
+    h::Ptr{Cvoid}
+    n::Int
+    bound::Vector{Any}
+    function ISFBatch(members::Vector{MemberDesc})
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:isf_batch_create, libisf), Cint, (Ref{Ptr{Cvoid}}, Cint, Ptr{MemberDesc}), h,
+                    length(members), members), "isf_batch_create")
+        b = new(h[], length(members), Any[nothing for _ in members])
+        finalizer(x -> ccall((:isf_batch_destroy, libisf), Cint, (Ptr{Cvoid},), x.h), b)
+        b
+    end
+end
+function member(b::ISFBatch, i::Integer)
+    p = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:isf_batch_member, libisf), Cint, (Ptr{Cvoid}, Cint, Ref{Ptr{Cvoid}}), b.h, i - 1, p),
+          "isf_batch_member")
+    p[]
+end
+function bind!(b::ISFBatch, i::Integer, psi::ISFArray, particles::ISFArray)
+    check(ccall((:isf_batch_bind, libisf), Cint, (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Clonglong), b.h, i - 1,
+                psi.ptr, particles.ptr, length(particles)), "isf_batch_bind")
+    b.bound[i] = (psi, particles)
+    b
+end
+step!(b::ISFBatch, iter::Integer) =
+    check(ccall((:isf_batch_step, libisf), Cint, (Ptr{Cvoid}, Cint), b.h, iter), "isf_batch_step")
+tune!(b::ISFBatch, cache::AbstractString = "") =
+    check(ccall((:isf_batch_tune, libisf), Cint, (Ptr{Cvoid}, Cstring), b.h, isempty(cache) ? C_NULL : cache),
+          "isf_batch_tune")
+synchronize(b::ISFBatch) = check(ccall((:isf_batch_synchronize, libisf), Cint, (Ptr{Cvoid},), b.h), "isf_batch_synchronize")
+
+# ---------------------------------------------------------------- vector-Jacobian products
+# One per operator (isf_vjp_*): the cotangent of the output becomes that of the input, in place.
+vjp_schroedinger_flow!(isf::ISF, psibar) =
+    check(ccall((:isf_vjp_schroedinger_flow, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), isf.h, devptr(psibar)), "vjp")
+vjp_normalize!(isf::ISF, psi_in, psibar) =
+    check(ccall((:isf_vjp_normalize, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}), isf.h, devptr(psi_in),
+                devptr(psibar)), "vjp")
+vjp_pressure_project!(isf::ISF, psi_in, psibar) =
+    check(ccall((:isf_vjp_pressure_project, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}), isf.h,
+                devptr(psi_in), devptr(psibar)), "vjp")
+
+# ---------------------------------------------------------------- differentiable rollout
+"Mirror of `isf_rollout_desc` (include/isf.h)."
+struct RolloutDesc
+    steps::Cint
+    checkpoint_every::Cint
+    constraint_iters::Cint
+    phase_res::NTuple{3,Cint}
+    splat_res::NTuple{3,Cint}
+    splat_sigma::Cfloat
+    splat_cutoff::Cfloat
+    nslots::Clonglong
+    n_init::Clonglong
+    fill_seed::Culonglong
+end
+
+"""
+    ISFRollout(isf, desc, target)
+
+The driver's set-up and `steps` simloop bodies as a function of the nozzle velocity and
+a coarse initial phase field, with a splat-density loss against `target`
+(an `ISFArray{Float32,3}` of `desc.splat_res`).
+"""
+mutable struct ISFRollout
+    h::Ptr{Cvoid}
+    isf::ISF
+    desc::RolloutDesc
+    function ISFRollout(isf::ISF, desc::RolloutDesc, target::ISFArray)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:isf_rollout_create, libisf), Cint, (Ref{Ptr{Cvoid}}, Ptr{Cvoid}, Ref{RolloutDesc}), h, isf.h,
+                    desc), "isf_rollout_create")
+        r = new(h[], isf, desc)
+        finalizer(x -> ccall((:isf_rollout_destroy, libisf), Cint, (Ptr{Cvoid},), x.h), r)
+        check(ccall((:isf_rollout_set_target, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), r.h, target.ptr),
+              "isf_rollout_set_target")
+        r
+    end
+end
+
+"Loss of one rollout for nozzle velocity `v` and phase field `theta` (an ISFArray or nothing)."
+function rollout_loss(r::ISFRollout, v::NTuple{3,<:Real}, theta)
+    loss = Ref{Cdouble}(0)
+    check(ccall((:isf_rollout_forward, libisf), Cint, (Ptr{Cvoid}, Ptr{Cfloat}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{Cdouble}),
+                r.h, Cfloat[v...], devptr(theta), C_NULL, loss), "isf_rollout_forward")
+    loss[]
+end
+
+"Gradients of the last `rollout_loss`: (dv::NTuple{3,Float64}, dtheta::ISFArray or nothing)."
+function rollout_gradient(r::ISFRollout, theta)
+    gv = zeros(Cdouble, 3)
+    gth = theta === nothing ? nothing : ISFArray{Float32,3}(size(theta))
+    check(ccall((:isf_rollout_backward, libisf), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                r.h, gv, devptr(gth), C_NULL, C_NULL), "isf_rollout_backward")
+    (Tuple(gv), gth)
+end
+
+# ChainRulesCore.rrule of rollout_loss, the Julia counterpart of a torch.autograd.Function:
+# defined when ChainRulesCore is loaded in the session.
+function __init__()
+    if isdefined(Main, :ChainRulesCore)
+        CRC = getfield(Main, :ChainRulesCore)
+        @eval function $(CRC).rrule(::typeof(rollout_loss), r::ISFRollout, v, theta)
+            y = rollout_loss(r, v, theta)
+            function rollout_pullback(ybar)
+                gv, gth = rollout_gradient(r, theta)
+                (($CRC).NoTangent(), ($CRC).NoTangent(), ybar .* gv,
+                 gth === nothing ? ($CRC).NoTangent() : (ybar, gth))
+            end
+            y, rollout_pullback
+        end
+    end
+end
 
 end # module
