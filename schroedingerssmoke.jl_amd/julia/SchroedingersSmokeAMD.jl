@@ -288,8 +288,10 @@ function __init__()
             y = rollout_loss(r, v, theta)
             function rollout_pullback(ybar)
                 gv, gth = rollout_gradient(r, theta)
+                # the theta cotangent stays on the device; it is returned for ybar == 1 only
+                gth === nothing || ybar == 1 || error("rollout_loss: device theta cotangent needs ybar == 1")
                 (($CRC).NoTangent(), ($CRC).NoTangent(), ybar .* gv,
-                 gth === nothing ? ($CRC).NoTangent() : (ybar, gth))
+                 gth === nothing ? ($CRC).NoTangent() : gth)
             end
             y, rollout_pullback
         end
