@@ -12,9 +12,15 @@ cylinder (Philox4x32-10, seed 20261203), 1000 ring-spawned particles per step.
 Inputs are resident in device memory before the timed region starts.
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import re
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -215,6 +221,94 @@ def bench_fwd_adjoint(reps):
     }
 
 
+# ---------------------------------------------------------------- HBM traffic per kernel
+# Two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) over a short
+# child run of this script (--traffic-probe), started before this process touches the GPU.
+# Each counter is scaled by what it reads for k_normalize, a streaming kernel that reads
+# and writes exactly 16 N bytes (64 MiB each way at 256x128x128).
+
+def traffic_probe(w):
+    """Child of the counter passes: psi normalize (the calibration kernel) and two steps."""
+    from isf_amd import ISF, DeviceArray
+    f32 = np.float32
+    isf = ISF(w["vol_size"], w["dims"], w["hbar"], w["dt"])
+    nx, ny, nz = w["dims"]
+    psi_h = np.zeros((nz, ny, nx, 4), f32)
+    psi_h[..., 0] = 1.0
+    psi_h[..., 2] = f32(0.01)
+    psi = isf.psi_array(psi_h)
+    isf.normalize_(psi)
+    v = np.array(w["jet_velocity"], f32)
+    hb = f32(w["hbar"])
+    isf.jet_configure(v / hb, f32(np.sum(v ** f32(2)) / (f32(2) * hb)),
+                      np.array(w["nozzle_cen"], np.float64).astype(f32), w["nozzle_len"], w["nozzle_rad"],
+                      w["n_spawn"], w["seed"], w["phase_mode"])
+    parts = DeviceArray.from_host(np.full((100_000, 3), -1.0, f32))
+    isf.fill_nozzle_(parts, 0, 100_000, w["seed"])
+    for it in (1, 2):
+        isf.step_(psi, parts, it)
+    isf.synchronize()
+
+
+def _counter_pass(counter, timeout_s=150):
+    rocprof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    out = tempfile.mkdtemp(prefix="isf_pmc_", dir="/tmp")
+    cmd = [rocprof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "probe", "--",
+           sys.executable, os.path.abspath(__file__), "--traffic-probe"]
+    env = dict(os.environ, TMPDIR="/tmp")
+    try:
+        subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                       timeout=timeout_s)
+        files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            return None
+        acc = {}
+        with open(files[0]) as fh:
+            for r in csv.DictReader(fh):
+                acc.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in acc.items()}
+    except (OSError, subprocess.SubprocessError):
+        return None
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+
+
+# stage name -> kernel name pattern (flow planes are k_plane<NY, NZ, 0>, Poisson planes <.., 1>)
+STAGE_KERNELS = {
+    "flow_x_fwd": r"k_flow_x_fwd<", "flow_plane": r"k_plane<\d+, \d+, 0>", "flow_x_inv_normalize": r"k_flow_x_inv<",
+    "p1_div": r"k_div_psi<", "p1_r2c_x": r"k_r2c_x<", "p1_plane": r"k_plane<\d+, \d+, 1>",
+    "p1_c2r_x_gauge_restrict": r"k_c2r_x<\d+, \d+, 2>", "spawn": r"k_spawn", "p2_div": r"k_div_psi<",
+    "p2_r2c_x": r"k_r2c_x<", "p2_plane": r"k_plane<\d+, \d+, 1>", "p2_c2r_x_gauge": r"k_c2r_x<\d+, \d+, 1>",
+    "velocity_sharp": r"k_velocity<", "advect": r"k_advect\(",
+}
+
+
+def measure_traffic(N):
+    """{stage: HBM bytes per launch} from the two counter passes, and the calibration."""
+    fetch = _counter_pass("FETCH_SIZE")
+    write = _counter_pass("WRITE_SIZE") if fetch else None
+    if not fetch or not write:
+        return None, "rocprofv3 counter passes unavailable"
+
+    def pick(table, pat):
+        vals = [v for k, v in table.items() if re.search(pat, k)]
+        return sum(vals) / len(vals) if vals else None
+
+    ref_kib = 16 * N / 1024.0
+    fn, wn = pick(fetch, r"k_normalize\("), pick(write, r"k_normalize\(")
+    if not fn or not wn:
+        return None, "calibration kernel missing from the counter passes"
+    ff, fw = ref_kib / fn, ref_kib / wn
+    out = {}
+    for stage, pat in STAGE_KERNELS.items():
+        f, wv = pick(fetch, pat), pick(write, pat)
+        if f is not None and wv is not None:
+            out[stage] = 1024.0 * (f * ff + wv * fw)
+    note = (f"FETCH_SIZE x {ff:.3f}, WRITE_SIZE x {fw:.3f} (k_normalize, 16N bytes each way); "
+            f"counts L2-to-fabric bytes, Infinity-Cache hits included")
+    return out, note
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,7 +320,12 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=60)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint rollout leg")
     ap.add_argument("--adjoint-reps", type=int, default=3)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(WORKLOAD)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               f"(the multi-GPU slab decomposition is a design, not an implementation)", file=sys.stderr)
@@ -235,6 +334,10 @@ def main():
     w = WORKLOAD
     nx, ny, nz = w["dims"]
     N = nx * ny * nz
+    traffic, traffic_note = None, "not measured (--no-traffic)"
+    if not args.no_traffic:  # child processes, before this one touches the GPU
+        print("[bench] rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE)", file=sys.stderr, flush=True)
+        traffic, traffic_note = measure_traffic(N)
     print(f"[bench] setup {nx}x{ny}x{nz}, {w['n_slots']} particle slots", file=sys.stderr, flush=True)
     isf, psi, parts = setup_gpu(w)
     it = 0
@@ -269,6 +372,10 @@ def main():
     achieved = sb[dom] / (stage_ms[dom] * 1e-3) / 1e9 if stage_ms else 0.0
     stages = {k: {"ms": round(stage_ms[k], 5), "GB": round(sb[k] / 1e9, 4),
                   "GB_s": round(sb[k] / max(stage_ms[k] * 1e-3, 1e-9) / 1e9, 1)} for k in stage_ms}
+    if traffic:
+        for k in stages:
+            if k in traffic:
+                stages[k]["traffic_GB"] = round(traffic[k] / 1e9, 4)
     balg = b_alg_step(N, w["n_slots"])
 
     out = {
@@ -288,7 +395,9 @@ def main():
                                "seeded in the nozzle, 1000 spawned per step, forward step",
                    "grid": [nx, ny, nz], "particles": w["n_slots"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(traffic[dom]) if traffic and dom in traffic else None,
+                     "algorithmic_bytes": sb.get(dom), "traffic_note": traffic_note},
         "step_roofline": {"B_alg_per_step": balg, "achieved_GB_s": round(balg * steps_s / 1e9, 1),
                           "frac": round(balg * steps_s / 1e9 / HBM_PEAK_GBS, 4)},
         "stages": stages,

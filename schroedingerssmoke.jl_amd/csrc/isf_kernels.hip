@@ -404,13 +404,37 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
 // ---- z-marching stencil tiles
 // A block of BX x 4 threads covers BX consecutive x and 4 consecutive y and walks TZ
 // consecutive z.  psi(c + e_z) of one z step is psi(c) of the next, psi(c + e_x) comes
-// from the neighbouring lane, so every forward-edge angle
+// from the neighbouring lane and psi(c + e_y) from the row above through LDS, so every
+// psi row is fetched once per block (plus one halo row on each side in y and z) and
+// every forward-edge angle
 // v_a(c) = angle(conj psi1(c) psi1(c+e_a) + conj psi2(c) psi2(c+e_a)) (src/parallel.jl:100-113)
-// is evaluated once per cell (plus a thin halo).
-constexpr int ST_BY = 4, ST_TZ = 8;
+// is evaluated once per cell (plus a thin halo).  Blocks are mapped XCD-aware: the
+// hardware deals consecutive block ids to the 8 XCDs in turn, so tile t is given to id
+// (t mod C) * 8 + t / C, C = blocks / 8; each XCD then walks one contiguous range of
+// tiles, and the halo rows shared by neighbouring tiles stay in that XCD's L2.
+constexpr int ST_BY = 4, ST_TZ = 16;
 
 __device__ __forceinline__ float4 shfl4(float4 v, int src) {
     return make_float4(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64), __shfl(v.w, src, 64));
+}
+
+// Tile of this block: x tile, y tile and z slab (member-major in batch mode).
+struct StTile {
+    int bx, by, bz;
+};
+__device__ __forceinline__ StTile stencil_tile() {
+    const unsigned T = gridDim.x * gridDim.y * gridDim.z;
+    unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if ((T & 7u) == 0u) {
+        const unsigned C = T >> 3;
+        lin = (lin & 7u) * C + (lin >> 3);
+    }
+    StTile t;
+    t.bx = (int)(lin % gridDim.x);
+    lin /= gridDim.x;
+    t.by = (int)(lin % gridDim.y);
+    t.bz = (int)(lin / gridDim.y);
+    return t;
 }
 
 // Fused velocity_one_form (hbar = 1) + div (src/parallel.jl:100-113,131-143,156-157):
@@ -418,56 +442,67 @@ __device__ __forceinline__ float4 shfl4(float4 v, int src) {
 template <int BX>
 __global__ void __launch_bounds__(BX * ST_BY)
 k_div_psi(const float4* __restrict__ psi, float* __restrict__ f, GridInfo g, Batch B) {
+    __shared__ float4 row_s[2][ST_BY][BX];
     __shared__ float vy_s[2][ST_BY][BX];
-    int zslab = blockIdx.z;
+    const StTile tl = stencil_tile();
+    int zslab = tl.bz;
     if (B.inst) {
         const int nzs = g.nz / ST_TZ;
-        const Inst& m = B.inst[blockIdx.z / nzs];
-        zslab = blockIdx.z % nzs;
+        const Inst& m = B.inst[tl.bz / nzs];
+        zslab = tl.bz % nzs;
         psi = m.psi;
         f = reinterpret_cast<float*>(m.S);
         g = m.g;
     }
     const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
     const int lane = threadIdx.x % 64;
-    const int x = blockIdx.x * BX + tx, y = blockIdx.y * ST_BY + ty, z0 = zslab * ST_TZ;
+    const int x = tl.bx * BX + tx, y = tl.by * ST_BY + ty, z0 = zslab * ST_TZ;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const size_t sy = nx, sz = (size_t)nx * ny;
     const int xp = x + 1 < nx ? x + 1 : 0, xm = x > 0 ? x - 1 : nx - 1;
     const int yp = y + 1 < ny ? y + 1 : 0, ym = y > 0 ? y - 1 : ny - 1;
+    const bool top = ty == ST_BY - 1, bottom = ty == 0;
     const float ds0 = 1.0f / (g.dx * g.dx), ds1 = 1.0f / (g.dy * g.dy), ds2 = 1.0f / (g.dz * g.dz);
     const int zb = z0 > 0 ? z0 - 1 : nz - 1;
-    const size_t cx = x + sy * y, cyp = x + sy * yp;
+    const size_t cx = x + sy * y;
+    // halo rows: y + 1 above the tile (top row), y - 1 below it (bottom row)
+    const size_t chalo = x + sy * (top ? yp : ym);
     float4 pc = psi[cx + sz * z0];
     float vzm = edge_angle(psi[cx + sz * zb], pc);  // v_z(c - e_z) for the first z
     // loads of the next z step are issued one step ahead
     int zp = z0 + 1 < nz ? z0 + 1 : 0;
     float4 pz = psi[cx + sz * zp];
-    float4 py = psi[cyp + sz * z0];
+    float4 ph = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (top | bottom) ph = psi[chalo + sz * z0];
 #pragma unroll 1
     for (int k = 0; k < ST_TZ; k++) {
         const int z = z0 + k;
         const size_t row = sy * y + sz * z;
-        float4 pz_n = pz, py_n = py;
+        float4 pz_n = pz, ph_n = ph;
         if (k + 1 < ST_TZ) {
             const int zn = zp + 1 < nz ? zp + 1 : 0;
             pz_n = psi[cx + sz * zn];
-            py_n = psi[cyp + sz * zp];
+            if (top | bottom) ph_n = psi[chalo + sz * zp];
             zp = zn;
         }
+        row_s[k & 1][ty][tx] = pc;
         float4 px = shfl4(pc, tx + 1 < BX ? lane + 1 : lane);
         if (tx + 1 == BX) px = psi[xp + row];
-        const float vx = edge_angle(pc, px), vy = edge_angle(pc, py), vz = edge_angle(pc, pz);
+        const float vx = edge_angle(pc, px), vz = edge_angle(pc, pz);
         float vxm = __shfl(vx, tx > 0 ? lane - 1 : lane, 64);
         if (tx == 0) vxm = edge_angle(psi[xm + row], pc);
+        __syncthreads();
+        float4 py = row_s[k & 1][top ? ty : ty + 1][tx];
+        if (top) py = ph;
+        const float vy = edge_angle(pc, py);
         vy_s[k & 1][ty][tx] = vy;
         __syncthreads();
-        const float vym = ty > 0 ? vy_s[k & 1][ty - 1][tx] : edge_angle(psi[x + sy * ym + sz * z], pc);
+        const float vym = bottom ? edge_angle(ph, pc) : vy_s[k & 1][ty - (bottom ? 0 : 1)][tx];
         f[x + row] = (vx - vxm) * ds0 + (vy - vym) * ds1 + (vz - vzm) * ds2;
         vzm = vz;
         pc = pz;
         pz = pz_n;
-        py = py_n;
+        ph = ph_n;
     }
 }
 
@@ -476,11 +511,13 @@ k_div_psi(const float4* __restrict__ psi, float* __restrict__ f, GridInfo g, Bat
 template <int BX>
 __global__ void __launch_bounds__(BX * ST_BY)
 k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, float hbar, int sharp, Batch B) {
-    int zslab = blockIdx.z;
+    __shared__ float4 row_s[2][ST_BY][BX];
+    const StTile tl = stencil_tile();
+    int zslab = tl.bz;
     if (B.inst) {
         const int nzs = g.nz / ST_TZ;
-        const Inst& m = B.inst[blockIdx.z / nzs];
-        zslab = blockIdx.z % nzs;
+        const Inst& m = B.inst[tl.bz / nzs];
+        zslab = tl.bz % nzs;
         psi = m.psi;
         vel = m.vel;
         g = m.g;
@@ -488,29 +525,35 @@ k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, 
     }
     const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
     const int lane = threadIdx.x % 64;
-    const int x = blockIdx.x * BX + tx, y = blockIdx.y * ST_BY + ty, z0 = zslab * ST_TZ;
+    const int x = tl.bx * BX + tx, y = tl.by * ST_BY + ty, z0 = zslab * ST_TZ;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const size_t sy = nx, sz = (size_t)nx * ny;
     const int xp = x + 1 < nx ? x + 1 : 0, yp = y + 1 < ny ? y + 1 : 0;
+    const bool top = ty == ST_BY - 1;
     const float s0 = sharp ? 1.0f / g.dx : 1.0f, s1 = sharp ? 1.0f / g.dy : 1.0f, s2 = sharp ? 1.0f / g.dz : 1.0f;
     const size_t cx = x + sy * y, cyp = x + sy * yp;
     float4 pc = psi[cx + sz * z0];
     int zp = z0 + 1 < nz ? z0 + 1 : 0;
     float4 pz = psi[cx + sz * zp];
-    float4 py = psi[cyp + sz * z0];
+    float4 ph = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (top) ph = psi[cyp + sz * z0];
 #pragma unroll 1
     for (int k = 0; k < ST_TZ; k++) {
         const int z = z0 + k;
         const size_t c = cx + sz * z;
-        float4 pz_n = pz, py_n = py;
+        float4 pz_n = pz, ph_n = ph;
         if (k + 1 < ST_TZ) {
             const int zn = zp + 1 < nz ? zp + 1 : 0;
             pz_n = psi[cx + sz * zn];
-            py_n = psi[cyp + sz * zp];
+            if (top) ph_n = psi[cyp + sz * zp];
             zp = zn;
         }
+        row_s[k & 1][ty][tx] = pc;
         float4 px = shfl4(pc, tx + 1 < BX ? lane + 1 : lane);
         if (tx + 1 == BX) px = psi[xp + sy * y + sz * z];
+        __syncthreads();
+        float4 py = row_s[k & 1][top ? ty : ty + 1][tx];
+        if (top) py = ph;
         float v0 = edge_angle(pc, px) * hbar, v1 = edge_angle(pc, py) * hbar, v2 = edge_angle(pc, pz) * hbar;
         if (sharp) {
             v0 = v0 * s0;
@@ -522,7 +565,7 @@ k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, 
         vel[3 * c + 2] = v2;
         pc = pz;
         pz = pz_n;
-        py = py_n;
+        ph = ph_n;
     }
 }
 
