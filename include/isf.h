@@ -53,7 +53,7 @@ int isf_memset(void* dst, int value, size_t bytes);
 
 /* ---- ISF instance ------------------------------------------------------------- */
 /* ISF{ArrayType,UInt32,Float32}(physical_size, dims, hbar, dt), src/parallel.jl:41-98.
- * res: grid_res (powers of two; x in 16..512, y and z in 16..128 with ny*nz <= 16384);
+ * res: grid_res (powers of two; x, y and z in 16..512; batch members y and z up to 128);
  * box: physical_size (integers, src/parallel.jl:12,45).  Builds the Schroedinger mask
  * and Poisson factor (:47-63) as per-axis FP64-derived tables, the FFT twiddles and
  * the scratch arrays. */

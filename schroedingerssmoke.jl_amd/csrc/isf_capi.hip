@@ -136,7 +136,7 @@ int isf_create(isf_t* out, const int res[3], const int box[3], float hbar, float
     }
     if (!x_len_supported(res[0])) return fail(ISF_EINVAL, "grid_res[0] = %d unsupported (16..512)", res[0]);
     if (!plane_supported(res[1], res[2]))
-        return fail(ISF_EINVAL, "grid_res[1:2] = (%d, %d) unsupported (16..128 each, ny*nz <= 16384)", res[1], res[2]);
+        return fail(ISF_EINVAL, "grid_res[1:2] = (%d, %d) unsupported (16..512 each)", res[1], res[2]);
     if (!(hbar > 0.0f) || !(dt > 0.0f)) return fail(ISF_EINVAL, "hbar and dt must be positive");
 
     isf_handle* h = new isf_handle();
@@ -909,6 +909,9 @@ int isf_batch_create(isf_batch_t* out, int n, const isf_member_desc* members) {
     };
     for (int i = 0; i < n; i++) {
         const isf_member_desc& d = members[i];
+        if (!plane_in_lds(d.res[1], d.res[2]))
+            return bail(fail(ISF_EINVAL, "batch member %d: grid_res[1:2] = (%d, %d); batches take y and z up to 128", i,
+                             d.res[1], d.res[2]));
         isf_t h = nullptr;
         int r = isf_create(&h, d.res, d.box, d.hbar, d.dt);
         if (r) return bail(fail(r, "batch member %d: %s", i, std::string(isf::last_error()).c_str()));

@@ -162,5 +162,6 @@ int launch_splat_adj(const float* parts, long long nslots, const SplatCfg& sc, c
 // Supported axis lengths.
 bool x_len_supported(int n);
 bool plane_supported(int ny, int nz);
+bool plane_in_lds(int ny, int nz);
 
 }  // namespace isf

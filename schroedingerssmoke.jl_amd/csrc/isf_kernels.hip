@@ -318,6 +318,84 @@ k_r2c_x(const float* __restrict__ f, float2* __restrict__ P, int ny, int nz, con
     }
 }
 
+// ---- planes too large for one workgroup's LDS (ny or nz above 128)
+// The plane pass is split into a y-line pass, a z-column pass that carries the spectral
+// multiplier between its forward and inverse transforms, and an inverse y-line pass.
+// Same multipliers and arithmetic as k_plane.
+
+template <int NY> struct YLineCfg {
+    static constexpr int E = DefaultE<NY>::value;
+    static constexpr int T = NY / E;                   // threads per line
+    static constexpr int R = T >= 256 ? 1 : 256 / T;   // lines (z rows) per block
+    static constexpr int NT = R * T;
+    static constexpr int LS = padded_len(NY);
+};
+
+// In-place y transforms (DIR -1 forward, +1 inverse) of R consecutive z rows of one plane.
+template <int NY, int DIR>
+__global__ void __launch_bounds__(YLineCfg<NY>::NT)
+k_line_y(float2* __restrict__ base, int nz, const float2* __restrict__ tw) {
+    using C = YLineCfg<NY>;
+    __shared__ float2 lds[C::R * C::LS];
+    __shared__ float2 twl[FftTw<NY, C::E>::SIZE];
+    const int tid = threadIdx.x;
+    float2* rows = base + ((size_t)blockIdx.y * nz + (size_t)blockIdx.x * C::R) * NY;
+    FftTw<NY, C::E>::fill(twl, tw, tid, C::NT);
+    for (int i = tid; i < C::R * NY; i += C::NT) lds[(i / NY) * C::LS + pad16(i % NY)] = rows[i];
+    __syncthreads();
+    fft_line<NY, C::E, DIR>(LayContig{lds + (tid / C::T) * C::LS}, tid % C::T, true, twl);
+    for (int i = tid; i < C::R * NY; i += C::NT) rows[i] = lds[(i / NY) * C::LS + pad16(i % NY)];
+}
+
+template <int NZ> struct ZColCfg {
+    static constexpr int E = DefaultE<NZ>::value;
+    static constexpr int T = NZ / E;                    // threads per column
+    static constexpr int CW = T * 16 >= 256 ? 16 : 256 / T;  // columns per block (>= 16: 128 B rows)
+    static constexpr int NT = CW * T;
+    static constexpr int LS = padded_len(NZ);
+};
+
+// z transforms of CW consecutive y columns of one plane, the multiplier of k_plane, and the
+// inverse z transforms.
+template <int NZ, int MODE>
+__global__ void __launch_bounds__(ZColCfg<NZ>::NT)
+k_line_z(float2* __restrict__ base, int nx, int ny, double scale, DevTables tb) {
+    using C = ZColCfg<NZ>;
+    __shared__ float2 lds[C::CW * C::LS];
+    __shared__ float2 twl[FftTw<NZ, C::E>::SIZE];
+    const int tid = threadIdx.x;
+    const int plane = blockIdx.y, y0 = blockIdx.x * C::CW;
+    float2* P = base + (size_t)plane * ny * NZ;
+    FftTw<NZ, C::E>::fill(twl, tb.tw, tid, C::NT);
+    for (int i = tid; i < C::CW * NZ; i += C::NT) {
+        const int c = i % C::CW, z = i / C::CW;
+        lds[c * C::LS + pad16(z)] = P[(size_t)z * ny + y0 + c];
+    }
+    __syncthreads();
+    const LayContig line{lds + (tid / C::T) * C::LS};
+    fft_line<NZ, C::E, -1>(line, tid % C::T, true, twl);
+    for (int i = tid; i < C::CW * NZ; i += C::NT) {
+        const int c = i % C::CW, kz = i / C::CW, ky = y0 + c;
+        float2& v = lds[c * C::LS + pad16(kz)];
+        if (MODE != PLANE_POISSON) {
+            const int kx = plane % nx;
+            const float2 m = cmul(cmul(tb.mask_x[kx], tb.mask_y[ky]), tb.mask_z[kz]);
+            v = cmul(v, MODE == PLANE_FLOW_ADJ ? cconj(m) : m);
+        } else {
+            const int kx = plane;
+            const float den = (float)((tb.s2x[kx] + tb.s2y[ky]) + tb.s2z[kz]);
+            const float m = (kx == 0 && ky == 0 && kz == 0) ? 0.0f : (float)scale * __builtin_amdgcn_rcpf(den);
+            v = cscale(v, m);
+        }
+    }
+    __syncthreads();
+    fft_line<NZ, C::E, 1>(line, tid % C::T, true, twl);
+    for (int i = tid; i < C::CW * NZ; i += C::NT) {
+        const int c = i % C::CW, z = i / C::CW;
+        P[(size_t)z * ny + y0 + c] = lds[c * C::LS + pad16(z)];
+    }
+}
+
 enum { C2R_Q = 0, C2R_GAUGE = 1, C2R_GAUGE_RESTRICT = 2 };
 
 // Complex-to-real inverse x transform of TY lines of P (src/parallel.jl:172) and, in the
@@ -790,8 +868,10 @@ k_spawn(float* __restrict__ parts, long long nslots, long long first, long long 
 
 bool x_len_supported(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
 
-static bool yz_len_ok(int n) { return n == 16 || n == 32 || n == 64 || n == 128; }
-bool plane_supported(int ny, int nz) { return yz_len_ok(ny) && yz_len_ok(nz) && ny * nz <= 16384; }
+static bool yz_len_ok(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
+bool plane_supported(int ny, int nz) { return yz_len_ok(ny) && yz_len_ok(nz); }
+// planes whose y-z transform runs in one workgroup (k_plane); larger ones are split
+bool plane_in_lds(int ny, int nz) { return ny <= 128 && nz <= 128; }
 
 // y lines per block of the x passes: g.ty when set (8, 16 or 32, not above ny; 32 only
 // for nx <= 256), else 16 (8 for nx = 512)
@@ -919,9 +999,59 @@ static int plane_t(int mode, float2* base, int nplanes, int nx, double scale, co
     }                                                                              \
     return (int)hipErrorInvalidValue;
 
+template <int NY, int DIR>
+static int line_y_t(float2* base, int nplanes, int nz, const DevTables& t, hipStream_t s) {
+    using C = YLineCfg<NY>;
+    if (nz % C::R) return (int)hipErrorInvalidValue;
+    ISF_LAUNCH((k_line_y<NY, DIR>), dim3(nz / C::R, nplanes), dim3(C::NT), s, base, nz, t.tw);
+}
+template <int DIR>
+static int line_y(int ny, float2* base, int nplanes, int nz, const DevTables& t, hipStream_t s) {
+    switch (ny) {
+        case 16: return line_y_t<16, DIR>(base, nplanes, nz, t, s);
+        case 32: return line_y_t<32, DIR>(base, nplanes, nz, t, s);
+        case 64: return line_y_t<64, DIR>(base, nplanes, nz, t, s);
+        case 128: return line_y_t<128, DIR>(base, nplanes, nz, t, s);
+        case 256: return line_y_t<256, DIR>(base, nplanes, nz, t, s);
+        case 512: return line_y_t<512, DIR>(base, nplanes, nz, t, s);
+    }
+    return (int)hipErrorInvalidValue;
+}
+template <int NZ>
+static int line_z_t(int mode, float2* base, int nplanes, int nx, int ny, double scale, const DevTables& t,
+                    hipStream_t s) {
+    using C = ZColCfg<NZ>;
+    if (ny % C::CW) return (int)hipErrorInvalidValue;
+    const dim3 grid(ny / C::CW, nplanes);
+    if (mode == PLANE_FLOW) ISF_LAUNCH((k_line_z<NZ, PLANE_FLOW>), grid, dim3(C::NT), s, base, nx, ny, scale, t);
+    if (mode == PLANE_FLOW_ADJ) ISF_LAUNCH((k_line_z<NZ, PLANE_FLOW_ADJ>), grid, dim3(C::NT), s, base, nx, ny, scale, t);
+    ISF_LAUNCH((k_line_z<NZ, PLANE_POISSON>), grid, dim3(C::NT), s, base, nx, ny, scale, t);
+}
+// y pass, z pass with the multiplier, inverse y pass
+static int plane_split(int mode, int ny, int nz, float2* base, int nplanes, int nx, double scale, const DevTables& t,
+                       hipStream_t s) {
+    int e = line_y<-1>(ny, base, nplanes, nz, t, s);
+    if (e) return e;
+    switch (nz) {
+        case 16: e = line_z_t<16>(mode, base, nplanes, nx, ny, scale, t, s); break;
+        case 32: e = line_z_t<32>(mode, base, nplanes, nx, ny, scale, t, s); break;
+        case 64: e = line_z_t<64>(mode, base, nplanes, nx, ny, scale, t, s); break;
+        case 128: e = line_z_t<128>(mode, base, nplanes, nx, ny, scale, t, s); break;
+        case 256: e = line_z_t<256>(mode, base, nplanes, nx, ny, scale, t, s); break;
+        case 512: e = line_z_t<512>(mode, base, nplanes, nx, ny, scale, t, s); break;
+        default: return (int)hipErrorInvalidValue;
+    }
+    if (e) return e;
+    return line_y<1>(ny, base, nplanes, nz, t, s);
+}
+
 static int plane_dispatch(int mode, int ny, int nz, float2* base, int nplanes, int nx, double scale,
                           const DevTables& t, hipStream_t s, Batch b) {
     if (!plane_supported(ny, nz)) return (int)hipErrorInvalidValue;
+    if (!plane_in_lds(ny, nz)) {
+        if (b.inst) return (int)hipErrorInvalidValue;  // batches hold LDS-sized planes only
+        return plane_split(mode, ny, nz, base, nplanes, nx, scale, t, s);
+    }
     switch (ny) {
         case 16: { ISF_PLANE_Z(16) }
         case 32: { ISF_PLANE_Z(32) }

@@ -39,6 +39,9 @@ def test_batch_create_rejects_bad_members_without_gpu():
     arr[0].hbar, arr[0].dt = 0.1, 0.02
     rc = lib.isf_batch_create(C.byref(h), 2, arr)
     assert rc == -1 and b"member 0" in lib.isf_last_error()
+    arr[0].res = (C.c_int * 3)(64, 256, 128)  # supported by isf_create, too large a plane for a batch
+    rc = lib.isf_batch_create(C.byref(h), 2, arr)
+    assert rc == -1 and b"up to 128" in lib.isf_last_error()
     assert lib.isf_batch_step(None, 1) == -4
     assert lib.isf_batch_destroy(None) == -4
 

@@ -268,3 +268,53 @@ def test_full_steps_c2_shape(isf_mod):
     assert rep["p99"] < 1e-4
     nrm = g.isf.step_norm2()
     assert abs(nrm - np.prod(cfg.dims)) < 1e-5 * np.prod(cfg.dims)
+
+
+# planes larger than one workgroup's LDS (y or z above 128): split y / z passes
+LARGE_SHAPES = [((32, 256, 128), (2, 16, 8)), ((16, 512, 64), (1, 32, 4)), ((64, 128, 256), (4, 8, 16)),
+                ((16, 64, 512), (1, 4, 32))]
+
+
+@pytest.mark.parametrize("n,L", LARGE_SHAPES)
+def test_large_plane_flow_and_poisson(isf_mod, n, L):
+    print(f"[large plane] n={n}", flush=True)
+    isf = isf_mod.ISF(L, n, 0.1, 1 / 48)
+    psi = rnd((n[2], n[1], n[0], 4), 3)
+    fac, mask = O64.tables(n, L, float(np.float32(0.1)), float(np.float32(1 / 48)))
+    d = isf.psi_array(psi)
+    isf.schroedinger_flow_(d)
+    err = rel_l2(d.to_host(), O64.schroedinger_flow(psi.astype(np.float64), mask))
+    f = rnd((n[2], n[1], n[0]), 4)
+    q = isf.poisson_solve(isf_mod.DeviceArray.from_host(f)).to_host()
+    errq = rel_l2(q, O64.poisson_solve(f.astype(np.float64), fac)[..., 0])
+    dd = [L[a] / n[a] for a in range(3)]
+    lap = rel_l2(laplacian(q.astype(np.float64), dd), f - f.mean())
+    print(f"  flow rel L2 {err:.3e}; poisson rel L2 {errq:.3e}, Laplacian residual {lap:.3e}")
+    assert err < 2e-6
+    assert errq < 2e-6
+    assert lap < 1e-4
+    # adjoint of the flow through the split passes: <A x, y> = <x, A^T y>
+    y = rnd((n[2], n[1], n[0], 4), 5)
+    b = isf.psi_array(y)
+    isf.vjp_schroedinger_flow_(b)
+    ax, aty = d.to_host().astype(np.float64), b.to_host().astype(np.float64)
+    lhs = float(np.sum(ax * y))
+    rhs = float(np.sum(psi.astype(np.float64) * aty))
+    scale = float(np.linalg.norm(ax) * np.linalg.norm(y))
+    print(f"  <Ax, y> = {lhs:.8e}, <x, A^T y> = {rhs:.8e} (scale {scale:.3e})")
+    assert abs(lhs - rhs) < 1e-6 * scale
+
+
+def test_full_steps_large_plane(isf_mod):
+    """A jet on a grid with 256 x 128 planes (split plane passes) against the FP64 oracle."""
+    cfg = JetConfig(vol_size=(2, 16, 8), dims=(32, 256, 128), nozzle_cen=(0.3, 8 - 0.034, 4 + 0.066),
+                    n_spawn=200, n_slots=20_000, n_init=10_000, constraint_iters=4)
+    g = GpuRun(cfg)
+    g.step(3)
+    o = oracle_run(cfg, np.float64)
+    o.step(3)
+    err = rel_l2(g.host_psi(), o.psi)
+    rep = particle_report(g.host_parts(), o.parts, cfg.vol_size)
+    print(f"  32x256x128 psi rel L2 vs FP64 = {err:.3e}; particles {rep}")
+    assert err < 1e-4
+    assert rep["status_mismatch"] <= 2 and rep["p99"] < 1e-4
