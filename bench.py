@@ -309,6 +309,90 @@ def measure_traffic(N):
     return out, note
 
 
+def bench_large(steps, warmup):
+    """Large single jet: 512x256x256 (box 16x8x8, d = 1/32), 4M particle slots filled in the
+    nozzle, 1000 spawned per step; forward steps, HIP events on the handle's stream."""
+    w = dict(WORKLOAD, vol_size=(16, 8, 8), dims=(512, 256, 256), nozzle_cen=(0.3, 4 - 0.034, 4 + 0.066),
+             n_slots=4_000_000, n_init=4_000_000)
+    isf, psi, parts = setup_gpu(w)
+    it = 0
+    for _ in range(warmup):
+        it += 1
+        isf.step_(psi, parts, it)
+    isf.synchronize()
+    isf.timer_start()
+    for _ in range(steps):
+        it += 1
+        isf.step_(psi, parts, it)
+    ms = isf.timer_stop() / steps
+    isf.synchronize()
+    N = 512 * 256 * 256
+    norm = isf.step_norm2() / N
+    balg = b_alg_step(N, w["n_slots"])
+    psi.free()
+    parts.free()
+    isf.close()
+    print(f"[bench] 512x256x256 / 4M particles: {ms:.4f} ms/step", file=sys.stderr, flush=True)
+    return {"workload": "single jet 512x256x256 (box 16x8x8), 4M particles seeded in the nozzle, 1000 spawned per step",
+            "steps_s": round(1000.0 / ms, 2), "ms_per_step": round(ms, 4), "B_alg_per_step": balg,
+            "achieved_GB_s": round(balg / (ms * 1e-3) / 1e9, 1), "frac": round(balg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "norm2_over_N": round(norm, 9), "steps": steps, "warmup": warmup}
+
+
+def bench_adjoint_batch(count=8, steps=32):
+    """A batch of `count` differentiable rollouts at 128^3 (box 4x4x4), each with its own
+    seeded nozzle velocity and target density, 32 steps with checkpoints every 4, 100k
+    particles, run concurrently (one handle and one host thread per rollout)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from isf_amd import ISF
+    from isf_amd.rollout import Rollout, gaussian_target
+    dims, box = (128, 128, 128), (4, 4, 4)
+    hb = np.float32(0.1)
+    rng = np.random.default_rng(20261203)
+    jobs = []
+    for i in range(count):
+        ct = rng.uniform(np.cos(np.radians(20)), 1.0)
+        ph = rng.uniform(0, 2 * np.pi)
+        sp = rng.uniform(0.5, 1.5)
+        st = np.sqrt(1 - ct * ct)
+        v = np.array([sp * ct, sp * st * np.cos(ph), sp * st * np.sin(ph)], np.float32)
+        isf = ISF(box, dims, hb, 1.0 / 48.0)
+        isf.jet_configure(v / hb, np.float32(np.sum(v ** np.float32(2)) / (np.float32(2) * hb)),
+                          np.array([0.3, 2 - 0.034, 2 + 0.066], np.float32), 0.5, 0.5, 1000, 1000 + i, 0)
+        r = Rollout(isf, steps, checkpoint_every=4, constraint_iters=10, phase_res=(16, 16, 16),
+                    splat_res=(16, 16, 16), splat_sigma=0.25, splat_cutoff=4.0, nslots=100_000, n_init=100_000,
+                    fill_seed=2000 + i)
+        r.set_target(gaussian_target(box, (16, 16, 16), 3000 + i, centre_lo=(0.3, 1.5, 1.5), centre_hi=(2.0, 2.5, 2.5)))
+        th = (0.2 * rng.standard_normal((16, 16, 16))).astype(np.float32)
+        jobs.append((isf, r, v, th))
+
+    def fwd(j):
+        return j[1].forward(j[2], theta=j[3])
+
+    def fwd_bwd(j):
+        j[1].forward(j[2], theta=j[3])
+        return j[1].backward(want_psi=False, want_parts=False)
+
+    with ThreadPoolExecutor(count) as ex:
+        list(ex.map(fwd_bwd, jobs))  # warm-up
+        t0 = time.perf_counter()
+        losses = list(ex.map(fwd, jobs))
+        t1 = time.perf_counter()
+        list(ex.map(fwd_bwd, jobs))
+        t2 = time.perf_counter()
+    for isf, r, _, _ in jobs:
+        r.close()
+        isf.close()
+    print(f"[bench] {count} rollouts 128^3 x {steps} steps: forward {t1 - t0:.3f} s, forward+adjoint {t2 - t1:.3f} s",
+          file=sys.stderr, flush=True)
+    return {"workload": f"{count} concurrent rollouts, 128^3 (box 4x4x4), {steps} steps, checkpoint every 4, "
+                        "100k particles each, distinct seeded nozzle velocities and targets",
+            "fwd_steps_s": round(count * steps / (t1 - t0), 2), "fwd_adjoint_steps_s": round(count * steps / (t2 - t1), 2),
+            "s_forward": round(t1 - t0, 4), "s_forward_adjoint": round(t2 - t1, 4),
+            "losses": [round(float(x), 9) for x in losses],
+            "timing": "wall clock over all rollouts, set-up and constraint iterations included"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -321,6 +405,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint rollout leg")
     ap.add_argument("--adjoint-reps", type=int, default=3)
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--no-large", action="store_true", help="skip the 512x256x256 and 128^3 adjoint-batch legs")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.traffic_probe:
@@ -408,6 +493,11 @@ def main():
     if not args.no_adjoint:
         print("[bench] forward+adjoint rollout leg", file=sys.stderr, flush=True)
         out["fwd_adjoint"] = bench_fwd_adjoint(args.adjoint_reps)
+    if not args.no_large:
+        print("[bench] large jet leg", file=sys.stderr, flush=True)
+        out["large"] = bench_large(20, 3)
+        print("[bench] adjoint batch leg", file=sys.stderr, flush=True)
+        out["adjoint_batch"] = bench_adjoint_batch()
     if not args.no_cpu:
         print("[bench] CPU baseline leg", file=sys.stderr, flush=True)
         cb = cpu_baseline(w)
