@@ -128,6 +128,33 @@ int isf_norm2(isf_t h, const float* psi, double* out);
 /* The same sum accumulated by the last isf_step (read after isf_synchronize). */
 int isf_step_norm2(isf_t h, double* out);
 
+/* ---- MatlabPort particle store (src/MatlabPort.jl:178-230) ----------------------------- */
+/* Particles{x, y, z, length, active}: Float32 structure-of-arrays positions with
+ * `capacity` slots, the number of slots filled so far (length) and the active slot list.
+ * Device-resident; every call is ordered on the handle's stream. */
+typedef struct isf_particles* isf_particles_t;
+int isf_particles_create(isf_particles_t* out, long long capacity);   /* Particles(zeros(...), 0, Int[]), simple.jl:74-78 */
+int isf_particles_destroy(isf_particles_t p);
+/* append!(p, x, y, z) (src/MatlabPort.jl:186-195): n device positions into slots length..;
+ * like the reference, refuses unless length + n < capacity */
+int isf_particles_append(isf_t h, isf_particles_t p, const float* x, const float* y, const float* z, long long n);
+/* the driver's births (examples/matlabport/simple.jl:113-118): n particles on the ring of
+ * radius 0.9 rad around the configured nozzle at x = c_x; angle from Philox4x32-10
+ * counter (j, step, 2, 0) and key seed; appended */
+int isf_particles_spawn_ring(isf_t h, isf_particles_t p, long long n, unsigned long long seed, unsigned step);
+/* removethedead!(p, vol_size) (simple.jl:63-70): keeps the active slots strictly inside
+ * the box, in order */
+int isf_particles_remove_dead(isf_t h, isf_particles_t p);
+/* StaggeredAdvect(p, isf, vx, vy, vz, dt) (src/MatlabPort.jl:201-222) over the active
+ * slots; vel NULL = the handle's velocity field */
+int isf_particles_advect(isf_t h, isf_particles_t p, const float* vel);
+/* Keep(p, ind) (src/MatlabPort.jl:226-230): positions become x[ind] (device, 0-based, n
+ * entries) in slots 0..n-1; length = n and all n are active */
+int isf_particles_keep(isf_t h, isf_particles_t p, const int* ind, long long n);
+/* length, active count (synchronises), device views of x, y, z and the active list */
+int isf_particles_state(isf_t h, isf_particles_t p, long long* length, long long* active_count, const float** x,
+                        const float** y, const float** z, const int** active);
+
 /* ---- batched uneven ensembles (no reference counterpart) ------------------------------- */
 /* One member of an ensemble: the arguments of isf_create and isf_jet_configure. */
 typedef struct {

@@ -140,3 +140,57 @@ def staggered_advect(px, py, pz, torus, vx, vy, vz, dt):  # :201-222
 def to_xyz(a):
     """Oracle layout (nz, ny, nx, ...) -> MatlabPort layout (nx, ny, nz, ...)."""
     return np.swapaxes(a, 0, 2)
+
+
+class Torus:
+    """The grid fields StaggeredVelocity reads (sizex.., dx.., resx..)."""
+
+    def __init__(self, n, L, d):
+        self.resx, self.resy, self.resz = n
+        self.sizex, self.sizey, self.sizez = L
+        self.dx, self.dy, self.dz = d
+
+
+class Particles:
+    """Particles, src/MatlabPort.jl:178-196: Float32 positions, length, active list."""
+
+    def __init__(self, capacity):
+        self.x = np.zeros(capacity, np.float32)  # zeros(Float32, max_particles), simple.jl:74-78
+        self.y = np.zeros(capacity, np.float32)
+        self.z = np.zeros(capacity, np.float32)
+        self.length = 0
+        self.active = []
+
+    def append(self, x, y, z):  # :186-195
+        n = len(x)
+        assert n + self.length < len(self.x)
+        r = np.arange(self.length, self.length + n)
+        self.x[r], self.y[r], self.z[r] = x, y, z
+        self.length += n
+        self.active.extend(int(i) for i in r)
+
+    def spawn_ring(self, n, cen, rad, seed, step):
+        """The driver's births (examples/matlabport/simple.jl:113-117) with u from
+        Philox4x32-10 counter (k, step, 2, 0) (the reference draws rand(Float32))."""
+        from .oracle import philox4x32_10
+        u = np.array([np.float32((philox4x32_10((k, step, 2, 0), (seed & 0xFFFFFFFF, seed >> 32))[0] >> 8))
+                      * np.float32(1.0 / 16777216.0) for k in range(n)], np.float32)
+        rt = u.astype(np.float64) * 2 * np.pi
+        self.append(np.full(n, cen[0]), cen[1] + 0.9 * rad * np.cos(rt), cen[2] + 0.9 * rad * np.sin(rt))
+
+    def remove_dead(self, vol_size):  # removethedead!, simple.jl:63-70 (isinside: strict, :9-14)
+        L = vol_size
+        self.active = [i for i in self.active
+                       if 0 < self.x[i] < L[0] and 0 < self.y[i] < L[1] and 0 < self.z[i] < L[2]]
+
+    def advect(self, torus, vx, vy, vz, dt):  # StaggeredAdvect, src/MatlabPort.jl:201-222
+        a = np.asarray(self.active, np.int64)
+        if a.size == 0:
+            return
+        px, py, pz = (self.x[a].astype(np.float64), self.y[a].astype(np.float64), self.z[a].astype(np.float64))
+        nx_, ny_, nz_ = staggered_advect(px, py, pz, torus, vx, vy, vz, dt)
+        self.x[a], self.y[a], self.z[a] = nx_, ny_, nz_
+
+    def keep(self, ind):  # Keep, :226-230
+        ind = np.asarray(ind, np.int64)
+        self.x, self.y, self.z = self.x[ind], self.y[ind], self.z[ind]

@@ -722,30 +722,6 @@ __global__ void __launch_bounds__(256) k_reduce_sum(const double* __restrict__ p
 
 // ======================================================================== particles
 
-// staggered_velocity, src/parallel.jl:219-246.  Weights are not divided by d (:238).
-// An index that mod() rounds onto the upper face wraps to the first cell.
-__device__ __forceinline__ void stag_vel(const float* __restrict__ vel, const float p_in[3], const GridInfo& g, float u[3]) {
-    int i[3], ip[3];
-    float w[3];
-    stag_index(p_in, g, i, ip, w);
-    // the 7 cells as 4 rows: (j,k) and (j+1,k), (j,k+1) at x = i and i+1, (j+1,k+1) at i
-    const unsigned sy = (unsigned)g.nx, sz = (unsigned)g.nx * (unsigned)g.ny;
-    const unsigned rjk = sy * i[1] + sz * i[2], rjpk = sy * ip[1] + sz * i[2];
-    const unsigned rjkp = sy * i[1] + sz * ip[2], rjpkp = sy * ip[1] + sz * ip[2];
-    const F3 a0 = ld3(vel + 3u * (i[0] + rjk)), a1 = ld3(vel + 3u * (ip[0] + rjk));
-    const F3 b0 = ld3(vel + 3u * (i[0] + rjpk)), b1 = ld3(vel + 3u * (ip[0] + rjpk));
-    const F3 c0 = ld3(vel + 3u * (i[0] + rjkp)), c1 = ld3(vel + 3u * (ip[0] + rjkp));
-    const float d0x = vel[3u * (i[0] + rjpkp)];
-    const float v0[3] = {a0.x, a0.y, a0.z};   // velocity[i, j, k]
-    const float vn[3] = {d0x, c1.y, b1.z};    // (v[i,j+1,k+1].x, v[i+1,j,k+1].y, v[i+1,j+1,k].z)
-    const float pp[3] = {b0.x, a1.y, a1.z};   // (pyp.x, pxp.y, pxp.z)
-    const float pp2[3] = {c0.x, c0.y, b0.z};  // (pzp.x, pzp.y, pyp.z)
-    const float w1[3] = {w[2], w[2], w[1]}, w2[3] = {w[1], w[0], w[0]};
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-        u[a] = (1.0f - w1[a]) * ((1.0f - w2[a]) * v0[a] + w2[a] * pp[a]) + w1[a] * ((1.0f - w2[a]) * pp2[a] + w2[a] * vn[a]);
-}
-
 // staggered_advect! (src/parallel.jl:190-216,248-261): RK4 for slots strictly inside
 // the box, the dead marker (-1,-1,-1) otherwise.  Thread k processes slot order[k]
 // (order == nullptr: slot k); every slot is independent, so the order only changes

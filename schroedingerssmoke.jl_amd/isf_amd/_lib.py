@@ -71,6 +71,16 @@ SIGNATURES = {
     "isf_step_stage_count": ([], C.c_int),
     "isf_step_stage_name": ([C.c_int], C.c_char_p),
     "isf_step_profile": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.POINTER(C.c_float)], C.c_int),
+    "isf_particles_create": ([C.POINTER(C.c_void_p), C.c_longlong], C.c_int),
+    "isf_particles_destroy": ([C.c_void_p], C.c_int),
+    "isf_particles_append": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong], C.c_int),
+    "isf_particles_spawn_ring": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_ulonglong, C.c_uint], C.c_int),
+    "isf_particles_remove_dead": ([C.c_void_p, C.c_void_p], C.c_int),
+    "isf_particles_advect": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_particles_keep": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong], C.c_int),
+    "isf_particles_state": ([C.c_void_p, C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                             C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                             C.POINTER(C.c_void_p)], C.c_int),
     "isf_batch_create": ([C.POINTER(C.c_void_p), C.c_int, C.POINTER(MemberDesc)], C.c_int),
     "isf_batch_destroy": ([C.c_void_p], C.c_int),
     "isf_batch_member": ([C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),

@@ -1,0 +1,68 @@
+"""The MatlabPort particle store over libisf (isf_particles_*): Particles{x, y, z,
+length, active} (src/MatlabPort.jl:178-230) held on the device, with append!, the
+driver's ring births, removethedead! (examples/matlabport/simple.jl:63-70),
+StaggeredAdvect over the active slots and Keep."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, load
+from .api import DeviceArray, _ptr
+
+
+class Particles:
+    def __init__(self, isf, capacity):
+        self.lib = load()
+        self.isf = isf
+        self.capacity = int(capacity)
+        h = C.c_void_p()
+        check(self.lib.isf_particles_create(C.byref(h), self.capacity), "isf_particles_create")
+        self.h = h.value
+
+    def append_(self, x, y, z):
+        """append!(p, x, y, z) with host or device arrays."""
+        arrs = [a if isinstance(a, DeviceArray) else DeviceArray.from_host(np.asarray(a, np.float32)) for a in (x, y, z)]
+        n = arrs[0].shape[0]
+        check(self.lib.isf_particles_append(self.isf.h, self.h, arrs[0].ptr, arrs[1].ptr, arrs[2].ptr, n),
+              "isf_particles_append")
+        self.isf.synchronize()  # the host arrays are freed on return
+
+    def spawn_ring_(self, n, seed, step):
+        check(self.lib.isf_particles_spawn_ring(self.isf.h, self.h, int(n), int(seed), int(step)),
+              "isf_particles_spawn_ring")
+
+    def remove_dead_(self):
+        check(self.lib.isf_particles_remove_dead(self.isf.h, self.h), "isf_particles_remove_dead")
+
+    def advect_(self, velocity=None):
+        check(self.lib.isf_particles_advect(self.isf.h, self.h, _ptr(velocity)), "isf_particles_advect")
+
+    def keep_(self, ind):
+        d = DeviceArray.from_host(np.asarray(ind, np.int32))
+        check(self.lib.isf_particles_keep(self.isf.h, self.h, d.ptr, d.shape[0]), "isf_particles_keep")
+
+    def state(self):
+        """(length, active indices, x, y, z) as host arrays (x, y, z over all capacity slots)."""
+        ln, cnt = C.c_longlong(), C.c_longlong()
+        px, py, pz, pa = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(self.lib.isf_particles_state(self.isf.h, self.h, C.byref(ln), C.byref(cnt), C.byref(px), C.byref(py),
+                                           C.byref(pz), C.byref(pa)), "isf_particles_state")
+        cap = self.capacity
+        x = DeviceArray.borrowed(px.value, (cap,), np.float32).to_host()
+        y = DeviceArray.borrowed(py.value, (cap,), np.float32).to_host()
+        z = DeviceArray.borrowed(pz.value, (cap,), np.float32).to_host()
+        act = DeviceArray.borrowed(pa.value, (max(cnt.value, 1),), np.int32).to_host()[:cnt.value]
+        return ln.value, act, x, y, z
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.isf_particles_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
