@@ -155,6 +155,19 @@ int isf_particles_keep(isf_t h, isf_particles_t p, const int* ind, long long n);
 int isf_particles_state(isf_t h, isf_particles_t p, long long* length, long long* active_count, const float** x,
                         const float** y, const float** z, const int** active);
 
+/* ---- asynchronous particle export (examples/jet.jl:83, Array(particles) per frame) ------ */
+/* Double-buffered copies of the particle positions (nslots x 3 floats) into pinned host
+ * memory.  push stages the positions on the handle's stream after all queued work and
+ * copies them to the host on a separate stream, so later steps do not wait for the PCIe
+ * transfer.  previous returns push k-1 after push k (waiting for its copy if needed);
+ * latest waits for the last push.  A returned buffer stays valid until the next push. */
+typedef struct isf_export* isf_export_t;
+int isf_export_create(isf_export_t* out, isf_t h, long long nslots);
+int isf_export_destroy(isf_export_t e);
+int isf_export_push(isf_export_t e, const float* parts);
+int isf_export_previous(isf_export_t e, const float** host, long long* frame);
+int isf_export_latest(isf_export_t e, const float** host, long long* frame);
+
 /* ---- batched uneven ensembles (no reference counterpart) ------------------------------- */
 /* One member of an ensemble: the arguments of isf_create and isf_jet_configure. */
 typedef struct {

@@ -81,6 +81,11 @@ SIGNATURES = {
     "isf_particles_state": ([C.c_void_p, C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                              C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                              C.POINTER(C.c_void_p)], C.c_int),
+    "isf_export_create": ([C.POINTER(C.c_void_p), C.c_void_p, C.c_longlong], C.c_int),
+    "isf_export_destroy": ([C.c_void_p], C.c_int),
+    "isf_export_push": ([C.c_void_p, C.c_void_p], C.c_int),
+    "isf_export_previous": ([C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_longlong)], C.c_int),
+    "isf_export_latest": ([C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_longlong)], C.c_int),
     "isf_batch_create": ([C.POINTER(C.c_void_p), C.c_int, C.POINTER(MemberDesc)], C.c_int),
     "isf_batch_destroy": ([C.c_void_p], C.c_int),
     "isf_batch_member": ([C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),

@@ -66,3 +66,43 @@ class Particles:
             self.close()
         except Exception:
             pass
+
+
+class ParticleExport:
+    """isf_export_*: double-buffered asynchronous copies of a particle buffer (nslots x 3
+    floats on the device) into pinned host memory, the per-frame `Array(particles)` of
+    examples/jet.jl:83 without stalling the simulation stream."""
+
+    def __init__(self, isf, nslots):
+        self.lib = load()
+        self.nslots = int(nslots)
+        h = C.c_void_p()
+        check(self.lib.isf_export_create(C.byref(h), isf.h, self.nslots), "isf_export_create")
+        self.h = h.value
+
+    def push(self, particles):
+        check(self.lib.isf_export_push(self.h, _ptr(particles)), "isf_export_push")
+
+    def _get(self, fn, name):
+        p, f = C.c_void_p(), C.c_longlong()
+        check(fn(self.h, C.byref(p), C.byref(f)), name)
+        arr = np.ctypeslib.as_array(C.cast(p.value, C.POINTER(C.c_float)), shape=(self.nslots, 3))
+        return f.value, arr.copy()
+
+    def previous(self):
+        """(frame index, positions) of the push before the last one."""
+        return self._get(self.lib.isf_export_previous, "isf_export_previous")
+
+    def latest(self):
+        return self._get(self.lib.isf_export_latest, "isf_export_latest")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.isf_export_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -123,3 +123,30 @@ def test_keep_compacts():
     length, active, x, y, z = state
     assert length == 4 and list(active) == [0, 1, 2, 3]
     assert np.allclose(x[:4], OP.x, atol=1e-4) and np.allclose(y[:4], OP.y, atol=1e-4)
+
+
+def test_async_particle_export_frames():
+    """Exported frames equal synchronous copies of the same (deterministic) run."""
+    from _harness import GpuRun
+    from isf_amd import ParticleExport
+    cfg = JetConfig(n_spawn=200, n_slots=50_000, n_init=30_000, constraint_iters=4, seed=11)
+    ref = GpuRun(cfg)
+    frames = []
+    for _ in range(6):
+        ref.step(1)
+        frames.append(ref.host_parts())
+    run = GpuRun(cfg)
+    ex = ParticleExport(run.isf, cfg.n_slots)
+    got = {}
+    for k in range(6):
+        run.step(1)  # no synchronisation between steps and exports
+        ex.push(run.parts)
+        if k >= 1:
+            f, pos = ex.previous()
+            got[f] = pos
+    f, pos = ex.latest()
+    got[f] = pos
+    assert sorted(got) == list(range(6))
+    for k in range(6):
+        assert np.array_equal(got[k], frames[k]), k
+    ex.close()
