@@ -309,6 +309,30 @@ def measure_traffic(N):
     return out, note
 
 
+def bench_export(isf, psi, parts, it, steps, every):
+    """Step rate with the particle positions exported to pinned host memory every `every`
+    steps (isf_export_push; the host takes the previous frame after each push)."""
+    from isf_amd import ParticleExport
+    ex = ParticleExport(isf, parts.shape[0])
+    isf.synchronize()
+    t0 = time.perf_counter()
+    pushes = 0
+    for k in range(steps):
+        it += 1
+        isf.step_(psi, parts, it)
+        if k % every == 0:
+            ex.push(parts)
+            pushes += 1
+            if pushes >= 2:
+                ex.previous()
+    ex.latest()
+    isf.synchronize()
+    dt = time.perf_counter() - t0
+    ex.close()
+    return it, {"every": every, "steps_s": round(steps / dt, 2), "pushes": pushes,
+                "MB_per_push": round(parts.shape[0] * 12 / 1e6, 2)}
+
+
 def bench_large(steps, warmup):
     """Large single jet: 512x256x256 (box 16x8x8, d = 1/32), 4M particle slots filled in the
     nozzle, 1000 spawned per step; forward steps, HIP events on the handle's stream."""
@@ -406,6 +430,7 @@ def main():
     ap.add_argument("--adjoint-reps", type=int, default=3)
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--no-large", action="store_true", help="skip the 512x256x256 and 128^3 adjoint-batch legs")
+    ap.add_argument("--no-export", action="store_true", help="skip the particle export timing")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.traffic_probe:
@@ -444,6 +469,13 @@ def main():
     norm = isf.step_norm2()
     print(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step (events), wall {wall:.3f} s, "
           f"norm2/N = {norm / N:.9f}", file=sys.stderr, flush=True)
+
+    exports = []
+    if not args.no_export:
+        for every in (1, 4):
+            it, rec = bench_export(isf, psi, parts, it, 60, every)
+            exports.append(rec)
+        print(f"[bench] export: {exports}", file=sys.stderr, flush=True)
 
     # per-stage HIP-event timing of the same step on the same stream
     acc = {}
@@ -486,6 +518,7 @@ def main():
         "step_roofline": {"B_alg_per_step": balg, "achieved_GB_s": round(balg * steps_s / 1e9, 1),
                           "frac": round(balg * steps_s / 1e9 / HBM_PEAK_GBS, 4)},
         "stages": stages,
+        "export": exports or None,
     }
     if not args.no_batch:
         print("[bench] batched ensemble leg", file=sys.stderr, flush=True)

@@ -75,6 +75,9 @@ int isf_normalize(isf_t h, float* psi);
  * the driver's in-place StaggeredSharp v .*= 1 ./ d (test/runtests.jl:61-62).
  * vel == NULL writes the handle's own velocity field. */
 int isf_velocity_one_form(isf_t h, const float* psi, float hbar, int sharp, float* vel);
+/* StaggeredSharp in place, velocity .* inv.(isf.d) (test/runtests.jl:61-62,
+ * src/MatlabPort.jl:99-104); vel NULL = the handle's velocity field. */
+int isf_staggered_sharp(isf_t h, float* vel);
 /* div!(isf), src/parallel.jl:131-152: f from a velocity field (vel == NULL: handle's). */
 int isf_div(isf_t h, const float* vel, float* f);
 /* poisson_solve(isf), src/parallel.jl:168-174: q = ifft(fac .* fft(f)).  q is returned

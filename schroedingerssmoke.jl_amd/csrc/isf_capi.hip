@@ -365,6 +365,12 @@ int isf_velocity_one_form(isf_t h, const float* psi, float hbar, int sharp, floa
     return ISF_OK;
 }
 
+int isf_staggered_sharp(isf_t h, float* vel) {
+    CHECK_HJ(h);
+    LAUNCH_CHECK(launch_sharp(h->g, vel ? vel : h->vel, h->stream), "StaggeredSharp");
+    return ISF_OK;
+}
+
 int isf_div(isf_t h, const float* vel, float* f) {
     CHECK_HJ(h);
     CHECK_PTR(f, "f");

@@ -91,6 +91,7 @@ int launch_velocity(const GridInfo& g, const float4* psi, float hbar, int sharp,
                     Batch b = Batch{});
 int launch_div_vel(const GridInfo& g, const float* vel, float* f, hipStream_t s);
 int launch_normalize(const GridInfo& g, float4* psi, hipStream_t s);
+int launch_sharp(const GridInfo& g, float* vel, hipStream_t s);
 int launch_gauge(const GridInfo& g, float4* psi, const float* q, hipStream_t s);
 int launch_isjet(const GridInfo& g, const JetParams& jet, unsigned char* out, hipStream_t s);
 int launch_restrict(const GridInfo& g, float4* psi, const unsigned char* isjet, const JetParams& jet, hipStream_t s);

@@ -665,6 +665,16 @@ __global__ void __launch_bounds__(256) k_div_vel(const float* __restrict__ vel, 
 
 // ======================================================================== pointwise
 
+// StaggeredSharp as the driver applies it in place, velocity .* inv.(isf.d)
+// (test/runtests.jl:61-62; src/MatlabPort.jl:99-104)
+__global__ void __launch_bounds__(256) k_sharp(float* __restrict__ vel, float s0, float s1, float s2, size_t N) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    vel[3 * c + 0] *= s0;
+    vel[3 * c + 1] *= s1;
+    vel[3 * c + 2] *= s2;
+}
+
 __global__ void __launch_bounds__(256) k_normalize(float4* __restrict__ psi, size_t N) {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
@@ -1068,6 +1078,9 @@ int launch_velocity(const GridInfo& g, const float4* psi, float hbar, int sharp,
 }
 int launch_div_vel(const GridInfo& g, const float* vel, float* f, hipStream_t s) {
     ISF_LAUNCH(k_div_vel, grid1d(ncells(g), 256), dim3(256), s, vel, f, g);
+}
+int launch_sharp(const GridInfo& g, float* vel, hipStream_t s) {
+    ISF_LAUNCH(k_sharp, grid1d(ncells(g), 256), dim3(256), s, vel, 1.0f / g.dx, 1.0f / g.dy, 1.0f / g.dz, ncells(g));
 }
 int launch_normalize(const GridInfo& g, float4* psi, hipStream_t s) {
     ISF_LAUNCH(k_normalize, grid1d(ncells(g), 256), dim3(256), s, psi, ncells(g));

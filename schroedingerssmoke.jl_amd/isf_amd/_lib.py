@@ -51,6 +51,7 @@ SIGNATURES = {
     "isf_schroedinger_flow": ([C.c_void_p, C.c_void_p], C.c_int),
     "isf_normalize": ([C.c_void_p, C.c_void_p], C.c_int),
     "isf_velocity_one_form": ([C.c_void_p, C.c_void_p, C.c_float, C.c_int, C.c_void_p], C.c_int),
+    "isf_staggered_sharp": ([C.c_void_p, C.c_void_p], C.c_int),
     "isf_div": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "isf_poisson_solve": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "isf_gauge_transform": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),

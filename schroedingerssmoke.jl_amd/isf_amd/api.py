@@ -165,6 +165,11 @@ class ISF:
                                              _ptr(out)), "velocity_one_form!")
         return out if out is not None else self.velocity
 
+    def staggered_sharp_(self, velocity=None):
+        """StaggeredSharp in place (test/runtests.jl:61-62); default: isf.velocity."""
+        check(self.lib.isf_staggered_sharp(self.h, _ptr(velocity)), "isf_staggered_sharp")
+        return velocity
+
     def div_(self, f, velocity=None):
         check(self.lib.isf_div(self.h, _ptr(velocity), _ptr(f)), "div!")
         return f

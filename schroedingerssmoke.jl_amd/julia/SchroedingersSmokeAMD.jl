@@ -12,7 +12,7 @@ module SchroedingersSmokeAMD
 
 export ISF, ISFArray, normalize_psi!, pressure_project!, velocity_one_form!, schroedinger_flow!
 export staggered_advect!, add_particles!, restrict_velocity!, isjet, synchronize
-export ISFBatch, ISFRollout, rollout_loss, rollout_gradient
+export ISFBatch, ISFRollout, rollout_loss, rollout_gradient, normalize_psi
 
 const libisf = joinpath(@__DIR__, "..", "lib", "libisf.so")
 
@@ -80,17 +80,36 @@ mutable struct ISF{A,I,F}
         d = Float32.(Tuple(collect(physical_size) ./ collect(dims)))
         isf = new{A,I,F}(h[], Tuple(dims), Tuple(physical_size), d, Float32(hbar), Float32(dt))
         finalizer(x -> ccall((:isf_destroy, libisf), Cint, (Ptr{Cvoid},), x.h), isf)
+        CURRENT[] = isf  # the grid the reference's psi broadcasts refer to
         isf
     end
 end
 
+const CURRENT = Ref{Any}(nothing)
+
+"`isf.velocity` (src/parallel.jl:14): the handle's NTuple{3,Float32} field (borrowed)."
+struct ISFVelocity
+    isf::Any
+end
+"`isf.positions` (src/parallel.jl:75-77): cell positions, only meaningful inside `isjet.(...)`."
+struct ISFPositions
+    isf::Any
+end
+function devptr(v::ISFVelocity)
+    p = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:isf_velocity_ptr, libisf), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), v.isf.h, p), "isf_velocity_ptr")
+    p[]
+end
+function Base.Array(v::ISFVelocity)
+    h = Array{NTuple{3,Float32},3}(undef, v.isf.grid_res)
+    check(ccall((:isf_memcpy_d2h, libisf), Cint, (Ptr{NTuple{3,Float32}}, Ptr{Cvoid}, Csize_t), h, devptr(v),
+                sizeof(h)), "isf_memcpy_d2h")
+    h
+end
+
 function Base.getproperty(isf::ISF, s::Symbol)
-    if s === :velocity
-        p = Ref{Ptr{Cvoid}}(C_NULL)
-        check(ccall((:isf_velocity_ptr, libisf), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), getfield(isf, :h), p),
-              "isf_velocity_ptr")
-        return p[]  # device pointer of the NTuple{3,Float32} field (borrowed)
-    end
+    s === :velocity && return ISFVelocity(isf)
+    s === :positions && return ISFPositions(isf)
     getfield(isf, s)
 end
 
@@ -156,6 +175,50 @@ jet_configure!(isf::ISF, kvec, omega, nozzle_cen, nozzle_len, nozzle_rad, n_spaw
 step!(isf::ISF, psi, particles::ISFArray, iter::Integer) =
     check(ccall((:isf_step, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Clonglong, Cint),
                 isf.h, devptr(psi), particles.ptr, length(particles), Cint(iter)), "isf_step")
+
+# ---------------------------------------------------------------- the reference's broadcast call forms
+# The reference drivers write (test/runtests.jl:21,25,49,61-62)
+#     psi .= normalize_psi.(psi)
+#     jet = isjet.(isf.positions, nozzle_cen, nozzle_len, nozzle_rad)
+#     broadcast!((x, y) -> x .* y, isf.velocity, isf.velocity, (inv.(isf.d),))
+# Each lowers to Base.broadcasted / materialize(!) / broadcast!, which are specialised
+# here so that the whole-array form runs as one libisf call: isf_normalize, isf_isjet,
+# isf_staggered_sharp.  The psi broadcast uses the grid of the most recently created ISF.
+
+"`normalize_psi` of the reference (src/parallel.jl:162-165); here only its broadcast form."
+function normalize_psi end
+struct NormalizeBC{A}
+    psi::A
+end
+Base.Broadcast.broadcasted(::typeof(normalize_psi), psi::ISFArray) = NormalizeBC(psi)
+function Base.Broadcast.materialize!(dest::ISFArray, bc::NormalizeBC)
+    if dest !== bc.psi
+        check(ccall((:isf_memcpy_d2d, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t), dest.ptr, bc.psi.ptr,
+                    prod(size(dest)) * sizeof(eltype(dest))), "isf_memcpy_d2d")
+    end
+    normalize_psi!(CURRENT[], dest)
+    dest
+end
+
+struct IsjetBC
+    isf::Any
+    args::Tuple
+end
+Base.Broadcast.broadcasted(::typeof(isjet), p::ISFPositions, cen, len, rad) = IsjetBC(p.isf, (cen, len, rad))
+Base.Broadcast.materialize(bc::IsjetBC) = isjet(bc.isf, bc.args...)
+
+# the driver's in-place StaggeredSharp: velocity .* inv.(isf.d) per component
+function Base.broadcast!(f, dest::ISFVelocity, src::ISFVelocity, scale::Tuple{<:Tuple})
+    dest.isf === src.isf || error("broadcast! between the velocities of two ISF instances")
+    all(Float32.(scale[1]) .== inv.(dest.isf.d)) ||
+        error("broadcast! on isf.velocity: only the StaggeredSharp scaling inv.(isf.d) is supported")
+    check(ccall((:isf_staggered_sharp, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), dest.isf.h, C_NULL),
+          "isf_staggered_sharp")
+    dest
+end
+
+"GPUArrays.synchronize(array) of the drivers (test/runtests.jl:65)."
+synchronize(::ISFArray) = synchronize(CURRENT[])
 
 # ---------------------------------------------------------------- batched uneven ensembles
 "Mirror of `isf_member_desc` (include/isf.h)."

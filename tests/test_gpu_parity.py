@@ -109,6 +109,12 @@ def test_pointwise_and_stencils(isf_mod, n, L):
         dv = wrap_diff(v / scale, ref / scale)
         # angle(z) is ill-conditioned where |z| is tiny (random psi); bound the bulk tightly
         assert np.percentile(dv, 99.9) < 2e-6 and np.max(dv) < 1e-3
+    # standalone StaggeredSharp: velocity_one_form(hbar) then .* inv.(d) (test/runtests.jl:59-62)
+    vs = isf.velocity_one_form_(dp, hbar=0.1, sharp=False).to_host()
+    vb = isf_mod.DeviceArray.from_host(vs)
+    isf.staggered_sharp_(vb)
+    inv_d = (np.float32(1) / np.asarray(d, np.float32)).astype(np.float32)
+    assert np.array_equal(vb.to_host(), (vs * inv_d).astype(np.float32))
     # div from a given velocity field
     vel = rnd((n[2], n[1], n[0], 3), 4)
     f = isf_mod.DeviceArray(isf.shape3, np.float32)
