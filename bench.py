@@ -273,6 +273,8 @@ def _counter_pass(counter, timeout_s=150):
         shutil.rmtree(out, ignore_errors=True)
 
 
+PARTICLE_STAGES = ("spawn", "particle_order", "advect")
+
 # stage name -> kernel name pattern (flow planes are k_plane<NY, NZ, 0>, Poisson planes <.., 1>)
 STAGE_KERNELS = {
     "flow_x_fwd": r"k_flow_x_fwd<", "flow_plane": r"k_plane<\d+, \d+, 0>", "flow_x_inv_normalize": r"k_flow_x_inv<",
@@ -470,13 +472,6 @@ def main():
     print(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step (events), wall {wall:.3f} s, "
           f"norm2/N = {norm / N:.9f}", file=sys.stderr, flush=True)
 
-    exports = []
-    if not args.no_export:
-        for every in (1, 4):
-            it, rec = bench_export(isf, psi, parts, it, 60, every)
-            exports.append(rec)
-        print(f"[bench] export: {exports}", file=sys.stderr, flush=True)
-
     # per-stage HIP-event timing of the same step on the same stream
     acc = {}
     for _ in range(args.profile_steps):
@@ -485,7 +480,10 @@ def main():
             acc.setdefault(k, []).append(v)
     stage_ms = {k: float(np.mean(v)) for k, v in acc.items()}
     sb = stage_bytes(nx, ny, nz, w["n_slots"], w["n_spawn"])
-    dom = max(stage_ms, key=stage_ms.get) if stage_ms else "flow_plane"
+    # the dominant kernel of the critical path: the particle stages (spawn, order, advect)
+    # run on a second stream, overlapped with the grid stages of the next step
+    grid = {k: v for k, v in stage_ms.items() if k not in PARTICLE_STAGES}
+    dom = max(grid, key=grid.get) if grid else "flow_plane"
     achieved = sb[dom] / (stage_ms[dom] * 1e-3) / 1e9 if stage_ms else 0.0
     stages = {k: {"ms": round(stage_ms[k], 5), "GB": round(sb[k] / 1e9, 4),
                   "GB_s": round(sb[k] / max(stage_ms[k] * 1e-3, 1e-9) / 1e9, 1)} for k in stage_ms}
@@ -494,6 +492,12 @@ def main():
             if k in traffic:
                 stages[k]["traffic_GB"] = round(traffic[k] / 1e9, 4)
     balg = b_alg_step(N, w["n_slots"])
+    exports = []
+    if not args.no_export:
+        for every in (1, 4):
+            it, rec = bench_export(isf, psi, parts, it, 60, every)
+            exports.append(rec)
+        print(f"[bench] export: {exports}", file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,
