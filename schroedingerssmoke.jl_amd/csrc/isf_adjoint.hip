@@ -348,9 +348,53 @@ __device__ __forceinline__ void stag_eval(const float* __restrict__ vel, const f
                w1[a] * ((1.0f - w2[a]) * e.pp2[a] + w2[a] * e.vn[a]);
 }
 
-// ubar at the evaluation point: position cotangent xbar (3) and scatter into velbar.
+// The 12 (cell, component) targets of one staggered evaluation are fixed by its base cell
+// e.i (the others are its +1 neighbours, periodic).  Lanes of a wavefront that share the
+// base cell in a contiguous run of lanes sum their 12 contributions by a segmented
+// inclusive scan (offsets 1..32); the last lane of each run issues the 12 atomics.  Lanes
+// in any order give the right sum; cell-sorted slots give long runs and few atomics.
+// Every lane of the wavefront calls this (act = false: nothing to add).
+__device__ __forceinline__ void wave_scatter12(float* __restrict__ velbar, const StagEval& e, const GridInfo& g,
+                                               float val[12], bool act) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const unsigned key = act ? (unsigned)cidx(e.i[0], e.i[1], e.i[2], g) : 0xFFFFFFFFu;
+    const unsigned kprev = __shfl_up(key, 1u, 64);
+    int head = (lane == 0 || kprev != key) ? lane : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(head, (unsigned)d, 64);
+        if (lane >= d) head = max(head, o);
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const bool take = lane - d >= head;
+#pragma unroll
+        for (int j = 0; j < 12; j++) {
+            const float o = __shfl_up(val[j], (unsigned)d, 64);
+            if (take) val[j] += o;
+        }
+    }
+    const unsigned knext = __shfl_down(key, 1u, 64);
+    if (!act || (lane != 63 && knext == key)) return;
+    const int *i = e.i, *ip = e.ip;
+    const size_t c000 = cidx(i[0], i[1], i[2], g);
+    const size_t cv[3][3] = {// (pp, pp2, vn) cells per component
+                             {cidx(i[0], ip[1], i[2], g), cidx(i[0], i[1], ip[2], g), cidx(i[0], ip[1], ip[2], g)},
+                             {cidx(ip[0], i[1], i[2], g), cidx(i[0], i[1], ip[2], g), cidx(ip[0], i[1], ip[2], g)},
+                             {cidx(ip[0], i[1], i[2], g), cidx(i[0], ip[1], i[2], g), cidx(ip[0], ip[1], i[2], g)}};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        unsafeAtomicAdd(velbar + 3 * c000 + a, val[4 * a]);
+        unsafeAtomicAdd(velbar + 3 * cv[a][0] + a, val[4 * a + 1]);
+        unsafeAtomicAdd(velbar + 3 * cv[a][1] + a, val[4 * a + 2]);
+        unsafeAtomicAdd(velbar + 3 * cv[a][2] + a, val[4 * a + 3]);
+    }
+}
+
+// ubar at the evaluation point: position cotangent xbar (3) and scatter into velbar
+// (every lane of the wavefront calls this, act marks real work).
 __device__ __forceinline__ void stag_adj(const StagEval& e, const float ub[3], const GridInfo& g, float* __restrict__ velbar,
-                                         float xb[3]) {
+                                         float xb[3], bool act) {
     const float w1[3] = {e.w[2], e.w[2], e.w[1]}, w2[3] = {e.w[1], e.w[0], e.w[0]};
     float dw1[3], dw2[3];
 #pragma unroll
@@ -362,36 +406,40 @@ __device__ __forceinline__ void stag_adj(const StagEval& e, const float ub[3], c
     xb[0] = ub[1] * dw2[1] + ub[2] * dw2[2];
     xb[1] = ub[0] * dw2[0] + ub[2] * dw1[2];
     xb[2] = ub[0] * dw1[0] + ub[1] * dw1[1];
-    const int *i = e.i, *ip = e.ip;
-    const size_t c000 = cidx(i[0], i[1], i[2], g);
-    const size_t cv[3][3] = {// (pp, pp2, vn) cells per component
-                             {cidx(i[0], ip[1], i[2], g), cidx(i[0], i[1], ip[2], g), cidx(i[0], ip[1], ip[2], g)},
-                             {cidx(ip[0], i[1], i[2], g), cidx(i[0], i[1], ip[2], g), cidx(ip[0], i[1], ip[2], g)},
-                             {cidx(ip[0], i[1], i[2], g), cidx(i[0], ip[1], i[2], g), cidx(ip[0], ip[1], i[2], g)}};
+    float val[12];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const float u = ub[a];
-        unsafeAtomicAdd(velbar + 3 * c000 + a, u * (1.0f - w1[a]) * (1.0f - w2[a]));
-        unsafeAtomicAdd(velbar + 3 * cv[a][0] + a, u * (1.0f - w1[a]) * w2[a]);
-        unsafeAtomicAdd(velbar + 3 * cv[a][1] + a, u * w1[a] * (1.0f - w2[a]));
-        unsafeAtomicAdd(velbar + 3 * cv[a][2] + a, u * w1[a] * w2[a]);
+        val[4 * a] = u * (1.0f - w1[a]) * (1.0f - w2[a]);
+        val[4 * a + 1] = u * (1.0f - w1[a]) * w2[a];
+        val[4 * a + 2] = u * w1[a] * (1.0f - w2[a]);
+        val[4 * a + 3] = u * w1[a] * w2[a];
     }
+    wave_scatter12(velbar, e, g, val, act);
 }
 
 // Adjoint of staggered_advect at the slot positions before the step (parts0): pbar (the
 // cotangent of the new positions) becomes the cotangent of parts0; velbar accumulates.
+// Thread k handles slot order[k] (order null: slot k).
 __global__ void __launch_bounds__(256)
-k_advect_adj(const float* __restrict__ parts0, long long nslots, const float* __restrict__ vel, GridInfo g, float dt,
-             float* __restrict__ pbar, float* __restrict__ velbar) {
-    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    const float p[3] = {parts0[3 * s], parts0[3 * s + 1], parts0[3 * s + 2]};
-    const bool in = (p[0] > 0.0f) & (p[0] < g.Lx) & (p[1] > 0.0f) & (p[1] < g.Ly) & (p[2] > 0.0f) & (p[2] < g.Lz);
-    if (!in) {  // the dead marker does not depend on anything
-        pbar[3 * s] = 0.0f;
-        pbar[3 * s + 1] = 0.0f;
-        pbar[3 * s + 2] = 0.0f;
-        return;
+k_advect_adj(const float* __restrict__ parts0, long long nslots, const int* __restrict__ order,
+             const float* __restrict__ vel, GridInfo g, float dt, float* __restrict__ pbar, float* __restrict__ velbar) {
+    // every lane runs to the end (the scatter works across the wavefront); lanes past the
+    // last slot or on a dead slot do the arithmetic on a stand-in position and add nothing
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool slot = k < nslots;
+    const long long s = slot ? (order ? (long long)order[k] : k) : 0;
+    float p[3] = {0.5f * g.Lx, 0.5f * g.Ly, 0.5f * g.Lz};
+    if (slot) {
+        p[0] = parts0[3 * s];
+        p[1] = parts0[3 * s + 1];
+        p[2] = parts0[3 * s + 2];
+    }
+    const bool in = slot & (p[0] > 0.0f) & (p[0] < g.Lx) & (p[1] > 0.0f) & (p[1] < g.Ly) & (p[2] > 0.0f) & (p[2] < g.Lz);
+    if (!in) {
+        p[0] = 0.5f * g.Lx;
+        p[1] = 0.5f * g.Ly;
+        p[2] = 0.5f * g.Lz;
     }
     StagEval e1, e2, e3, e4;
     float k1[3], k2[3], k3[3], k4[3], t[3];
@@ -405,7 +453,12 @@ k_advect_adj(const float* __restrict__ parts0, long long nslots, const float* __
 #pragma unroll
     for (int a = 0; a < 3; a++) t[a] = p[a] + k3[a] * dt;
     stag_eval(vel, t, g, e4, k4);
-    const float pb[3] = {pbar[3 * s], pbar[3 * s + 1], pbar[3 * s + 2]};
+    float pb[3] = {0.0f, 0.0f, 0.0f};
+    if (in) {
+        pb[0] = pbar[3 * s];
+        pb[1] = pbar[3 * s + 1];
+        pb[2] = pbar[3 * s + 2];
+    }
     const float c6 = dt / 6.0f;
     float acc[3], kb1[3], kb2[3], kb3[3], kb4[3], xb[3];
 #pragma unroll
@@ -416,27 +469,30 @@ k_advect_adj(const float* __restrict__ parts0, long long nslots, const float* __
         kb3[a] = 2.0f * c6 * pb[a];
         kb4[a] = c6 * pb[a];
     }
-    stag_adj(e4, kb4, g, velbar, xb);
+    stag_adj(e4, kb4, g, velbar, xb, in);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         acc[a] += xb[a];
         kb3[a] += dt * xb[a];
     }
-    stag_adj(e3, kb3, g, velbar, xb);
+    stag_adj(e3, kb3, g, velbar, xb, in);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         acc[a] += xb[a];
         kb2[a] += dt * 0.5f * xb[a];
     }
-    stag_adj(e2, kb2, g, velbar, xb);
+    stag_adj(e2, kb2, g, velbar, xb, in);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         acc[a] += xb[a];
         kb1[a] += dt * 0.5f * xb[a];
     }
-    stag_adj(e1, kb1, g, velbar, xb);
+    stag_adj(e1, kb1, g, velbar, xb, in);
+    if (slot) {
+        // the dead marker does not depend on anything
 #pragma unroll
-    for (int a = 0; a < 3; a++) pbar[3 * s + a] = acc[a] + xb[a];
+        for (int a = 0; a < 3; a++) pbar[3 * s + a] = in ? acc[a] + xb[a] : 0.0f;
+    }
 }
 
 // Slots overwritten by add_particles! (test/runtests.jl:52-54) carry no cotangent back.
@@ -489,6 +545,51 @@ k_splat(const float* __restrict__ parts, long long nslots, SplatCfg sc, double* 
             }
         }
     }
+}
+
+// Same density, per block in LDS (M <= SPLAT_LDS cells): block b accumulates the slots
+// b, b + grid, ... into its own grid and writes it to partial[b * M ...]; k_splat_reduce
+// sums the block grids in block order.
+constexpr int SPLAT_LDS = 8192;
+__global__ void __launch_bounds__(256)
+k_splat_lds(const float* __restrict__ parts, long long nslots, SplatCfg sc, double* __restrict__ partial) {
+    __shared__ double acc[SPLAT_LDS];
+    const int M = sc.res[0] * sc.res[1] * sc.res[2];
+    for (int j = threadIdx.x; j < M; j += 256) acc[j] = 0.0;
+    __syncthreads();
+    const float R2 = sc.R * sc.R, k = -0.5f / (sc.sigma * sc.sigma);
+    for (long long s = (long long)blockIdx.x * 256 + threadIdx.x; s < nslots; s += (long long)gridDim.x * 256) {
+        const float p[3] = {parts[3 * s], parts[3 * s + 1], parts[3 * s + 2]};
+        if (!splat_live(p, sc)) continue;
+        int lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) splat_range(p[a], sc.h[a], sc.R, sc.res[a], lo[a], hi[a]);
+        for (int jz = lo[2]; jz <= hi[2]; jz++) {
+            const float dz = ((float)jz + 0.5f) * sc.h[2] - p[2];
+            for (int jy = lo[1]; jy <= hi[1]; jy++) {
+                const float dy = ((float)jy + 0.5f) * sc.h[1] - p[1];
+                const float r2yz = dy * dy + dz * dz;
+                if (r2yz > R2) continue;
+                for (int jx = lo[0]; jx <= hi[0]; jx++) {
+                    const float dx = ((float)jx + 0.5f) * sc.h[0] - p[0];
+                    const float r2 = dx * dx + r2yz;
+                    if (r2 > R2) continue;
+                    atomicAdd(&acc[jx + sc.res[0] * (jy + sc.res[1] * jz)], (double)(__expf(r2 * k) * sc.inv_n));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    double* out = partial + (size_t)blockIdx.x * M;
+    for (int j = threadIdx.x; j < M; j += 256) out[j] = acc[j];
+}
+__global__ void __launch_bounds__(256)
+k_splat_reduce(const double* __restrict__ partial, int nblocks, int M, double* __restrict__ rho) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= M) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; b++) s += partial[(size_t)b * M + j];
+    rho[j] = s;
 }
 
 // resid = 2/M (rho - target) (the loss cotangent of rho); per-block FP64 sums of (rho - target)^2
@@ -597,16 +698,30 @@ int launch_restrict_adj(const GridInfo& g, float4* psibar, const float4* psi_in,
 int launch_normalize_adj(const GridInfo& g, float4* psibar, const float4* psi_in, hipStream_t s) {
     ISF_LAUNCH(k_normalize_adj, grid1d(ncells(g), 256), dim3(256), s, psibar, psi_in, ncells(g));
 }
-int launch_advect_adj(const GridInfo& g, const float* parts0, long long nslots, const float* vel, float dt, float* pbar,
-                      float* velbar, hipStream_t s) {
+int launch_advect_adj(const GridInfo& g, const float* parts0, long long nslots, const int* order, const float* vel,
+                      float dt, float* pbar, float* velbar, hipStream_t s) {
     if (nslots <= 0) return 0;
-    ISF_LAUNCH(k_advect_adj, grid1d((size_t)nslots, 256), dim3(256), s, parts0, nslots, vel, g, dt, pbar, velbar);
+    ISF_LAUNCH(k_advect_adj, grid1d((size_t)nslots, 256), dim3(256), s, parts0, nslots, order, vel, g, dt, pbar, velbar);
 }
 int launch_zero_slots(float* pbar, long long nslots, long long first, long long n, hipStream_t s) {
     if (n <= 0 || nslots <= 0) return 0;
     ISF_LAUNCH(k_zero_slots, grid1d((size_t)n, 256), dim3(256), s, pbar, nslots, first, n);
 }
-int launch_splat(const float* parts, long long nslots, const SplatCfg& sc, double* rho, hipStream_t s) {
+constexpr int SPLAT_BLOCKS = 256;
+size_t splat_partial_doubles(const SplatCfg& sc) {
+    const size_t M = (size_t)sc.res[0] * sc.res[1] * sc.res[2];
+    return M <= (size_t)SPLAT_LDS ? M * SPLAT_BLOCKS : 0;
+}
+// rho must hold zeros when partial is null (global atomics); otherwise it is overwritten
+int launch_splat(const float* parts, long long nslots, const SplatCfg& sc, double* rho, double* partial,
+                 hipStream_t s) {
+    const int M = sc.res[0] * sc.res[1] * sc.res[2];
+    if (partial && M <= SPLAT_LDS) {
+        hipLaunchKernelGGL(k_splat_lds, dim3(SPLAT_BLOCKS), dim3(256), 0, s, parts, nslots, sc, partial);
+        int e = (int)hipGetLastError();
+        if (e) return e;
+        ISF_LAUNCH(k_splat_reduce, grid1d((size_t)M, 256), dim3(256), s, partial, SPLAT_BLOCKS, M, rho);
+    }
     if (nslots <= 0) return 0;
     ISF_LAUNCH(k_splat, grid1d((size_t)nslots, 256), dim3(256), s, parts, nslots, sc, rho);
 }

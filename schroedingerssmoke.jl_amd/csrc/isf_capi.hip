@@ -281,6 +281,8 @@ int isf_destroy(isf_t h) {
     if (h->order_scratch) (void)hipFree(h->order_scratch);
     if (h->jump_partial) (void)hipFree(h->jump_partial);
     if (h->vjp_mem) (void)hipFree(h->vjp_mem);
+    if (h->vjp_order) (void)hipFree(h->vjp_order);
+    if (h->vjp_order_scratch) (void)hipFree(h->vjp_order_scratch);
     if (h->table_mem) (void)hipFree(h->table_mem);
     h->magic = 0;
     delete h;

@@ -79,6 +79,10 @@ struct isf_handle {
     bool order_sorted;               // slots are advected through `order` (else slot order)
     unsigned* jump_partial;          // per-block counts of the order measure
     void* vjp_mem;                   // scratch of the operator VJPs (allocated on first use)
+    int* vjp_order;                  // cell-sorted slot order of the advection VJP
+    void* vjp_order_scratch;
+    size_t vjp_order_bytes;
+    long long vjp_order_cap;
     long long jump_cap;
 };
 

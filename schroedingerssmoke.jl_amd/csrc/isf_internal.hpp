@@ -119,6 +119,9 @@ int launch_bump_counter(int* counter, hipStream_t s);
 size_t order_scratch_bytes(const GridInfo& g, long long nslots);
 int launch_particle_order(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
                           size_t scratch_bytes, hipStream_t s);
+// the same with single-cell buckets (the adjoint's scatter order)
+int launch_particle_order_cells(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
+                                size_t scratch_bytes, hipStream_t s);
 
 // ---- adjoint of the step and the differentiable rollout (isf_adjoint.hip)
 // Particle-density grid: cells of size h over the box L, Gaussian width sigma truncated
@@ -150,10 +153,16 @@ size_t restrict_adj_blocks(const GridInfo& g);
 int launch_restrict_adj(const GridInfo& g, float4* psibar, const float4* psi_in, const JetParams& j, double* partials,
                         double* out4, hipStream_t s);
 int launch_normalize_adj(const GridInfo& g, float4* psibar, const float4* psi_in, hipStream_t s);
-int launch_advect_adj(const GridInfo& g, const float* parts0, long long nslots, const float* vel, float dt, float* pbar,
-                      float* velbar, hipStream_t s);
+// order: slot processing order (null: slot order); cell-sorted slots make the velocity
+// scatter of a wavefront hit few cells
+int launch_advect_adj(const GridInfo& g, const float* parts0, long long nslots, const int* order, const float* vel,
+                      float dt, float* pbar, float* velbar, hipStream_t s);
 int launch_zero_slots(float* pbar, long long nslots, long long first, long long n, hipStream_t s);
-int launch_splat(const float* parts, long long nslots, const SplatCfg& sc, double* rho, hipStream_t s);
+// density: with partial (splat_partial_doubles(sc) doubles, 0 = unavailable) per-block LDS
+// grids reduced in block order, rho overwritten; without, global atomics into a zeroed rho
+size_t splat_partial_doubles(const SplatCfg& sc);
+int launch_splat(const float* parts, long long nslots, const SplatCfg& sc, double* rho, double* partial,
+                 hipStream_t s);
 size_t splat_loss_blocks(int M);
 int launch_splat_loss(const double* rho, const float* target, int M, float* resid, double* partials, double* out,
                       hipStream_t s);

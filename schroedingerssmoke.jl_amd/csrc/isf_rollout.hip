@@ -50,6 +50,10 @@ struct isf_rollout {
     float4 *W, *psibar;
     float *f, *fbar, *vel, *velbar, *parts, *parts_final, *pbar, *phibar, *target, *resid;
     double *rho, *partials, *loss_dev, *vacc;
+    double* splat_part;  // per-block density grids (null: global atomics)
+    int* adj_order;      // cell-sorted slot order of the advection adjoint
+    char* adj_order_scratch;
+    size_t adj_order_bytes;
     // parameters of the last forward
     float v[3];
     float omega;
@@ -187,7 +191,12 @@ static int step_adjoint(isf_rollout_t r, int s, int k) {
     LAUNCH_CHECK(launch_velocity(g, psi5, h->hbar, 1, r->vel, st), "velocity_one_form");
     LAUNCH_CHECK(launch_fill_f32(r->velbar, 0.0f, 3 * r->N, st), "fill");
     if (r->P > 0) {
-        LAUNCH_CHECK(launch_advect_adj(g, r->t_p0[k], r->P, r->vel, h->dt, r->pbar, r->velbar, st), "advect adjoint");
+        // cell-sorted slot order: long same-cell runs for the scatter's wavefront reduction
+        LAUNCH_CHECK(launch_particle_order_cells(g, r->t_p0[k], r->P, r->adj_order, r->adj_order_scratch,
+                                                 r->adj_order_bytes, st),
+                     "particle order");
+        LAUNCH_CHECK(launch_advect_adj(g, r->t_p0[k], r->P, r->adj_order, r->vel, h->dt, r->pbar, r->velbar, st),
+                     "advect adjoint");
         long long first0, nsp;
         spawn_slots(r, it, &first0, &nsp);
         LAUNCH_CHECK(launch_zero_slots(r->pbar, r->P, first0, nsp, st), "spawn adjoint");
@@ -314,6 +323,12 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
     RA(&r->partials, std::max(4 * restrict_adj_blocks(r->g), splat_loss_blocks(r->M)));
     RA(&r->loss_dev, 1);
     RA(&r->vacc, 4 * (size_t)(r->T + r->C));
+    if (const size_t sp = splat_partial_doubles(r->sc)) RA(&r->splat_part, sp);
+    if (r->P > 0) {
+        r->adj_order_bytes = order_scratch_bytes(r->g, r->P);
+        RA(&r->adj_order, (size_t)r->P);
+        RA(&r->adj_order_scratch, r->adj_order_bytes);
+    }
 #undef RA
     hipStream_t st = h->stream;
     if ((e = join(h))) return bail(e);
@@ -389,7 +404,7 @@ int isf_rollout_forward(isf_rollout_t r, const float velocity[3], const float* t
         HIP_CHECK(hipMemcpyAsync(r->parts_final, r->parts, sizeof(float) * 3 * r->P, hipMemcpyDeviceToDevice, st),
                   "copy");
     LAUNCH_CHECK(launch_fill_f64(r->rho, 0.0, (size_t)r->M, st), "fill");
-    LAUNCH_CHECK(launch_splat(r->parts_final, r->P, r->sc, r->rho, st), "splat");
+    LAUNCH_CHECK(launch_splat(r->parts_final, r->P, r->sc, r->rho, r->splat_part, st), "splat");
     LAUNCH_CHECK(launch_splat_loss(r->rho, r->target, r->M, r->resid, r->partials, r->loss_dev, st), "loss");
     double sum = 0.0;
     HIP_CHECK(hipMemcpyAsync(&sum, r->loss_dev, sizeof(double), hipMemcpyDeviceToHost, st), "loss copy");
@@ -593,7 +608,26 @@ int isf_vjp_staggered_advect(isf_t h, const float* parts0, long long nslots, con
     CHECK_PTR(pbar, "pbar");
     CHECK_PTR(velbar, "velbar");
     if (nslots < 0) return fail(ISF_EINVAL, "bad nslots");
-    LAUNCH_CHECK(launch_advect_adj(h->g, parts0, nslots, vel ? vel : h->vel, h->dt, pbar, velbar, h->stream),
+    if (nslots == 0) return ISF_OK;
+    if (nslots > h->vjp_order_cap) {
+        if (h->vjp_order) (void)hipFree(h->vjp_order);
+        if (h->vjp_order_scratch) (void)hipFree(h->vjp_order_scratch);
+        h->vjp_order = nullptr;
+        h->vjp_order_scratch = nullptr;
+        h->vjp_order_cap = 0;
+        const size_t sb = order_scratch_bytes(h->g, nslots);
+        hipError_t he = hipMalloc(&h->vjp_order, sizeof(int) * (size_t)nslots);
+        if (he == hipSuccess) he = hipMalloc(&h->vjp_order_scratch, sb);
+        if (he != hipSuccess) return fail(ISF_ENOMEM, "advection VJP order: %s", hipGetErrorString(he));
+        h->vjp_order_bytes = sb;
+        h->vjp_order_cap = nslots;
+    }
+    // cell-sorted slot order: long same-cell runs for the scatter's wavefront reduction
+    LAUNCH_CHECK(launch_particle_order_cells(h->g, parts0, nslots, h->vjp_order, h->vjp_order_scratch,
+                                             h->vjp_order_bytes, h->stream),
+                 "particle order");
+    LAUNCH_CHECK(launch_advect_adj(h->g, parts0, nslots, h->vjp_order, vel ? vel : h->vel, h->dt, pbar, velbar,
+                                   h->stream),
                  "advect adjoint");
     return ISF_OK;
 }

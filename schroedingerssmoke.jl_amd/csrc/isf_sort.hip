@@ -7,6 +7,7 @@
 // order never changes a result.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -16,16 +17,18 @@
 
 namespace isf {
 
-static int bucket_count(const GridInfo& g) { return (g.nx / 8) * g.ny * g.nz; }
+// xs: log2 of the x cells per bucket (3: buckets of 8 cells for the forward advection,
+// 0: single cells for the adjoint's scatter)
+static int bucket_count(const GridInfo& g, int xs) { return (g.nx >> xs) * g.ny * g.nz; }
 
-static unsigned key_bits(const GridInfo& g) {
+static unsigned key_bits(const GridInfo& g, int xs) {
     unsigned b = 1;
-    while ((1ull << b) <= (unsigned long long)bucket_count(g)) b++;
+    while ((1ull << b) <= (unsigned long long)bucket_count(g, xs)) b++;
     return b;
 }
 
 __global__ void __launch_bounds__(256)
-k_bucket_keys(const float* __restrict__ parts, long long nslots, GridInfo g, int nb, unsigned* __restrict__ keys,
+k_bucket_keys(const float* __restrict__ parts, long long nslots, GridInfo g, int nb, int xs, unsigned* __restrict__ keys,
               int* __restrict__ slots) {
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
@@ -34,7 +37,7 @@ k_bucket_keys(const float* __restrict__ parts, long long nslots, GridInfo g, int
     if ((x > 0.0f) & (x < g.Lx) & (y > 0.0f) & (y < g.Ly) & (z > 0.0f) & (z < g.Lz)) {
         const int ix = min((int)(x * g.idx), g.nx - 1), iy = min((int)(y * g.idy), g.ny - 1);
         const int iz = min((int)(z * g.idz), g.nz - 1);
-        key = (unsigned)((ix >> 3) + (g.nx >> 3) * (iy + g.ny * iz));
+        key = (unsigned)((ix >> xs) + (g.nx >> xs) * (iy + g.ny * iz));
     }
     keys[s] = key;
     slots[s] = (int)s;
@@ -42,19 +45,21 @@ k_bucket_keys(const float* __restrict__ parts, long long nslots, GridInfo g, int
 
 static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
-static size_t rocprim_bytes(const GridInfo& g, long long n) {
+static size_t rocprim_bytes(const GridInfo& g, long long n, int xs) {
     size_t bytes = 0;
     (void)rocprim::radix_sort_pairs(nullptr, bytes, (unsigned*)nullptr, (unsigned*)nullptr, (int*)nullptr,
-                                    (int*)nullptr, (size_t)n, 0u, key_bits(g));
+                                    (int*)nullptr, (size_t)n, 0u, key_bits(g, xs));
     return bytes;
 }
 
+// scratch for either key width
 size_t order_scratch_bytes(const GridInfo& g, long long nslots) {
-    return 3 * align256(sizeof(int) * (size_t)nslots) + align256(rocprim_bytes(g, nslots));
+    return 3 * align256(sizeof(int) * (size_t)nslots) +
+           align256(std::max(rocprim_bytes(g, nslots, 3), rocprim_bytes(g, nslots, 0)));
 }
 
-int launch_particle_order(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
-                          size_t scratch_bytes, hipStream_t s) {
+static int particle_order(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
+                          size_t scratch_bytes, hipStream_t s, int xs) {
     if (nslots <= 0) return 0;
     if (scratch_bytes < order_scratch_bytes(g, nslots)) return (int)hipErrorInvalidValue;
     char* p = static_cast<char*>(scratch);
@@ -64,13 +69,23 @@ int launch_particle_order(const GridInfo& g, const float* parts, long long nslot
     int* slots = reinterpret_cast<int*>(p + 2 * a);
     void* tmp = p + 3 * a;
     size_t tmp_bytes = scratch_bytes - 3 * a;
-    const int nb = bucket_count(g);
-    hipLaunchKernelGGL(k_bucket_keys, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, parts, nslots, g, nb,
+    const int nb = bucket_count(g, xs);
+    hipLaunchKernelGGL(k_bucket_keys, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, parts, nslots, g, nb, xs,
                        keys_in, slots);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
-    e = rocprim::radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, slots, order, (size_t)nslots, 0u, key_bits(g), s);
+    e = rocprim::radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, slots, order, (size_t)nslots, 0u, key_bits(g, xs),
+                                  s);
     return (int)e;
+}
+
+int launch_particle_order(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
+                          size_t scratch_bytes, hipStream_t s) {
+    return particle_order(g, parts, nslots, order, scratch, scratch_bytes, s, 3);
+}
+int launch_particle_order_cells(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
+                                size_t scratch_bytes, hipStream_t s) {
+    return particle_order(g, parts, nslots, order, scratch, scratch_bytes, s, 0);
 }
 
 // ---- initial particle cloud uniform in the nozzle cylinder, stored in a spatially
