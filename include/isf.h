@@ -158,6 +158,46 @@ int isf_particles_keep(isf_t h, isf_particles_t p, const int* ind, long long n);
 int isf_particles_state(isf_t h, isf_particles_t p, long long* length, long long* active_count, const float** x,
                         const float** y, const float** z, const int** active);
 
+/* ---- MatlabPort operators (src/MatlabPort.jl:87-273) on that module's layout ------------ */
+/* psi1, psi2: separate complex arrays (2 floats per cell, re/im); vx, vy, vz, f: separate
+ * real arrays; q: complex; all x fastest, N = nx*ny*nz cells.  Float32 arithmetic with the
+ * cell formulas of the operators above.  Outputs may be the inputs (in place) except
+ * where noted; results are returned in new arrays by the reference, in out* here. */
+/* SchroedingerFlow(obj, psi1, psi2), src/MatlabPort.jl:87-93 */
+int isf_mp_schroedinger_flow(isf_t h, const float* psi1, const float* psi2, float* out1, float* out2);
+/* Normalize(psi1, psi2), src/MatlabPort.jl:155-160 */
+int isf_mp_normalize(isf_t h, const float* psi1, const float* psi2, float* out1, float* out2);
+/* VelocityOneForm(obj, psi1, psi2, hbar), src/MatlabPort.jl:106-126 (hbar 1 in the
+ * reference's default); vx, vy, vz must not overlap psi1, psi2 */
+int isf_mp_velocity_one_form(isf_t h, const float* psi1, const float* psi2, float hbar, float* vx, float* vy,
+                             float* vz);
+/* StaggeredSharp(obj, vx, vy, vz) = (vx/dx, vy/dy, vz/dz), src/MatlabPort.jl:99-104 */
+int isf_mp_staggered_sharp(isf_t h, const float* vx, const float* vy, const float* vz, float* ux, float* uy,
+                           float* uz);
+/* Div(obj, vx, vy, vz), src/MatlabPort.jl:162-170; f must not overlap vx, vy, vz */
+int isf_mp_div(isf_t h, const float* vx, const float* vy, const float* vz, float* f);
+/* PoissonSolve(obj, f), src/MatlabPort.jl:128-138: q complex with imaginary part 0 (the
+ * reference's is FFT rounding noise of a real solve) */
+int isf_mp_poisson_solve(isf_t h, const float* f, float* q);
+/* GaugeTransform(psi1, psi2, q) = psi * exp(i q) for complex q, src/MatlabPort.jl:140-145 */
+int isf_mp_gauge_transform(isf_t h, const float* psi1, const float* psi2, const float* q, float* out1, float* out2);
+/* PressureProject(obj, psi1, psi2), src/MatlabPort.jl:147-153 (q = -PoissonSolve(Div(
+ * VelocityOneForm(psi))), psi exp(i q)); stages psi in a 16 N byte buffer of the handle,
+ * allocated on first use */
+int isf_mp_pressure_project(isf_t h, const float* psi1, const float* psi2, float* out1, float* out2);
+/* The MatlabPort driver's jet constraint (examples/matlabport/simple.jl:104-110), in place:
+ * psi_a[isJet] = |psi_a[isJet]| exp(i phase); phase = k.x - omgterm (phase_mode 0, the
+ * driver's form) or sum(k + x) - omgterm (1, src/particles.jl:1); isjet NULL = the nozzle
+ * of isf_jet_configure */
+int isf_mp_restrict_velocity(isf_t h, float* psi1, float* psi2, const unsigned char* isjet, const float kvec[3],
+                              float omgterm, int phase_mode);
+/* StaggeredVelocity(px, py, pz, torus, vx, vy, vz) at n points, src/MatlabPort.jl:237-273 */
+int isf_mp_staggered_velocity(isf_t h, const float* px, const float* py, const float* pz, long long n, const float* vx,
+                              const float* vy, const float* vz, float* ux, float* uy, float* uz);
+/* StaggeredAdvect(p, torus, vx, vy, vz, dt) over the active slots of a particle store,
+ * src/MatlabPort.jl:201-222 */
+int isf_mp_staggered_advect(isf_t h, isf_particles_t p, const float* vx, const float* vy, const float* vz, float dt);
+
 /* ---- asynchronous particle export (examples/jet.jl:83, Array(particles) per frame) ------ */
 /* Double-buffered copies of the particle positions (nslots x 3 floats) into pinned host
  * memory.  push stages the positions on the handle's stream after all queued work and

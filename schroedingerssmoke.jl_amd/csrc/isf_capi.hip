@@ -37,8 +37,8 @@ int hip_fail(hipError_t e, const char* what) {
 const char* last_error() { return g_err.c_str(); }
 }  // namespace isf
 
-static float* f_of(isf_t h) { return reinterpret_cast<float*>(h->S); }
-static float2* P_of(isf_t h) { return reinterpret_cast<float2*>(reinterpret_cast<char*>(h->S) + 4 * h->N); }
+float* isf::f_of(isf_t h) { return reinterpret_cast<float*>(h->S); }
+float2* isf::P_of(isf_t h) { return reinterpret_cast<float2*>(reinterpret_cast<char*>(h->S) + 4 * h->N); }
 
 int isf::check_handle(isf_t h) {
     if (!h || h->magic != kMagic) return fail(ISF_EHANDLE, "null or destroyed isf handle");
@@ -281,6 +281,7 @@ int isf_destroy(isf_t h) {
     if (h->order_scratch) (void)hipFree(h->order_scratch);
     if (h->jump_partial) (void)hipFree(h->jump_partial);
     if (h->vjp_mem) (void)hipFree(h->vjp_mem);
+    if (h->mp_psi) (void)hipFree(h->mp_psi);
     if (h->vjp_order) (void)hipFree(h->vjp_order);
     if (h->vjp_order_scratch) (void)hipFree(h->vjp_order_scratch);
     if (h->table_mem) (void)hipFree(h->table_mem);
@@ -334,10 +335,12 @@ static int flow(isf_t h, float* psi, bool normalize) {
     return ISF_OK;
 }
 
-static double poisson_scale(isf_t h) { return -0.25 * 2.0 / (double)h->N; }
+}  // extern "C"
+
+double isf::poisson_scale(isf_t h) { return -0.25 * 2.0 / (double)h->N; }
 
 // velocity one-form (hbar=1) + div + Poisson + gauge; mode C2R_GAUGE(_RESTRICT)
-static int project(isf_t h, float* psi, int c2r_mode, const JetParams* jet, double* parts) {
+int isf::project(isf_t h, float* psi, int c2r_mode, const JetParams* jet, double* parts) {
     float4* p = reinterpret_cast<float4*>(psi);
     LAUNCH_CHECK(launch_div_psi(h->g, p, f_of(h), h->stream), "div(one_form(psi))");
     LAUNCH_CHECK(launch_r2c_x(h->g, f_of(h), P_of(h), h->tb, h->stream), "poisson x forward");
@@ -345,6 +348,8 @@ static int project(isf_t h, float* psi, int c2r_mode, const JetParams* jet, doub
     LAUNCH_CHECK(launch_c2r_x(h->g, P_of(h), c2r_mode, nullptr, p, jet, parts, h->tb, h->stream), "poisson x inverse + gauge");
     return ISF_OK;
 }
+
+extern "C" {
 
 int isf_schroedinger_flow(isf_t h, float* psi) {
     CHECK_HJ(h);

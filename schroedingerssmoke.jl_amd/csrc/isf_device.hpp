@@ -165,4 +165,25 @@ __device__ __forceinline__ void stag_vel(const float* __restrict__ vel, const fl
         u[a] = (1.0f - w1[a]) * ((1.0f - w2[a]) * v0[a] + w2[a] * pp[a]) + w1[a] * ((1.0f - w2[a]) * pp2[a] + w2[a] * vn[a]);
 }
 
+// StaggeredVelocity on separate component arrays vx, vy, vz (src/MatlabPort.jl:237-273):
+// the same cells, weights and arithmetic as stag_vel, so both layouts give equal results.
+__device__ __forceinline__ void stag_vel_split(const float* __restrict__ vx, const float* __restrict__ vy,
+                                               const float* __restrict__ vz, const float p_in[3], const GridInfo& g,
+                                               float u[3]) {
+    int i[3], ip[3];
+    float w[3];
+    stag_index(p_in, g, i, ip, w);
+    const unsigned sy = (unsigned)g.nx, sz = (unsigned)g.nx * (unsigned)g.ny;
+    const unsigned rjk = sy * i[1] + sz * i[2], rjpk = sy * ip[1] + sz * i[2];
+    const unsigned rjkp = sy * i[1] + sz * ip[2], rjpkp = sy * ip[1] + sz * ip[2];
+    const float v0[3] = {vx[i[0] + rjk], vy[i[0] + rjk], vz[i[0] + rjk]};
+    const float vn[3] = {vx[i[0] + rjpkp], vy[ip[0] + rjkp], vz[ip[0] + rjpk]};
+    const float pp[3] = {vx[i[0] + rjpk], vy[ip[0] + rjk], vz[ip[0] + rjk]};
+    const float pp2[3] = {vx[i[0] + rjkp], vy[i[0] + rjkp], vz[i[0] + rjpk]};
+    const float w1[3] = {w[2], w[2], w[1]}, w2[3] = {w[1], w[0], w[0]};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        u[a] = (1.0f - w1[a]) * ((1.0f - w2[a]) * v0[a] + w2[a] * pp[a]) + w1[a] * ((1.0f - w2[a]) * pp2[a] + w2[a] * vn[a]);
+}
+
 }  // namespace isf

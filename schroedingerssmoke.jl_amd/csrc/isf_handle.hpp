@@ -79,6 +79,7 @@ struct isf_handle {
     bool order_sorted;               // slots are advected through `order` (else slot order)
     unsigned* jump_partial;          // per-block counts of the order measure
     void* vjp_mem;                   // scratch of the operator VJPs (allocated on first use)
+    float4* mp_psi;                  // interleaved staging of the MatlabPort PressureProject
     int* vjp_order;                  // cell-sorted slot order of the advection VJP
     void* vjp_order_scratch;
     size_t vjp_order_bytes;
@@ -89,6 +90,14 @@ struct isf_handle {
 
 namespace isf {
 int check_handle(isf_t h);
+// the spectrum buffer's regions: the Poisson right-hand side (first 4N bytes, also the
+// real q of a Poisson solve) and the Poisson half spectrum (from byte 4N)
+float* f_of(isf_t h);
+float2* P_of(isf_t h);
+double poisson_scale(isf_t h);
+// pressure projection of interleaved psi: one-form (hbar 1) + div + Poisson + gauge
+// (c2r_mode 1), or + jet restriction (mode 2)
+int project(isf_t h, float* psi, int c2r_mode, const JetParams* jet, double* parts);
 // order the handle's streams after its pending side-stream and batched work
 int join(isf_t h);
 }  // namespace isf

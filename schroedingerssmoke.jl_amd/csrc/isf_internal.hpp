@@ -76,6 +76,11 @@ int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const Dev
 int launch_flow_plane(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s, Batch b = Batch{});
 int launch_flow_x_inv(const GridInfo& g, const float2* S, float4* psi, bool normalize, const DevTables& t,
                       hipStream_t s, Batch b = Batch{});
+// the same x passes on the split-component layout (psi1 and psi2 as separate complex arrays)
+int launch_flow_x_fwd_split(const GridInfo& g, const float2* sa, const float2* sb, float2* S, const DevTables& t,
+                            hipStream_t s);
+int launch_flow_x_inv_split(const GridInfo& g, const float2* S, float2* oa, float2* ob, const DevTables& t,
+                            hipStream_t s);
 int launch_r2c_x(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s, Batch b = Batch{});
 int launch_poisson_plane(const GridInfo& g, float2* P, double scale, const DevTables& t, hipStream_t s,
                          Batch b = Batch{});

@@ -33,11 +33,13 @@ template <int NX> struct XCfg {
 // ======================================================================== Schroedinger x passes
 
 // Forward x transform of TY psi lines (fixed z), both components, written into the
-// (c, kx) planes of S with y fastest (src/parallel.jl:179-181, x part of fft!).
+// (c, kx) planes of S with y fastest (src/parallel.jl:179-181, x part of fft!).  With
+// sa != null the components come from two separate complex arrays sa, sb (the MatlabPort
+// layout, src/MatlabPort.jl:87-93) instead of psi.
 template <int NX, int TY>
 __global__ void __launch_bounds__(2 * TY * XCfg<NX>::T)
-k_flow_x_fwd(const float4* __restrict__ psi, float2* __restrict__ S, int ny, int nz, const float2* __restrict__ tw,
-             Batch B) {
+k_flow_x_fwd(const float4* __restrict__ psi, const float2* __restrict__ sa, const float2* __restrict__ sb,
+             float2* __restrict__ S, int ny, int nz, const float2* __restrict__ tw, Batch B) {
     constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
     if (B.inst) {
         const Inst& m = B.inst[blockIdx.z];
@@ -51,12 +53,21 @@ k_flow_x_fwd(const float4* __restrict__ psi, float2* __restrict__ S, int ny, int
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * TY, z = blockIdx.y;
     FftTw<NX, E>::fill(twl, tw, tid, NT);
-    const float4* src = psi + ((size_t)z * ny + y0) * NX;
-    for (int i = tid; i < TY * NX; i += NT) {
-        const int l = i / NX, x = i % NX;
-        const float4 a = src[i];
-        lds[l * LS + pad16(x)] = make_float2(a.x, a.y);
-        lds[(TY + l) * LS + pad16(x)] = make_float2(a.z, a.w);
+    const size_t line0 = ((size_t)z * ny + y0) * NX;
+    if (sa) {
+        for (int i = tid; i < TY * NX; i += NT) {
+            const int l = i / NX, x = i % NX;
+            lds[l * LS + pad16(x)] = sa[line0 + i];
+            lds[(TY + l) * LS + pad16(x)] = sb[line0 + i];
+        }
+    } else {
+        const float4* src = psi + line0;
+        for (int i = tid; i < TY * NX; i += NT) {
+            const int l = i / NX, x = i % NX;
+            const float4 a = src[i];
+            lds[l * LS + pad16(x)] = make_float2(a.x, a.y);
+            lds[(TY + l) * LS + pad16(x)] = make_float2(a.z, a.w);
+        }
     }
     __syncthreads();
     const int line = tid / T;
@@ -68,11 +79,12 @@ k_flow_x_fwd(const float4* __restrict__ psi, float2* __restrict__ S, int ny, int
 }
 
 // Inverse x transform from S back to psi lines; optionally normalize_psi
-// (src/parallel.jl:162-165,184-185).  The 1/N of ifft is folded into the mask.
+// (src/parallel.jl:162-165,184-185).  The 1/N of ifft is folded into the mask.  With
+// oa != null the components go to two separate complex arrays oa, ob instead of psi.
 template <int NX, int TY, bool NORMALIZE>
 __global__ void __launch_bounds__(2 * TY * XCfg<NX>::T)
-k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, int ny, int nz, const float2* __restrict__ tw,
-             Batch B) {
+k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, float2* __restrict__ oa, float2* __restrict__ ob,
+             int ny, int nz, const float2* __restrict__ tw, Batch B) {
     constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
     if (B.inst) {
         const Inst& m = B.inst[blockIdx.z];
@@ -93,13 +105,25 @@ k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, int ny, int
     __syncthreads();
     const int line = tid / T;
     fft_line<NX, E, 1>(LayContig{lds + line * LS}, tid % T, true, twl);
-    float4* dst = psi + ((size_t)z * ny + y0) * NX;
-    for (int i = tid; i < TY * NX; i += NT) {
-        const int l = i / NX, x = i % NX;
-        const float2 a = lds[l * LS + pad16(x)], b = lds[(TY + l) * LS + pad16(x)];
-        float4 p = make_float4(a.x, a.y, b.x, b.y);
-        if (NORMALIZE) p = normalized(p);
-        dst[i] = p;
+    const size_t line0 = ((size_t)z * ny + y0) * NX;
+    if (oa) {
+        for (int i = tid; i < TY * NX; i += NT) {
+            const int l = i / NX, x = i % NX;
+            const float2 a = lds[l * LS + pad16(x)], b = lds[(TY + l) * LS + pad16(x)];
+            float4 p = make_float4(a.x, a.y, b.x, b.y);
+            if (NORMALIZE) p = normalized(p);
+            oa[line0 + i] = make_float2(p.x, p.y);
+            ob[line0 + i] = make_float2(p.z, p.w);
+        }
+    } else {
+        float4* dst = psi + line0;
+        for (int i = tid; i < TY * NX; i += NT) {
+            const int l = i / NX, x = i % NX;
+            const float2 a = lds[l * LS + pad16(x)], b = lds[(TY + l) * LS + pad16(x)];
+            float4 p = make_float4(a.x, a.y, b.x, b.y);
+            if (NORMALIZE) p = normalized(p);
+            dst[i] = p;
+        }
     }
 }
 
@@ -480,8 +504,8 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
 
 
 // ---- z-marching stencil tiles
-// A block of BX x 4 threads covers BX consecutive x and 4 consecutive y and walks TZ
-// consecutive z.  psi(c + e_z) of one z step is psi(c) of the next, psi(c + e_x) comes
+// A block of BX x 4 threads covers BX consecutive x and 4 consecutive y and walks tz
+// consecutive z (tz = 16, or fewer on small grids so that a launch has enough blocks).  psi(c + e_z) of one z step is psi(c) of the next, psi(c + e_x) comes
 // from the neighbouring lane and psi(c + e_y) from the row above through LDS, so every
 // psi row is fetched once per block (plus one halo row on each side in y and z) and
 // every forward-edge angle
@@ -490,7 +514,7 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
 // hardware deals consecutive block ids to the 8 XCDs in turn, so tile t is given to id
 // (t mod C) * 8 + t / C, C = blocks / 8; each XCD then walks one contiguous range of
 // tiles, and the halo rows shared by neighbouring tiles stay in that XCD's L2.
-constexpr int ST_BY = 4, ST_TZ = 16;
+constexpr int ST_BY = 4, ST_TZ = 16;  // rows per block, longest z walk
 
 __device__ __forceinline__ float4 shfl4(float4 v, int src) {
     return make_float4(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64), __shfl(v.w, src, 64));
@@ -519,13 +543,13 @@ __device__ __forceinline__ StTile stencil_tile() {
 // f(c) = sum_a (v_a(c) - v_a(c - e_a)) / d_a^2.
 template <int BX>
 __global__ void __launch_bounds__(BX * ST_BY)
-k_div_psi(const float4* __restrict__ psi, float* __restrict__ f, GridInfo g, Batch B) {
+k_div_psi(const float4* __restrict__ psi, float* __restrict__ f, GridInfo g, int tz, Batch B) {
     __shared__ float4 row_s[2][ST_BY][BX];
     __shared__ float vy_s[2][ST_BY][BX];
     const StTile tl = stencil_tile();
     int zslab = tl.bz;
     if (B.inst) {
-        const int nzs = g.nz / ST_TZ;
+        const int nzs = g.nz / tz;
         const Inst& m = B.inst[tl.bz / nzs];
         zslab = tl.bz % nzs;
         psi = m.psi;
@@ -534,7 +558,7 @@ k_div_psi(const float4* __restrict__ psi, float* __restrict__ f, GridInfo g, Bat
     }
     const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
     const int lane = threadIdx.x % 64;
-    const int x = tl.bx * BX + tx, y = tl.by * ST_BY + ty, z0 = zslab * ST_TZ;
+    const int x = tl.bx * BX + tx, y = tl.by * ST_BY + ty, z0 = zslab * tz;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const size_t sy = nx, sz = (size_t)nx * ny;
     const int xp = x + 1 < nx ? x + 1 : 0, xm = x > 0 ? x - 1 : nx - 1;
@@ -553,11 +577,11 @@ k_div_psi(const float4* __restrict__ psi, float* __restrict__ f, GridInfo g, Bat
     float4 ph = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (top | bottom) ph = psi[chalo + sz * z0];
 #pragma unroll 1
-    for (int k = 0; k < ST_TZ; k++) {
+    for (int k = 0; k < tz; k++) {
         const int z = z0 + k;
         const size_t row = sy * y + sz * z;
         float4 pz_n = pz, ph_n = ph;
-        if (k + 1 < ST_TZ) {
+        if (k + 1 < tz) {
             const int zn = zp + 1 < nz ? zp + 1 : 0;
             pz_n = psi[cx + sz * zn];
             if (top | bottom) ph_n = psi[chalo + sz * zp];
@@ -588,12 +612,13 @@ k_div_psi(const float4* __restrict__ psi, float* __restrict__ f, GridInfo g, Bat
 // in-place StaggeredSharp v_a *= 1/d_a (test/runtests.jl:61-62).
 template <int BX>
 __global__ void __launch_bounds__(BX * ST_BY)
-k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, float hbar, int sharp, Batch B) {
+k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, float hbar, int sharp, int tz,
+           Batch B) {
     __shared__ float4 row_s[2][ST_BY][BX];
     const StTile tl = stencil_tile();
     int zslab = tl.bz;
     if (B.inst) {
-        const int nzs = g.nz / ST_TZ;
+        const int nzs = g.nz / tz;
         const Inst& m = B.inst[tl.bz / nzs];
         zslab = tl.bz % nzs;
         psi = m.psi;
@@ -603,7 +628,7 @@ k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, 
     }
     const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
     const int lane = threadIdx.x % 64;
-    const int x = tl.bx * BX + tx, y = tl.by * ST_BY + ty, z0 = zslab * ST_TZ;
+    const int x = tl.bx * BX + tx, y = tl.by * ST_BY + ty, z0 = zslab * tz;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const size_t sy = nx, sz = (size_t)nx * ny;
     const int xp = x + 1 < nx ? x + 1 : 0, yp = y + 1 < ny ? y + 1 : 0;
@@ -616,11 +641,11 @@ k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, 
     float4 ph = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (top) ph = psi[cyp + sz * z0];
 #pragma unroll 1
-    for (int k = 0; k < ST_TZ; k++) {
+    for (int k = 0; k < tz; k++) {
         const int z = z0 + k;
         const size_t c = cx + sz * z;
         float4 pz_n = pz, ph_n = ph;
-        if (k + 1 < ST_TZ) {
+        if (k + 1 < tz) {
             const int zn = zp + 1 < nz ? zp + 1 : 0;
             pz_n = psi[cx + sz * zn];
             if (top) ph_n = psi[cyp + sz * zp];
@@ -882,16 +907,19 @@ static int x_tile(const GridInfo& g) {
 static inline unsigned members_of(const Batch& b) { return b.inst ? (unsigned)b.members : 1u; }
 
 template <int NX, int TY>
-static int flow_x_fwd_t(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s, Batch b) {
+static int flow_x_fwd_t(const GridInfo& g, const float4* psi, const float2* sa, const float2* sb, float2* S,
+                        const DevTables& t, hipStream_t s, Batch b) {
     dim3 grid(g.ny / TY, g.nz, members_of(b));
-    ISF_LAUNCH((k_flow_x_fwd<NX, TY>), grid, dim3(2 * TY * XCfg<NX>::T), s, psi, S, g.ny, g.nz, t.tw, b);
+    ISF_LAUNCH((k_flow_x_fwd<NX, TY>), grid, dim3(2 * TY * XCfg<NX>::T), s, psi, sa, sb, S, g.ny, g.nz, t.tw, b);
 }
 template <int NX, int TY>
-static int flow_x_inv_t(const GridInfo& g, const float2* S, float4* psi, bool nrm, const DevTables& t, hipStream_t s,
-                        Batch b) {
+static int flow_x_inv_t(const GridInfo& g, const float2* S, float4* psi, float2* oa, float2* ob, bool nrm,
+                        const DevTables& t, hipStream_t s, Batch b) {
     dim3 grid(g.ny / TY, g.nz, members_of(b));
-    if (nrm) ISF_LAUNCH((k_flow_x_inv<NX, TY, true>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, g.ny, g.nz, t.tw, b);
-    ISF_LAUNCH((k_flow_x_inv<NX, TY, false>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, g.ny, g.nz, t.tw, b);
+    if (nrm)
+        ISF_LAUNCH((k_flow_x_inv<NX, TY, true>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, oa, ob, g.ny, g.nz, t.tw,
+                   b);
+    ISF_LAUNCH((k_flow_x_inv<NX, TY, false>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, oa, ob, g.ny, g.nz, t.tw, b);
 }
 template <int NX, int TY>
 static int r2c_x_t(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s, Batch b) {
@@ -932,11 +960,19 @@ static int c2r_x_t(const GridInfo& g, const float2* P, int mode, float* q, float
     } while (0)
 
 int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s, Batch b) {
-    ISF_X_DISPATCH(flow_x_fwd_t, g, psi, S, t, s, b);
+    ISF_X_DISPATCH(flow_x_fwd_t, g, psi, nullptr, nullptr, S, t, s, b);
 }
 int launch_flow_x_inv(const GridInfo& g, const float2* S, float4* psi, bool normalize, const DevTables& t,
                       hipStream_t s, Batch b) {
-    ISF_X_DISPATCH(flow_x_inv_t, g, S, psi, normalize, t, s, b);
+    ISF_X_DISPATCH(flow_x_inv_t, g, S, psi, nullptr, nullptr, normalize, t, s, b);
+}
+int launch_flow_x_fwd_split(const GridInfo& g, const float2* sa, const float2* sb, float2* S, const DevTables& t,
+                            hipStream_t s) {
+    ISF_X_DISPATCH(flow_x_fwd_t, g, nullptr, sa, sb, S, t, s, Batch{});
+}
+int launch_flow_x_inv_split(const GridInfo& g, const float2* S, float2* oa, float2* ob, const DevTables& t,
+                            hipStream_t s) {
+    ISF_X_DISPATCH(flow_x_inv_t, g, S, nullptr, oa, ob, false, t, s, Batch{});
 }
 int launch_r2c_x(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s, Batch b) {
     ISF_X_DISPATCH(r2c_x_t, g, f, P, t, s, b);
@@ -1060,21 +1096,32 @@ int launch_poisson_plane(const GridInfo& g, float2* P, double scale, const DevTa
 static inline dim3 grid1d(size_t n, int nt) { return dim3((unsigned)((n + nt - 1) / nt)); }
 static inline size_t ncells(const GridInfo& g) { return (size_t)g.nx * g.ny * g.nz; }
 
-static dim3 stencil_grid(const GridInfo& g, int bx, const Batch& b) {
-    return dim3(g.nx / bx, g.ny / ST_BY, (g.nz / ST_TZ) * members_of(b));
+// z planes walked per block: 16, halved (down to 1) while the launch would have fewer
+// than kStencilBlocks blocks; nz >= 16 is a power of two, so tz divides it
+constexpr long kStencilBlocks = 2048;
+static int stencil_tz(const GridInfo& g, int bx, const Batch& b) {
+    const long base = (long)(g.nx / bx) * (g.ny / ST_BY) * members_of(b);
+    int tz = ST_TZ;
+    while (tz > 1 && base * (g.nz / tz) < kStencilBlocks) tz >>= 1;
+    return tz;
+}
+static dim3 stencil_grid(const GridInfo& g, int bx, int tz, const Batch& b) {
+    return dim3(g.nx / bx, g.ny / ST_BY, (g.nz / tz) * members_of(b));
 }
 
 int launch_div_psi(const GridInfo& g, const float4* psi, float* f, hipStream_t s, Batch b) {
-    if (g.nx >= 64) ISF_LAUNCH(k_div_psi<64>, stencil_grid(g, 64, b), dim3(64 * ST_BY), s, psi, f, g, b);
-    if (g.nx == 32) ISF_LAUNCH(k_div_psi<32>, stencil_grid(g, 32, b), dim3(32 * ST_BY), s, psi, f, g, b);
-    ISF_LAUNCH(k_div_psi<16>, stencil_grid(g, 16, b), dim3(16 * ST_BY), s, psi, f, g, b);
+    const int bx = g.nx >= 64 ? 64 : g.nx, tz = stencil_tz(g, bx, b);
+    const dim3 grid = stencil_grid(g, bx, tz, b);
+    if (bx == 64) ISF_LAUNCH(k_div_psi<64>, grid, dim3(64 * ST_BY), s, psi, f, g, tz, b);
+    if (bx == 32) ISF_LAUNCH(k_div_psi<32>, grid, dim3(32 * ST_BY), s, psi, f, g, tz, b);
+    ISF_LAUNCH(k_div_psi<16>, grid, dim3(16 * ST_BY), s, psi, f, g, tz, b);
 }
 int launch_velocity(const GridInfo& g, const float4* psi, float hbar, int sharp, float* vel, hipStream_t s, Batch b) {
-    if (g.nx >= 64)
-        ISF_LAUNCH(k_velocity<64>, stencil_grid(g, 64, b), dim3(64 * ST_BY), s, psi, vel, g, hbar, sharp, b);
-    if (g.nx == 32)
-        ISF_LAUNCH(k_velocity<32>, stencil_grid(g, 32, b), dim3(32 * ST_BY), s, psi, vel, g, hbar, sharp, b);
-    ISF_LAUNCH(k_velocity<16>, stencil_grid(g, 16, b), dim3(16 * ST_BY), s, psi, vel, g, hbar, sharp, b);
+    const int bx = g.nx >= 64 ? 64 : g.nx, tz = stencil_tz(g, bx, b);
+    const dim3 grid = stencil_grid(g, bx, tz, b);
+    if (bx == 64) ISF_LAUNCH(k_velocity<64>, grid, dim3(64 * ST_BY), s, psi, vel, g, hbar, sharp, tz, b);
+    if (bx == 32) ISF_LAUNCH(k_velocity<32>, grid, dim3(32 * ST_BY), s, psi, vel, g, hbar, sharp, tz, b);
+    ISF_LAUNCH(k_velocity<16>, grid, dim3(16 * ST_BY), s, psi, vel, g, hbar, sharp, tz, b);
 }
 int launch_div_vel(const GridInfo& g, const float* vel, float* f, hipStream_t s) {
     ISF_LAUNCH(k_div_vel, grid1d(ncells(g), 256), dim3(256), s, vel, f, g);

@@ -130,22 +130,30 @@ __global__ void k_copy_count(const int* __restrict__ src, int* __restrict__ dst)
 // in-grid test (the driver filters first)
 __global__ void __launch_bounds__(NT)
 k_advect_active(float* __restrict__ x, float* __restrict__ y, float* __restrict__ z, const int* __restrict__ active,
-                const int* __restrict__ count, const float* __restrict__ vel, GridInfo g, float dt) {
+                const int* __restrict__ count, const float* __restrict__ vel, const float* __restrict__ vx,
+                const float* __restrict__ vy, const float* __restrict__ vz, GridInfo g, float dt) {
     const int i = blockIdx.x * NT + threadIdx.x;
     if (i >= *count) return;
     const int a = active[i];
     const float p[3] = {x[a], y[a], z[a]};
+    // interleaved velocity (vel) or separate component arrays (vx, vy, vz)
+    auto U = [&](const float q[3], float u[3]) {
+        if (vel)
+            stag_vel(vel, q, g, u);
+        else
+            stag_vel_split(vx, vy, vz, q, g, u);
+    };
     float k1[3], k2[3], k3[3], k4[3], t[3];
-    stag_vel(vel, p, g, k1);
+    U(p, k1);
 #pragma unroll
     for (int c = 0; c < 3; c++) t[c] = p[c] + k1[c] * dt / 2.0f;
-    stag_vel(vel, t, g, k2);
+    U(t, k2);
 #pragma unroll
     for (int c = 0; c < 3; c++) t[c] = p[c] + k2[c] * dt / 2.0f;
-    stag_vel(vel, t, g, k3);
+    U(t, k3);
 #pragma unroll
     for (int c = 0; c < 3; c++) t[c] = p[c] + k3[c] * dt;
-    stag_vel(vel, t, g, k4);
+    U(t, k4);
     x[a] = p[0] + dt / 6.0f * (((k1[0] + 2.0f * k2[0]) + 2.0f * k3[0]) + k4[0]);
     y[a] = p[1] + dt / 6.0f * (((k1[1] + 2.0f * k2[1]) + 2.0f * k3[1]) + k4[1]);
     z[a] = p[2] + dt / 6.0f * (((k1[2] + 2.0f * k2[2]) + 2.0f * k3[2]) + k4[2]);
@@ -310,7 +318,21 @@ int isf_particles_advect(isf_t h, isf_particles_t p, const float* vel) {
     if (r) return r;
     const int n = std::max(p->length, 1);
     hipLaunchKernelGGL(k_advect_active, dim3(nblocks(n)), dim3(NT), 0, s, p->x, p->y, p->z, p->active, p->count,
-                       vel ? vel : h->vel, h->g, h->dt);
+                       vel ? vel : h->vel, nullptr, nullptr, nullptr, h->g, h->dt);
+    HIP_CHECK(hipGetLastError(), "StaggeredAdvect");
+    return ISF_OK;
+}
+
+int isf_mp_staggered_advect(isf_t h, isf_particles_t p, const float* vx, const float* vy, const float* vz, float dt) {
+    hipStream_t s;
+    int r = stream_of(h, p, &s);
+    if (r) return r;
+    CHECK_PTR(vx, "vx");
+    CHECK_PTR(vy, "vy");
+    CHECK_PTR(vz, "vz");
+    const int n = std::max(p->length, 1);
+    hipLaunchKernelGGL(k_advect_active, dim3(nblocks(n)), dim3(NT), 0, s, p->x, p->y, p->z, p->active, p->count,
+                       nullptr, vx, vy, vz, h->g, dt);
     HIP_CHECK(hipGetLastError(), "StaggeredAdvect");
     return ISF_OK;
 }

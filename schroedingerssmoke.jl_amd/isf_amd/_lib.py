@@ -82,6 +82,20 @@ SIGNATURES = {
     "isf_particles_state": ([C.c_void_p, C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                              C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                              C.POINTER(C.c_void_p)], C.c_int),
+    "isf_mp_schroedinger_flow": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_mp_normalize": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_mp_velocity_one_form": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p],
+                                 C.c_int),
+    "isf_mp_staggered_sharp": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_mp_div": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_mp_poisson_solve": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_mp_gauge_transform": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_mp_pressure_project": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_mp_restrict_velocity": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, c_float3, C.c_float, C.c_int],
+                                 C.c_int),
+    "isf_mp_staggered_velocity": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "isf_mp_staggered_advect": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float], C.c_int),
     "isf_export_create": ([C.POINTER(C.c_void_p), C.c_void_p, C.c_longlong], C.c_int),
     "isf_export_destroy": ([C.c_void_p], C.c_int),
     "isf_export_push": ([C.c_void_p, C.c_void_p], C.c_int),

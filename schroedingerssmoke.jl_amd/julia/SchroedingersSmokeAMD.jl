@@ -361,4 +361,247 @@ function __init__()
     end
 end
 
+# ---------------------------------------------------------------- MatlabPort names
+"""
+    SchroedingersSmokeAMD.MatlabPort
+
+The CamelCase API of the reference's MatlabPort module (src/MatlabPort.jl) over libisf's
+`isf_mp_*` entry points, with that module's layout: psi1, psi2 separate complex arrays,
+velocity as separate vx, vy, vz.  Device arguments are `ISFArray{ComplexF32,3}` /
+`ISFArray{Float32,3}` and give device results; host `Array`s are copied to the device,
+computed in Float32 and returned as host arrays, so the driver of
+examples/matlabport/simple.jl keeps its call forms.
+"""
+module MatlabPort
+
+import ..SchroedingersSmokeAMD: libisf, check, ISFArray, devptr
+export ISF, land, SchroedingerFlow, Normalize, VelocityOneForm, StaggeredSharp, Div, PoissonSolve, GaugeTransform
+export PressureProject, StaggeredVelocity, StaggeredAdvect, Particles, Keep, removethedead!
+
+"land(a, b), src/MatlabPort.jl:5-8."
+land(a, b) = a .& b
+
+"""
+    ISF(vol_size, vol_res, hbar, dt)
+
+src/MatlabPort.jl:38-73.  Grid fields as in the reference (`sizex`.., `resx`.., `dx`..,
+`hbar`, `dt`, host `px`, `py`, `pz`); the Schroedinger mask of BuildSchroedinger (:75-83)
+lives in the device tables of the handle.
+"""
+mutable struct ISF
+    h::Ptr{Cvoid}
+    sizex::Int; sizey::Int; sizez::Int
+    resx::Int; resy::Int; resz::Int
+    dx::Float64; dy::Float64; dz::Float64
+    hbar::Float64
+    dt::Float64
+    px::Array{Float64,3}; py::Array{Float64,3}; pz::Array{Float64,3}
+    function ISF(vol_size::NTuple{3}, vol_res::NTuple{3}, hbar, dt)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        res = Cint[round(Int, v) for v in vol_res]
+        box = Cint.(collect(vol_size))
+        check(ccall((:isf_create, libisf), Cint, (Ref{Ptr{Cvoid}}, Ptr{Cint}, Ptr{Cint}, Cfloat, Cfloat),
+                    h, res, box, Float32(hbar), Float32(dt)), "isf_create")
+        d = collect(vol_size) ./ res
+        px = [(i - 1) * d[1] for i in 1:res[1], j in 1:res[2], k in 1:res[3]]
+        py = [(j - 1) * d[2] for i in 1:res[1], j in 1:res[2], k in 1:res[3]]
+        pz = [(k - 1) * d[3] for i in 1:res[1], j in 1:res[2], k in 1:res[3]]
+        obj = new(h[], vol_size..., res..., d..., Float64(hbar), Float64(dt), px, py, pz)
+        finalizer(x -> ccall((:isf_destroy, libisf), Cint, (Ptr{Cvoid},), x.h), obj)
+        CURRENT[] = obj  # Normalize and GaugeTransform take no ISF argument
+        obj
+    end
+end
+const CURRENT = Ref{Any}(nothing)
+dims(obj::ISF) = (obj.resx, obj.resy, obj.resz)
+
+# device view of an argument; host arrays are uploaded (converted to Float32 types)
+dev(a::ISFArray, ::Type{T}) where {T} = (eltype(a) === T || error("expected an ISFArray{$T}"); a)
+dev(a::AbstractArray, ::Type{T}) where {T} = ISFArray(convert(Array{T}, a))
+ondevice(args...) = all(a -> a isa ISFArray, args)
+result(ondev::Bool, outs...) = ondev ? outs : map(Array, outs)
+newc(obj) = ISFArray{ComplexF32,3}(dims(obj))
+newf(obj) = ISFArray{Float32,3}(dims(obj))
+
+"SchroedingerFlow(obj, psi1, psi2), src/MatlabPort.jl:87-93."
+function SchroedingerFlow(obj::ISF, psi1, psi2)
+    a, b = dev(psi1, ComplexF32), dev(psi2, ComplexF32)
+    o1, o2 = newc(obj), newc(obj)
+    check(ccall((:isf_mp_schroedinger_flow, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                obj.h, a.ptr, b.ptr, o1.ptr, o2.ptr), "SchroedingerFlow")
+    result(ondevice(psi1, psi2), o1, o2)
+end
+
+"Normalize(psi1, psi2), src/MatlabPort.jl:155-160 (on the ISF constructed last)."
+function Normalize(psi1, psi2)
+    obj = CURRENT[]
+    a, b = dev(psi1, ComplexF32), dev(psi2, ComplexF32)
+    o1, o2 = newc(obj), newc(obj)
+    check(ccall((:isf_mp_normalize, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                obj.h, a.ptr, b.ptr, o1.ptr, o2.ptr), "Normalize")
+    result(ondevice(psi1, psi2), o1, o2)
+end
+
+"VelocityOneForm(obj, psi1, psi2, hbar=1.0), src/MatlabPort.jl:106-126."
+function VelocityOneForm(obj::ISF, psi1, psi2, hbar = 1.0)
+    a, b = dev(psi1, ComplexF32), dev(psi2, ComplexF32)
+    vx, vy, vz = newf(obj), newf(obj), newf(obj)
+    check(ccall((:isf_mp_velocity_one_form, libisf), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cfloat, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                obj.h, a.ptr, b.ptr, Float32(hbar), vx.ptr, vy.ptr, vz.ptr), "VelocityOneForm")
+    result(ondevice(psi1, psi2), vx, vy, vz)
+end
+
+"StaggeredSharp(obj, vx, vy, vz), src/MatlabPort.jl:99-104."
+function StaggeredSharp(obj::ISF, vx, vy, vz)
+    a, b, c = dev(vx, Float32), dev(vy, Float32), dev(vz, Float32)
+    ux, uy, uz = newf(obj), newf(obj), newf(obj)
+    check(ccall((:isf_mp_staggered_sharp, libisf), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                obj.h, a.ptr, b.ptr, c.ptr, ux.ptr, uy.ptr, uz.ptr), "StaggeredSharp")
+    result(ondevice(vx, vy, vz), ux, uy, uz)
+end
+
+"Div(obj, vx, vy, vz), src/MatlabPort.jl:162-170."
+function Div(obj::ISF, vx, vy, vz)
+    a, b, c = dev(vx, Float32), dev(vy, Float32), dev(vz, Float32)
+    f = newf(obj)
+    check(ccall((:isf_mp_div, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                obj.h, a.ptr, b.ptr, c.ptr, f.ptr), "Div")
+    result(ondevice(vx, vy, vz), f)[1]
+end
+
+"PoissonSolve(obj, f), src/MatlabPort.jl:128-138 (complex result, imaginary part 0)."
+function PoissonSolve(obj::ISF, f)
+    fd = dev(f, Float32)
+    q = newc(obj)
+    check(ccall((:isf_mp_poisson_solve, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}), obj.h, fd.ptr, q.ptr),
+          "PoissonSolve")
+    result(ondevice(f), q)[1]
+end
+
+"GaugeTransform(psi1, psi2, q) = psi .* exp.(im .* q), src/MatlabPort.jl:140-145."
+function GaugeTransform(psi1, psi2, q)
+    obj = CURRENT[]
+    a, b, qd = dev(psi1, ComplexF32), dev(psi2, ComplexF32), dev(q, ComplexF32)
+    o1, o2 = newc(obj), newc(obj)
+    check(ccall((:isf_mp_gauge_transform, libisf), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                obj.h, a.ptr, b.ptr, qd.ptr, o1.ptr, o2.ptr), "GaugeTransform")
+    result(ondevice(psi1, psi2, q), o1, o2)
+end
+
+"PressureProject(obj, psi1, psi2), src/MatlabPort.jl:147-153."
+function PressureProject(obj::ISF, psi1, psi2)
+    a, b = dev(psi1, ComplexF32), dev(psi2, ComplexF32)
+    o1, o2 = newc(obj), newc(obj)
+    check(ccall((:isf_mp_pressure_project, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                obj.h, a.ptr, b.ptr, o1.ptr, o2.ptr), "PressureProject")
+    result(ondevice(psi1, psi2), o1, o2)
+end
+
+"StaggeredVelocity(px, py, pz, torus, vx, vy, vz), src/MatlabPort.jl:237-273."
+function StaggeredVelocity(px, py, pz, torus::ISF, vx, vy, vz)
+    n = length(px)
+    p = (dev(px, Float32), dev(py, Float32), dev(pz, Float32))
+    v = (dev(vx, Float32), dev(vy, Float32), dev(vz, Float32))
+    u = (ISFArray{Float32,1}((n,)), ISFArray{Float32,1}((n,)), ISFArray{Float32,1}((n,)))
+    check(ccall((:isf_mp_staggered_velocity, libisf), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Clonglong, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                 Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                torus.h, p[1].ptr, p[2].ptr, p[3].ptr, n, v[1].ptr, v[2].ptr, v[3].ptr, u[1].ptr, u[2].ptr, u[3].ptr),
+          "StaggeredVelocity")
+    result(ondevice(px, py, pz), u...)
+end
+
+"""
+    Particles(x, y, z, length, active)
+
+src/MatlabPort.jl:178-184 on the device (`isf_particles_*`): capacity `length(x)`.  The
+driver's start `Particles(zeros(Float32, n), zeros(Float32, n), zeros(Float32, n), 0, Int[])`
+(examples/matlabport/simple.jl:74-78), or the first `length` slots filled and active.
+`p.x`, `p.y`, `p.z` and `p.active` (1-based) read back host copies.
+"""
+mutable struct Particles
+    h::Ptr{Cvoid}
+    isf::ISF
+    capacity::Int
+    function Particles(x::AbstractVector, y::AbstractVector, z::AbstractVector, len::Integer, active::AbstractVector)
+        obj = CURRENT[]
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:isf_particles_create, libisf), Cint, (Ref{Ptr{Cvoid}}, Clonglong), h, length(x)),
+              "isf_particles_create")
+        p = new(h[], obj, length(x))
+        finalizer(q -> ccall((:isf_particles_destroy, libisf), Cint, (Ptr{Cvoid},), q.h), p)
+        if len > 0
+            active == 1:len || error("Particles: active must be 1:length")
+            append!(p, x[1:len], y[1:len], z[1:len])
+        end
+        p
+    end
+end
+
+function state(p::Particles)
+    len, cnt = Ref{Clonglong}(0), Ref{Clonglong}(0)
+    px, py, pz, pa = Ref{Ptr{Cvoid}}(C_NULL), Ref{Ptr{Cvoid}}(C_NULL), Ref{Ptr{Cvoid}}(C_NULL), Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:isf_particles_state, libisf), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ref{Clonglong}, Ref{Clonglong}, Ref{Ptr{Cvoid}}, Ref{Ptr{Cvoid}},
+                 Ref{Ptr{Cvoid}}, Ref{Ptr{Cvoid}}),
+                p.isf.h, p.h, len, cnt, px, py, pz, pa), "isf_particles_state")
+    (len[], cnt[], px[], py[], pz[], pa[])
+end
+function download(ptr::Ptr{Cvoid}, ::Type{T}, n::Integer) where {T}
+    h = Vector{T}(undef, n)
+    n > 0 && check(ccall((:isf_memcpy_d2h, libisf), Cint, (Ptr{T}, Ptr{Cvoid}, Csize_t), h, ptr, n * sizeof(T)),
+                   "isf_memcpy_d2h")
+    h
+end
+Base.length(p::Particles) = Int(state(p)[1])
+function Base.getproperty(p::Particles, s::Symbol)
+    s in (:h, :isf, :capacity) && return getfield(p, s)
+    st = state(p)
+    s === :length && return Int(st[1])
+    s === :x && return download(st[3], Float32, p.capacity)
+    s === :y && return download(st[4], Float32, p.capacity)
+    s === :z && return download(st[5], Float32, p.capacity)
+    s === :active && return Int.(download(st[6], Cint, st[2])) .+ 1
+    error("Particles has no field $s")
+end
+
+"append!(p, x, y, z), src/MatlabPort.jl:186-195 (refuses unless length + n < capacity)."
+function Base.append!(p::Particles, x, y, z)
+    a, b, c = dev(x, Float32), dev(y, Float32), dev(z, Float32)
+    check(ccall((:isf_particles_append, libisf), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Clonglong),
+                p.isf.h, p.h, a.ptr, b.ptr, c.ptr, length(x)), "append!")
+    check(ccall((:isf_synchronize, libisf), Cint, (Ptr{Cvoid},), p.isf.h), "isf_synchronize")
+    nothing
+end
+
+"removethedead!(p, vol_size) of examples/matlabport/simple.jl:63-70 (strictly inside the box)."
+removethedead!(p::Particles, vol_size = nothing) =
+    check(ccall((:isf_particles_remove_dead, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), p.isf.h, p.h), "removethedead!")
+
+"StaggeredAdvect(p, torus, vx, vy, vz, dt), src/MatlabPort.jl:201-222, over the active slots."
+function StaggeredAdvect(p::Particles, torus::ISF, vx, vy, vz, dt)
+    v = (dev(vx, Float32), dev(vy, Float32), dev(vz, Float32))
+    check(ccall((:isf_mp_staggered_advect, libisf), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cfloat),
+                torus.h, p.h, v[1].ptr, v[2].ptr, v[3].ptr, Float32(dt)), "StaggeredAdvect")
+    ondevice(vx, vy, vz) ||
+        check(ccall((:isf_synchronize, libisf), Cint, (Ptr{Cvoid},), torus.h), "isf_synchronize")
+    nothing
+end
+
+"Keep(particle, ind), src/MatlabPort.jl:226-230 (1-based ind)."
+function Keep(p::Particles, ind)
+    d = ISFArray(Cint.(collect(ind)) .- Cint(1))
+    check(ccall((:isf_particles_keep, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Clonglong), p.isf.h, p.h,
+                d.ptr, length(d)), "Keep")
+    check(ccall((:isf_synchronize, libisf), Cint, (Ptr{Cvoid},), p.isf.h), "isf_synchronize")
+    nothing
+end
+
+end # module MatlabPort
+
 end # module

@@ -62,3 +62,9 @@ def test_python_mirror_exposes_reference_names():
     for name in ("ISF", "schroedinger_flow_", "normalize_psi_", "pressure_project_", "velocity_one_form_",
                  "isjet", "restrict_velocity_", "add_particles_", "staggered_advect_"):
         assert hasattr(isf_amd, name)
+    # the MatlabPort names (src/MatlabPort.jl:38-273, examples/matlabport/simple.jl:6-7)
+    from isf_amd import matlabport
+    for name in ("ISF", "land", "SchroedingerFlow", "Normalize", "VelocityOneForm", "StaggeredSharp", "Div",
+                 "PoissonSolve", "GaugeTransform", "PressureProject", "StaggeredVelocity", "StaggeredAdvect",
+                 "Particles", "Keep"):
+        assert callable(getattr(matlabport, name))
