@@ -348,14 +348,43 @@ __device__ __forceinline__ void stag_eval(const float* __restrict__ vel, const f
                w1[a] * ((1.0f - w2[a]) * e.pp2[a] + w2[a] * e.vn[a]);
 }
 
+// Workgroup-local accumulation of the velocity cotangent: four LDS windows of ADJ_W
+// cells (3 components each) starting at cells cmin + {0, nx, nx ny, nx + nx ny}, cmin =
+// the lowest base cell among the workgroup's evaluations.  Every target of an evaluation
+// at base cell c is c + {0, 1, nx, nx + 1, nx ny, nx ny + 1, nx + nx ny} away from the
+// periodic wrap, so with cell-sorted slots almost all targets fall in a window; the rest
+// (and wrapped ones) go straight to global atomics.  The windows are added to velbar once
+// per workgroup, one atomic per touched (cell, component).
+constexpr int ADJ_W = 256;
+typedef __attribute__((address_space(3))) float lds_float;  // an LDS address (ds_* instructions)
+struct AdjAcc {
+    lds_float* lds;      // [4][ADJ_W][3]
+    long long cmin;      // first cell of window 0 (workgroup-uniform)
+    long long sy, sz;    // window w starts at cmin + (w & 1) sy + (w >> 1) sz
+    float* velbar;
+    __device__ __forceinline__ long long base(int w) const { return cmin + (w & 1) * sy + (w >> 1) * sz; }
+    __device__ __forceinline__ void add(size_t t, int a, float v) const {
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            const long long d = (long long)t - base(w);
+            if (d >= 0 && d < ADJ_W) {
+                __hip_atomic_fetch_add(lds + (w * ADJ_W + (int)d) * 3 + a, v, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                return;
+            }
+        }
+        unsafeAtomicAdd(velbar + 3 * t + a, v);
+    }
+};
+
 // The 12 (cell, component) targets of one staggered evaluation are fixed by its base cell
 // e.i (the others are its +1 neighbours, periodic).  Lanes of a wavefront that share the
 // base cell in a contiguous run of lanes sum their 12 contributions by a segmented
-// inclusive scan (offsets 1..32); the last lane of each run issues the 12 atomics.  Lanes
-// in any order give the right sum; cell-sorted slots give long runs and few atomics.
+// inclusive scan (offsets 1..32); the last lane of each run adds them to the workgroup's
+// accumulator.  Lanes in any order give the right sum; cell-sorted slots give long runs.
 // Every lane of the wavefront calls this (act = false: nothing to add).
-__device__ __forceinline__ void wave_scatter12(float* __restrict__ velbar, const StagEval& e, const GridInfo& g,
-                                               float val[12], bool act) {
+__device__ __forceinline__ void wave_scatter12(const AdjAcc& acc, const StagEval& e, const GridInfo& g, float val[12],
+                                               bool act) {
     const int lane = (int)(threadIdx.x & 63u);
     const unsigned key = act ? (unsigned)cidx(e.i[0], e.i[1], e.i[2], g) : 0xFFFFFFFFu;
     const unsigned kprev = __shfl_up(key, 1u, 64);
@@ -384,16 +413,16 @@ __device__ __forceinline__ void wave_scatter12(float* __restrict__ velbar, const
                              {cidx(ip[0], i[1], i[2], g), cidx(i[0], ip[1], i[2], g), cidx(ip[0], ip[1], i[2], g)}};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        unsafeAtomicAdd(velbar + 3 * c000 + a, val[4 * a]);
-        unsafeAtomicAdd(velbar + 3 * cv[a][0] + a, val[4 * a + 1]);
-        unsafeAtomicAdd(velbar + 3 * cv[a][1] + a, val[4 * a + 2]);
-        unsafeAtomicAdd(velbar + 3 * cv[a][2] + a, val[4 * a + 3]);
+        acc.add(c000, a, val[4 * a]);
+        acc.add(cv[a][0], a, val[4 * a + 1]);
+        acc.add(cv[a][1], a, val[4 * a + 2]);
+        acc.add(cv[a][2], a, val[4 * a + 3]);
     }
 }
 
-// ubar at the evaluation point: position cotangent xbar (3) and scatter into velbar
-// (every lane of the wavefront calls this, act marks real work).
-__device__ __forceinline__ void stag_adj(const StagEval& e, const float ub[3], const GridInfo& g, float* __restrict__ velbar,
+// ubar at the evaluation point: position cotangent xbar (3) and scatter into the velocity
+// cotangent (every lane of the wavefront calls this, act marks real work).
+__device__ __forceinline__ void stag_adj(const StagEval& e, const float ub[3], const GridInfo& g, const AdjAcc& acc,
                                          float xb[3], bool act) {
     const float w1[3] = {e.w[2], e.w[2], e.w[1]}, w2[3] = {e.w[1], e.w[0], e.w[0]};
     float dw1[3], dw2[3];
@@ -415,7 +444,7 @@ __device__ __forceinline__ void stag_adj(const StagEval& e, const float ub[3], c
         val[4 * a + 2] = u * w1[a] * (1.0f - w2[a]);
         val[4 * a + 3] = u * w1[a] * w2[a];
     }
-    wave_scatter12(velbar, e, g, val, act);
+    wave_scatter12(acc, e, g, val, act);
 }
 
 // Adjoint of staggered_advect at the slot positions before the step (parts0): pbar (the
@@ -424,8 +453,13 @@ __device__ __forceinline__ void stag_adj(const StagEval& e, const float ub[3], c
 __global__ void __launch_bounds__(256)
 k_advect_adj(const float* __restrict__ parts0, long long nslots, const int* __restrict__ order,
              const float* __restrict__ vel, GridInfo g, float dt, float* __restrict__ pbar, float* __restrict__ velbar) {
-    // every lane runs to the end (the scatter works across the wavefront); lanes past the
-    // last slot or on a dead slot do the arithmetic on a stand-in position and add nothing
+    // every lane runs to the end (the scatter works across the wavefront and the
+    // workgroup); lanes past the last slot or on a dead slot do the arithmetic on a
+    // stand-in position and add nothing
+    __shared__ float win[4 * ADJ_W * 3];
+    __shared__ unsigned cmin_s;
+    for (int i = threadIdx.x; i < 4 * ADJ_W * 3; i += blockDim.x) win[i] = 0.0f;
+    if (threadIdx.x == 0) cmin_s = 0xFFFFFFFFu;
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool slot = k < nslots;
     const long long s = slot ? (order ? (long long)order[k] : k) : 0;
@@ -441,57 +475,84 @@ k_advect_adj(const float* __restrict__ parts0, long long nslots, const int* __re
         p[1] = 0.5f * g.Ly;
         p[2] = 0.5f * g.Lz;
     }
-    StagEval e1, e2, e3, e4;
-    float k1[3], k2[3], k3[3], k4[3], t[3];
-    stag_eval(vel, p, g, e1, k1);
+    // forward RK4 evaluation points P[0..3] (staggered_advect, src/parallel.jl:201-212);
+    // the reverse sweep re-gathers each stage's cells, so only the points stay live
+    float P[4][3];
+    {
+        StagEval e;
+        float kk[3];
 #pragma unroll
-    for (int a = 0; a < 3; a++) t[a] = p[a] + k1[a] * dt * 0.5f;
-    stag_eval(vel, t, g, e2, k2);
+        for (int a = 0; a < 3; a++) P[0][a] = p[a];
+        stag_eval(vel, P[0], g, e, kk);
 #pragma unroll
-    for (int a = 0; a < 3; a++) t[a] = p[a] + k2[a] * dt * 0.5f;
-    stag_eval(vel, t, g, e3, k3);
+        for (int a = 0; a < 3; a++) P[1][a] = p[a] + kk[a] * dt * 0.5f;
+        stag_eval(vel, P[1], g, e, kk);
 #pragma unroll
-    for (int a = 0; a < 3; a++) t[a] = p[a] + k3[a] * dt;
-    stag_eval(vel, t, g, e4, k4);
+        for (int a = 0; a < 3; a++) P[2][a] = p[a] + kk[a] * dt * 0.5f;
+        stag_eval(vel, P[2], g, e, kk);
+#pragma unroll
+        for (int a = 0; a < 3; a++) P[3][a] = p[a] + kk[a] * dt;
+    }
+    // lowest base cell of the workgroup's evaluations
+    unsigned m = 0xFFFFFFFFu;
+    if (in) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int i[3], ip[3];
+            float w[3];
+            stag_index(P[q], g, i, ip, w);
+            m = min(m, (unsigned)cidx(i[0], i[1], i[2], g));
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = min(m, (unsigned)__shfl_xor((int)m, off, 64));
+    __syncthreads();  // zeroed windows and cmin_s
+    if ((threadIdx.x & 63u) == 0) atomicMin(&cmin_s, m);
+    __syncthreads();
+    AdjAcc acc;
+    acc.lds = (lds_float*)win;
+    acc.velbar = velbar;
+    acc.cmin = (long long)__builtin_amdgcn_readfirstlane(cmin_s);
+    acc.sy = g.nx;
+    acc.sz = (long long)g.nx * g.ny;
     float pb[3] = {0.0f, 0.0f, 0.0f};
     if (in) {
         pb[0] = pbar[3 * s];
         pb[1] = pbar[3 * s + 1];
         pb[2] = pbar[3 * s + 2];
     }
+    // reverse sweep: kb[q] = cotangent of k_{q+1}; xb = cotangent of the point P[q]
     const float c6 = dt / 6.0f;
-    float acc[3], kb1[3], kb2[3], kb3[3], kb4[3], xb[3];
+    const float wk[4] = {c6, 2.0f * c6, 2.0f * c6, c6};  // weights of k1..k4 in the update
+    const float wp[4] = {0.0f, 0.5f * dt, 0.5f * dt, dt}; // P[q] = p + wp[q] k_q
+    float pacc[3], kb[3], xb[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        acc[a] = pb[a];
-        kb1[a] = c6 * pb[a];
-        kb2[a] = 2.0f * c6 * pb[a];
-        kb3[a] = 2.0f * c6 * pb[a];
-        kb4[a] = c6 * pb[a];
-    }
-    stag_adj(e4, kb4, g, velbar, xb, in);
+    for (int a = 0; a < 3; a++) pacc[a] = pb[a];
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        acc[a] += xb[a];
-        kb3[a] += dt * xb[a];
-    }
-    stag_adj(e3, kb3, g, velbar, xb, in);
+    for (int q = 3; q >= 0; q--) {
+        // cotangent of k_{q+1}: its weight in the update plus what P[q+1] passed back
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        acc[a] += xb[a];
-        kb2[a] += dt * 0.5f * xb[a];
-    }
-    stag_adj(e2, kb2, g, velbar, xb, in);
+        for (int a = 0; a < 3; a++) kb[a] = wk[q] * pb[a] + (q < 3 ? wp[q + 1] * xb[a] : 0.0f);
+        StagEval e;
+        float u[3];
+        stag_eval(vel, P[q], g, e, u);
+        stag_adj(e, kb, g, acc, xb, in);
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        acc[a] += xb[a];
-        kb1[a] += dt * 0.5f * xb[a];
+        for (int a = 0; a < 3; a++) pacc[a] += xb[a];
     }
-    stag_adj(e1, kb1, g, velbar, xb, in);
     if (slot) {
         // the dead marker does not depend on anything
 #pragma unroll
-        for (int a = 0; a < 3; a++) pbar[3 * s + a] = in ? acc[a] + xb[a] : 0.0f;
+        for (int a = 0; a < 3; a++) pbar[3 * s + a] = in ? pacc[a] : 0.0f;
+    }
+    // add the windows to velbar (entries never touched are zero and skipped)
+    __syncthreads();
+    for (int i = threadIdx.x; i < 4 * ADJ_W * 3; i += blockDim.x) {
+        const float v = win[i];
+        if (v != 0.0f) {
+            const int w = i / (ADJ_W * 3), r = i - w * (ADJ_W * 3);
+            unsafeAtomicAdd(velbar + 3 * (acc.base(w) + r / 3) + r % 3, v);
+        }
     }
 }
 

@@ -124,9 +124,11 @@ int launch_bump_counter(int* counter, hipStream_t s);
 size_t order_scratch_bytes(const GridInfo& g, long long nslots);
 int launch_particle_order(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
                           size_t scratch_bytes, hipStream_t s);
-// the same with single-cell buckets (the adjoint's scatter order)
-int launch_particle_order_cells(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
-                                size_t scratch_bytes, hipStream_t s);
+// slots grouped by exact cell (counting sort; order inside a cell arbitrary), for the
+// scatter of the advection adjoint
+size_t cell_order_scratch_bytes(const GridInfo& g, long long nslots);
+int launch_cell_order(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
+                      size_t scratch_bytes, hipStream_t s);
 
 // ---- adjoint of the step and the differentiable rollout (isf_adjoint.hip)
 // Particle-density grid: cells of size h over the box L, Gaussian width sigma truncated

@@ -192,7 +192,7 @@ static int step_adjoint(isf_rollout_t r, int s, int k) {
     LAUNCH_CHECK(launch_fill_f32(r->velbar, 0.0f, 3 * r->N, st), "fill");
     if (r->P > 0) {
         // cell-sorted slot order: long same-cell runs for the scatter's wavefront reduction
-        LAUNCH_CHECK(launch_particle_order_cells(g, r->t_p0[k], r->P, r->adj_order, r->adj_order_scratch,
+        LAUNCH_CHECK(launch_cell_order(g, r->t_p0[k], r->P, r->adj_order, r->adj_order_scratch,
                                                  r->adj_order_bytes, st),
                      "particle order");
         LAUNCH_CHECK(launch_advect_adj(g, r->t_p0[k], r->P, r->adj_order, r->vel, h->dt, r->pbar, r->velbar, st),
@@ -325,7 +325,7 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
     RA(&r->vacc, 4 * (size_t)(r->T + r->C));
     if (const size_t sp = splat_partial_doubles(r->sc)) RA(&r->splat_part, sp);
     if (r->P > 0) {
-        r->adj_order_bytes = order_scratch_bytes(r->g, r->P);
+        r->adj_order_bytes = cell_order_scratch_bytes(r->g, r->P);
         RA(&r->adj_order, (size_t)r->P);
         RA(&r->adj_order_scratch, r->adj_order_bytes);
     }
@@ -615,7 +615,7 @@ int isf_vjp_staggered_advect(isf_t h, const float* parts0, long long nslots, con
         h->vjp_order = nullptr;
         h->vjp_order_scratch = nullptr;
         h->vjp_order_cap = 0;
-        const size_t sb = order_scratch_bytes(h->g, nslots);
+        const size_t sb = cell_order_scratch_bytes(h->g, nslots);
         hipError_t he = hipMalloc(&h->vjp_order, sizeof(int) * (size_t)nslots);
         if (he == hipSuccess) he = hipMalloc(&h->vjp_order_scratch, sb);
         if (he != hipSuccess) return fail(ISF_ENOMEM, "advection VJP order: %s", hipGetErrorString(he));
@@ -623,7 +623,7 @@ int isf_vjp_staggered_advect(isf_t h, const float* parts0, long long nslots, con
         h->vjp_order_cap = nslots;
     }
     // cell-sorted slot order: long same-cell runs for the scatter's wavefront reduction
-    LAUNCH_CHECK(launch_particle_order_cells(h->g, parts0, nslots, h->vjp_order, h->vjp_order_scratch,
+    LAUNCH_CHECK(launch_cell_order(h->g, parts0, nslots, h->vjp_order, h->vjp_order_scratch,
                                              h->vjp_order_bytes, h->stream),
                  "particle order");
     LAUNCH_CHECK(launch_advect_adj(h->g, parts0, nslots, h->vjp_order, vel ? vel : h->vel, h->dt, pbar, velbar,

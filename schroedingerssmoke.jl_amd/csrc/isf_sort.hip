@@ -1,24 +1,28 @@
-// isf_sort.hip -- spatial processing order of the particle slots for the advection.
+// isf_sort.hip -- spatial processing orders of the particle slots.
 //
-// Slots are sorted by bucket = (ix / 8) + (nx / 8) * (iy + ny * iz) of the cell holding
-// the particle (slots outside the box last), with rocPRIM's stable LSD radix sort of
-// (bucket, slot) pairs.  A wavefront then advects particles from the same few cells and
-// its velocity gathers share cache lines.  Each slot is advanced independently, so the
-// order never changes a result.
+// Advection: slots are sorted by bucket = (ix / 8) + (nx / 8) * (iy + ny * iz) of the
+// cell holding the particle (slots outside the box last), with rocPRIM's stable LSD radix
+// sort of (bucket, slot) pairs.  A wavefront then advects particles from the same few
+// cells and its velocity gathers share cache lines.  Each slot is advanced independently,
+// so the order never changes a result.
+//
+// Advection adjoint: slots grouped by their exact cell with a counting sort (count per
+// cell, exclusive scan, scatter; order inside a cell arbitrary), so the velocity scatter
+// of a wavefront and of a workgroup hits few cells.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "isf_internal.hpp"
 #include "isf_philox.hpp"
 
 namespace isf {
 
-// xs: log2 of the x cells per bucket (3: buckets of 8 cells for the forward advection,
-// 0: single cells for the adjoint's scatter)
+// xs: log2 of the x cells per bucket (3: buckets of 8 cells; 0: single cells)
 static int bucket_count(const GridInfo& g, int xs) { return (g.nx >> xs) * g.ny * g.nz; }
 
 static unsigned key_bits(const GridInfo& g, int xs) {
@@ -27,20 +31,46 @@ static unsigned key_bits(const GridInfo& g, int xs) {
     return b;
 }
 
-__global__ void __launch_bounds__(256)
-k_bucket_keys(const float* __restrict__ parts, long long nslots, GridInfo g, int nb, int xs, unsigned* __restrict__ keys,
-              int* __restrict__ slots) {
-    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    const float x = parts[3 * s], y = parts[3 * s + 1], z = parts[3 * s + 2];
+// bucket of a particle: cell (x / 2^xs, y, z) strictly inside the box, else nb
+__device__ __forceinline__ unsigned particle_bucket(float x, float y, float z, const GridInfo& g, int nb, int xs) {
     unsigned key = (unsigned)nb;
     if ((x > 0.0f) & (x < g.Lx) & (y > 0.0f) & (y < g.Ly) & (z > 0.0f) & (z < g.Lz)) {
         const int ix = min((int)(x * g.idx), g.nx - 1), iy = min((int)(y * g.idy), g.ny - 1);
         const int iz = min((int)(z * g.idz), g.nz - 1);
         key = (unsigned)((ix >> xs) + (g.nx >> xs) * (iy + g.ny * iz));
     }
-    keys[s] = key;
+    return key;
+}
+
+__global__ void __launch_bounds__(256)
+k_bucket_keys(const float* __restrict__ parts, long long nslots, GridInfo g, int nb, int xs, unsigned* __restrict__ keys,
+              int* __restrict__ slots) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    keys[s] = particle_bucket(parts[3 * s], parts[3 * s + 1], parts[3 * s + 2], g, nb, xs);
     slots[s] = (int)s;
+}
+
+// counting sort by cell: rank of each slot inside its cell, then the scatter
+__global__ void __launch_bounds__(256) k_fill_u32(unsigned* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+__global__ void __launch_bounds__(256)
+k_cell_rank(const float* __restrict__ parts, long long nslots, GridInfo g, int nb, unsigned* __restrict__ count,
+            unsigned* __restrict__ key, unsigned* __restrict__ rank) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const unsigned k = particle_bucket(parts[3 * s], parts[3 * s + 1], parts[3 * s + 2], g, nb, 0);
+    key[s] = k;
+    rank[s] = atomicAdd(count + k, 1u);
+}
+__global__ void __launch_bounds__(256)
+k_cell_scatter(long long nslots, const unsigned* __restrict__ key, const unsigned* __restrict__ rank,
+               const unsigned* __restrict__ start, int* __restrict__ order) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    order[start[key[s]] + rank[s]] = (int)s;
 }
 
 static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
@@ -52,10 +82,47 @@ static size_t rocprim_bytes(const GridInfo& g, long long n, int xs) {
     return bytes;
 }
 
-// scratch for either key width
 size_t order_scratch_bytes(const GridInfo& g, long long nslots) {
-    return 3 * align256(sizeof(int) * (size_t)nslots) +
-           align256(std::max(rocprim_bytes(g, nslots, 3), rocprim_bytes(g, nslots, 0)));
+    return 3 * align256(sizeof(int) * (size_t)nslots) + align256(rocprim_bytes(g, nslots, 3));
+}
+
+static size_t scan_bytes(size_t n) {
+    size_t bytes = 0;
+    (void)rocprim::exclusive_scan(nullptr, bytes, (unsigned*)nullptr, (unsigned*)nullptr, 0u, n,
+                                  rocprim::plus<unsigned>());
+    return bytes;
+}
+
+// counts and starts (cells + 1 each), keys and ranks (nslots each), scan temporary
+size_t cell_order_scratch_bytes(const GridInfo& g, long long nslots) {
+    const size_t nb = (size_t)bucket_count(g, 0) + 1;
+    return 2 * align256(sizeof(unsigned) * nb) + 2 * align256(sizeof(unsigned) * (size_t)nslots) +
+           align256(scan_bytes(nb));
+}
+
+int launch_cell_order(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
+                      size_t scratch_bytes, hipStream_t s) {
+    if (nslots <= 0) return 0;
+    if (scratch_bytes < cell_order_scratch_bytes(g, nslots)) return (int)hipErrorInvalidValue;
+    const int nbk = bucket_count(g, 0);
+    const size_t nb = (size_t)nbk + 1, ab = align256(sizeof(unsigned) * nb),
+                 an = align256(sizeof(unsigned) * (size_t)nslots);
+    char* p = static_cast<char*>(scratch);
+    unsigned* count = reinterpret_cast<unsigned*>(p);
+    unsigned* start = reinterpret_cast<unsigned*>(p + ab);
+    unsigned* key = reinterpret_cast<unsigned*>(p + 2 * ab);
+    unsigned* rank = reinterpret_cast<unsigned*>(p + 2 * ab + an);
+    void* tmp = p + 2 * ab + 2 * an;
+    size_t tmp_bytes = scratch_bytes - (2 * ab + 2 * an);
+    const dim3 gs((unsigned)((nslots + 255) / 256));
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, count, nb);
+    hipLaunchKernelGGL(k_cell_rank, gs, dim3(256), 0, s, parts, nslots, g, nbk, count, key, rank);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    e = rocprim::exclusive_scan(tmp, tmp_bytes, count, start, 0u, nb, rocprim::plus<unsigned>(), s);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(k_cell_scatter, gs, dim3(256), 0, s, nslots, key, rank, start, order);
+    return (int)hipGetLastError();
 }
 
 static int particle_order(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
@@ -82,10 +149,6 @@ static int particle_order(const GridInfo& g, const float* parts, long long nslot
 int launch_particle_order(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
                           size_t scratch_bytes, hipStream_t s) {
     return particle_order(g, parts, nslots, order, scratch, scratch_bytes, s, 3);
-}
-int launch_particle_order_cells(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
-                                size_t scratch_bytes, hipStream_t s) {
-    return particle_order(g, parts, nslots, order, scratch, scratch_bytes, s, 0);
 }
 
 // ---- initial particle cloud uniform in the nozzle cylinder, stored in a spatially
