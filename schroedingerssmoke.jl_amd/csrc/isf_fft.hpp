@@ -136,6 +136,26 @@ struct LayStrided {
     __device__ __forceinline__ void st(int e, float2 v) const { p[e * s] = v; }
 };
 
+// Global-to-LDS staging by NT threads (thread tid < NT): element i = tid + k NT,
+// k < ceil(COUNT / NT), is read with ld(i) and written with st(i, value).  All of a
+// thread's loads are issued before its first store, so their latencies overlap (a loop
+// that stores each value as it arrives waits for every load in turn).
+template <int COUNT, int NT, class Ld, class St>
+__device__ __forceinline__ void stage(int tid, const Ld& ld, const St& st) {
+    constexpr int K = (COUNT + NT - 1) / NT;
+    decltype(ld(0)) v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT;
+        if (COUNT % NT == 0 || i < COUNT) v[k] = ld(i);
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT;
+        if (COUNT % NT == 0 || i < COUNT) st(i, v[k]);
+    }
+}
+
 // ------------------------------------------------------------------ twiddle tables
 // Pass (N, R, NS) needs w(kk, r) = exp(-2 pi i r kk / (NS R)) for kk < NS, r < R.  It is
 // kept in LDS as a [kk][r] table with row pitch R + 1: lanes of one wavefront read
@@ -144,12 +164,18 @@ struct LayStrided {
 template <int N, int R, int NS>
 struct PassTw {
     static constexpr int SIZE = NS > 1 ? NS * (R + 1) : 0;
-    // fill from the global table of N-th roots (tw_n = tw_all + N); caller synchronises
-    __device__ __forceinline__ static void fill(float2* t, const float2* __restrict__ tw_n, int tid, int nt) {
-        for (int i = tid; i < SIZE; i += nt) {
-            const int kk = i / (R + 1), r = i % (R + 1);
-            t[i] = r < R ? tw_n[(r * kk * (N / (NS * R))) % N] : make_float2(1.0f, 0.0f);
-        }
+    // fill from the global table of N-th roots (tw_n = tw_all + N) by NT threads; caller
+    // synchronises
+    template <int NT>
+    __device__ __forceinline__ static void fill(float2* t, const float2* __restrict__ tw_n, int tid) {
+        if constexpr (SIZE > 0)
+            stage<SIZE, NT>(
+                tid,
+                [&](int i) {
+                    const int kk = i / (R + 1), r = i % (R + 1);
+                    return r < R ? tw_n[(r * kk * (N / (NS * R))) % N] : make_float2(1.0f, 0.0f);
+                },
+                [&](int i, float2 v) { t[i] = v; });
     }
     __device__ __forceinline__ static float2 get(const float2* t, int kk, int r) { return t[kk * (R + 1) + r]; }
 };
@@ -199,9 +225,10 @@ struct Sched {
     static constexpr int REM = N / NS;
     static constexpr int R = E < REM ? E : REM;
     static constexpr int TW = PassTw<N, R, NS>::SIZE + Sched<N, E, NS * R>::TW;
-    __device__ __forceinline__ static void fill(float2* t, const float2* __restrict__ tw_n, int tid, int nt) {
-        PassTw<N, R, NS>::fill(t, tw_n, tid, nt);
-        Sched<N, E, NS * R>::fill(t + PassTw<N, R, NS>::SIZE, tw_n, tid, nt);
+    template <int NT>
+    __device__ __forceinline__ static void fill(float2* t, const float2* __restrict__ tw_n, int tid) {
+        PassTw<N, R, NS>::template fill<NT>(t, tw_n, tid);
+        Sched<N, E, NS * R>::template fill<NT>(t + PassTw<N, R, NS>::SIZE, tw_n, tid);
     }
     template <class Lay, int DIR>
     __device__ __forceinline__ static void run(const Lay& lay, int t, bool active, const float2* tw) {
@@ -212,7 +239,8 @@ struct Sched {
 template <int N, int E>
 struct Sched<N, E, N> {
     static constexpr int TW = 0;
-    __device__ __forceinline__ static void fill(float2*, const float2* __restrict__, int, int) {}
+    template <int NT>
+    __device__ __forceinline__ static void fill(float2*, const float2* __restrict__, int) {}
     template <class Lay, int DIR>
     __device__ __forceinline__ static void run(const Lay&, int, bool, const float2*) {}
 };
@@ -221,9 +249,11 @@ struct Sched<N, E, N> {
 template <int N, int E>
 struct FftTw {
     static constexpr int SIZE = Sched<N, E, 1>::TW > 0 ? Sched<N, E, 1>::TW : 1;
-    // tw_all: combined table of roots (length-N roots at tw_all + N); caller synchronises
-    __device__ __forceinline__ static void fill(float2* t, const float2* __restrict__ tw_all, int tid, int nt) {
-        Sched<N, E, 1>::fill(t, tw_all + N, tid, nt);
+    // tw_all: combined table of roots (length-N roots at tw_all + N), NT threads; caller
+    // synchronises
+    template <int NT>
+    __device__ __forceinline__ static void fill(float2* t, const float2* __restrict__ tw_all, int tid) {
+        Sched<N, E, 1>::template fill<NT>(t, tw_all + N, tid);
     }
 };
 
@@ -237,7 +267,8 @@ __device__ __forceinline__ void fft_line(const Lay& lay, int t, bool active, con
 // Copy the length-N twiddle table (at tw_all + N) into LDS; the caller synchronises.
 template <int N, int NT>
 __device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restrict__ tw_all) {
-    for (int i = threadIdx.x; i < N; i += NT) dst[i] = tw_all[N + i];
+    stage<N, NT>(
+        (int)threadIdx.x, [&](int i) { return tw_all[N + i]; }, [&](int i, float2 v) { dst[i] = v; });
 }
 
 // ------------------------------------------------------------------ register passes

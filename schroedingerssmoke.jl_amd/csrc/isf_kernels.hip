@@ -52,22 +52,30 @@ k_flow_x_fwd(const float4* __restrict__ psi, const float2* __restrict__ sa, cons
     __shared__ float2 twl[FftTw<NX, E>::SIZE];
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * TY, z = blockIdx.y;
-    FftTw<NX, E>::fill(twl, tw, tid, NT);
     const size_t line0 = ((size_t)z * ny + y0) * NX;
+    // the block's lines: all loads issued before the twiddle-table loads and the LDS stores
+    constexpr int K = TY * NX / NT;
+    float2 va[K], vb[K];
     if (sa) {
-        for (int i = tid; i < TY * NX; i += NT) {
-            const int l = i / NX, x = i % NX;
-            lds[l * LS + pad16(x)] = sa[line0 + i];
-            lds[(TY + l) * LS + pad16(x)] = sb[line0 + i];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            va[k] = sa[line0 + tid + k * NT];
+            vb[k] = sb[line0 + tid + k * NT];
         }
     } else {
-        const float4* src = psi + line0;
-        for (int i = tid; i < TY * NX; i += NT) {
-            const int l = i / NX, x = i % NX;
-            const float4 a = src[i];
-            lds[l * LS + pad16(x)] = make_float2(a.x, a.y);
-            lds[(TY + l) * LS + pad16(x)] = make_float2(a.z, a.w);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const float4 a = psi[line0 + tid + k * NT];
+            va[k] = make_float2(a.x, a.y);
+            vb[k] = make_float2(a.z, a.w);
         }
+    }
+    FftTw<NX, E>::template fill<NT>(twl, tw, tid);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT, l = i / NX, x = i % NX;
+        lds[l * LS + pad16(x)] = va[k];
+        lds[(TY + l) * LS + pad16(x)] = vb[k];
     }
     __syncthreads();
     const int line = tid / T;
@@ -97,10 +105,18 @@ k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, float2* __r
     __shared__ float2 twl[FftTw<NX, E>::SIZE];
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * TY, z = blockIdx.y;
-    FftTw<NX, E>::fill(twl, tw, tid, NT);
-    for (int i = tid; i < 2 * NX * TY; i += NT) {
-        const int ly = i % TY, kx = (i / TY) % NX, c = i / (TY * NX);
-        lds[(c * TY + ly) * LS + pad16(kx)] = S[(((size_t)c * NX + kx) * nz + z) * ny + y0 + ly];
+    constexpr int K = 2 * NX * TY / NT;
+    float2 v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT, ly = i % TY, kx = (i / TY) % NX, c = i / (TY * NX);
+        v[k] = S[(((size_t)c * NX + kx) * nz + z) * ny + y0 + ly];
+    }
+    FftTw<NX, E>::template fill<NT>(twl, tw, tid);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT, ly = i % TY, kx = (i / TY) % NX, c = i / (TY * NX);
+        lds[(c * TY + ly) * LS + pad16(kx)] = v[k];
     }
     __syncthreads();
     const int line = tid / T;
@@ -187,25 +203,27 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
     __shared__ float2 twz[TZ2::SIZE + TZ2I::SIZE + 1];
     __shared__ double mtab[NY + NZ];  // FLOW: float2 mask_y, mask_z ; POISSON: s2y, s2z
     const int tid = threadIdx.x;
-    TY2::fill(twy, tb.tw + NY, tid, NT);
-    TY2I::fill(twy + TY2::SIZE, tb.tw + NY, tid, NT);
-    TZ2::fill(twz, tb.tw + NZ, tid, NT);
-    TZ2I::fill(twz + TZ2::SIZE, tb.tw + NZ, tid, NT);
+    const int zl = tid / C::TY, ty = tid % C::TY;  // y transforms: line z = zl
+    using Y1 = RPass<NY, 16, 1, E>;
+    float2 v[E], pf[C::PREFETCH ? E : 1];
+    // the first plane's load is issued before the table loads
+    if (blockIdx.x < nplanes)
+        Y1::load(C::PREFETCH ? pf : v, LayGlobal{base + (size_t)blockIdx.x * NY * NZ + zl * NY}, ty);
+    TY2::template fill<NT>(twy, tb.tw + NY, tid);
+    TY2I::template fill<NT>(twy + TY2::SIZE, tb.tw + NY, tid);
+    TZ2::template fill<NT>(twz, tb.tw + NZ, tid);
+    TZ2I::template fill<NT>(twz + TZ2::SIZE, tb.tw + NZ, tid);
     float2* mt2 = reinterpret_cast<float2*>(mtab);
     for (int i = tid; i < NY + NZ; i += NT) {
         if (MODE != PLANE_POISSON) mt2[i] = i < NY ? tb.mask_y[i] : tb.mask_z[i - NY];
         else mtab[i] = i < NY ? tb.s2y[i] : tb.s2z[i - NY];
     }
-    const int zl = tid / C::TY, ty = tid % C::TY;  // y transforms: line z = zl
     const int yl = tid % NY, tz = tid / NY;        // z transforms: line y = yl
     const LayContig ly{lds + zl * LSY};
     const LayStrided lz{lds + pad16(yl), LSY};
-    using Y1 = RPass<NY, 16, 1, E>;
     using Z1 = RPass<NZ, 16, 1, E>;
 
-    float2 v[E], pf[C::PREFETCH ? E : 1];
     int plane = blockIdx.x;
-    if (C::PREFETCH && plane < nplanes) Y1::load(pf, LayGlobal{base + (size_t)plane * NY * NZ + zl * NY}, ty);
 #pragma unroll 1
     for (; plane < nplanes; plane += gridDim.x) {
         float2* P = base + (size_t)plane * NY * NZ;
@@ -214,7 +232,7 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
             for (int e = 0; e < E; e++) v[e] = pf[e];
             const int next = plane + gridDim.x;
             if (next < nplanes) Y1::load(pf, LayGlobal{base + (size_t)next * NY * NZ + zl * NY}, ty);
-        } else {
+        } else if (plane != (int)blockIdx.x) {
             Y1::load(v, LayGlobal{P + zl * NY}, ty);
         }
 
@@ -325,10 +343,18 @@ k_r2c_x(const float* __restrict__ f, float2* __restrict__ P, int ny, int nz, con
     __shared__ float2 twn[NX];
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * TY, z = blockIdx.y;
-    FftTw<M, E>::fill(twm, tw, tid, NT);
-    load_twiddles<NX, NT>(twn, tw);
     const float2* src = reinterpret_cast<const float2*>(f + ((size_t)z * ny + y0) * NX);
-    for (int i = tid; i < TY * M; i += NT) lds[(i / M) * LS + pad16(i % M)] = src[i];
+    constexpr int K = TY * M / NT;
+    float2 v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = src[tid + k * NT];  // issued before the table loads
+    FftTw<M, E>::template fill<NT>(twm, tw, tid);
+    load_twiddles<NX, NT>(twn, tw);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT;
+        lds[(i / M) * LS + pad16(i % M)] = v[k];
+    }
     __syncthreads();
     fft_line<M, E, -1>(LayContig{lds + (tid / T) * LS}, tid % T, true, twm);
     for (int i = tid; i < (M + 1) * TY; i += NT) {
@@ -364,8 +390,9 @@ k_line_y(float2* __restrict__ base, int nz, const float2* __restrict__ tw) {
     __shared__ float2 twl[FftTw<NY, C::E>::SIZE];
     const int tid = threadIdx.x;
     float2* rows = base + ((size_t)blockIdx.y * nz + (size_t)blockIdx.x * C::R) * NY;
-    FftTw<NY, C::E>::fill(twl, tw, tid, C::NT);
-    for (int i = tid; i < C::R * NY; i += C::NT) lds[(i / NY) * C::LS + pad16(i % NY)] = rows[i];
+    stage<C::R * NY, C::NT>(
+        tid, [&](int i) { return rows[i]; }, [&](int i, float2 v) { lds[(i / NY) * C::LS + pad16(i % NY)] = v; });
+    FftTw<NY, C::E>::template fill<C::NT>(twl, tw, tid);
     __syncthreads();
     fft_line<NY, C::E, DIR>(LayContig{lds + (tid / C::T) * C::LS}, tid % C::T, true, twl);
     for (int i = tid; i < C::R * NY; i += C::NT) rows[i] = lds[(i / NY) * C::LS + pad16(i % NY)];
@@ -390,11 +417,10 @@ k_line_z(float2* __restrict__ base, int nx, int ny, double scale, DevTables tb) 
     const int tid = threadIdx.x;
     const int plane = blockIdx.y, y0 = blockIdx.x * C::CW;
     float2* P = base + (size_t)plane * ny * NZ;
-    FftTw<NZ, C::E>::fill(twl, tb.tw, tid, C::NT);
-    for (int i = tid; i < C::CW * NZ; i += C::NT) {
-        const int c = i % C::CW, z = i / C::CW;
-        lds[c * C::LS + pad16(z)] = P[(size_t)z * ny + y0 + c];
-    }
+    stage<C::CW * NZ, C::NT>(
+        tid, [&](int i) { return P[(size_t)(i / C::CW) * ny + y0 + i % C::CW]; },
+        [&](int i, float2 v) { lds[(i % C::CW) * C::LS + pad16(i / C::CW)] = v; });
+    FftTw<NZ, C::E>::template fill<C::NT>(twl, tb.tw, tid);
     __syncthreads();
     const LayContig line{lds + (tid / C::T) * C::LS};
     fft_line<NZ, C::E, -1>(line, tid % C::T, true, twl);
@@ -450,11 +476,21 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
     const int tid = threadIdx.x;
     const int ny = g.ny, nz = g.nz;
     const int y0 = blockIdx.x * TY, z = blockIdx.y;
-    FftTw<M, E>::fill(twm, tw, tid, NT);
-    load_twiddles<NX, NT>(twn, tw);
-    for (int i = tid; i < (M + 1) * TY; i += NT) {
-        const int ly = i % TY, k = i / TY;
-        lds[ly * LS + pad16(k)] = P[((size_t)k * nz + z) * ny + y0 + ly];
+    {
+        constexpr int CNT = (M + 1) * TY, K = (CNT + NT - 1) / NT;
+        float2 v[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {  // issued before the table loads
+            const int i = tid + k * NT, ly = i % TY, kk = i / TY;
+            if (CNT % NT == 0 || i < CNT) v[k] = P[((size_t)kk * nz + z) * ny + y0 + ly];
+        }
+        FftTw<M, E>::template fill<NT>(twm, tw, tid);
+        load_twiddles<NX, NT>(twn, tw);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = tid + k * NT, ly = i % TY, kk = i / TY;
+            if (CNT % NT == 0 || i < CNT) lds[ly * LS + pad16(kk)] = v[k];
+        }
     }
     __syncthreads();
     for (int i = tid; i < (M / 2 + 1) * TY; i += NT) {
@@ -477,21 +513,34 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
     fft_line<M, E, 1>(LayContig{lds + (tid / T) * LS}, tid % T, true, twm);
     const size_t row0 = ((size_t)z * ny + y0) * NX;
     double acc = 0.0;
-    for (int i = tid; i < TY * NX; i += NT) {
-        const int l = i / NX, x = i % NX;
-        const float2 zz = lds[l * LS + pad16(x >> 1)];
-        const float q = (x & 1) ? zz.y : zz.x;
-        if (MODE == C2R_Q) {
-            qout[row0 + i] = q;
-        } else {
-            float4 p = psi[row0 + i];
-            float s, c;
-            sincosf(q, &s, &c);  // exp(-i q) = (c, -s)
-            if (qout) qout[row0 + i] = q;  // kept for the adjoint
-            p = make_float4(p.x * c + p.y * s, p.y * c - p.x * s, p.z * c + p.w * s, p.w * c - p.z * s);
-            if (MODE == C2R_GAUGE_RESTRICT && is_jet_cell(x, y0 + l, z, g, jet)) p = jet_phase(p, x, y0 + l, z, g, jet);
-            if (partials) acc += (double)p.x * p.x + (double)p.y * p.y + (double)p.z * p.z + (double)p.w * p.w;
-            psi[row0 + i] = p;
+    // psi is read in groups of G cells per thread, the group's loads issued together
+    constexpr int CNT = TY * NX, G = 8;
+    static_assert(CNT % (NT * G) == 0, "c2r output tiling");
+#pragma unroll 1
+    for (int i0 = tid; i0 < CNT; i0 += NT * G) {
+        float4 pg[G];
+        if (MODE != C2R_Q) {
+#pragma unroll
+            for (int gi = 0; gi < G; gi++) pg[gi] = psi[row0 + i0 + gi * NT];
+        }
+#pragma unroll
+        for (int gi = 0; gi < G; gi++) {
+            const int i = i0 + gi * NT;
+            const int l = i / NX, x = i % NX;
+            const float2 zz = lds[l * LS + pad16(x >> 1)];
+            const float q = (x & 1) ? zz.y : zz.x;
+            if (MODE == C2R_Q) {
+                qout[row0 + i] = q;
+            } else {
+                float4 p = pg[gi];
+                float s, c;
+                sincosf(q, &s, &c);  // exp(-i q) = (c, -s)
+                if (qout) qout[row0 + i] = q;  // kept for the adjoint
+                p = make_float4(p.x * c + p.y * s, p.y * c - p.x * s, p.z * c + p.w * s, p.w * c - p.z * s);
+                if (MODE == C2R_GAUGE_RESTRICT && is_jet_cell(x, y0 + l, z, g, jet)) p = jet_phase(p, x, y0 + l, z, g, jet);
+                if (partials) acc += (double)p.x * p.x + (double)p.y * p.y + (double)p.z * p.z + (double)p.w * p.w;
+                psi[row0 + i] = p;
+            }
         }
     }
     if (MODE != C2R_Q && partials) {
