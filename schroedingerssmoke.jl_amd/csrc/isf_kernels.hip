@@ -887,30 +887,30 @@ k_spawn(float* __restrict__ parts, long long nslots, long long first, long long 
     if (base >= n) return;  // uniform per block (batch members with fewer draws)
     const int m = (int)min((long long)SPAWN_CHUNK, n - base);
     uint32_t raw[4];
+    uint32_t kv = 0xFFFFFFFFu;  // unused entries sort last
     if (t < m) {
         spawn_draw((uint32_t)(base + t), step, seed, raw);
-        key[t] = ((raw[0] >> 10) << 10) | (uint32_t)t;
-    } else {
-        key[t] = 0xFFFFFFFFu;
+        kv = ((raw[0] >> 10) << 10) | (uint32_t)t;
     }
-    __syncthreads();
-    // bitonic sort of the 1024 keys (unused entries sort last)
+    // bitonic sort of the 1024 keys, one key per thread: compare-exchange partners closer
+    // than a wavefront trade through cross-lane shuffles, the rest through LDS
     for (int size = 2; size <= SPAWN_CHUNK; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            const int o = t ^ stride;
-            if (o > t) {
-                const uint32_t a = key[t], b = key[o];
-                const bool up = (t & size) == 0;
-                if ((a > b) == up) {
-                    key[t] = b;
-                    key[o] = a;
-                }
+            uint32_t other;
+            if (stride >= 64) {
+                key[t] = kv;
+                __syncthreads();
+                other = key[t ^ stride];
+                __syncthreads();
+            } else {
+                other = (uint32_t)__shfl_xor((int)kv, stride, 64);
             }
-            __syncthreads();
+            const bool up = (t & size) == 0, lower = (t & stride) == 0;
+            kv = (lower == up) ? min(kv, other) : max(kv, other);
         }
     }
     if (t >= m) return;
-    const uint32_t j = (uint32_t)base + (key[t] & (SPAWN_CHUNK - 1));
+    const uint32_t j = (uint32_t)base + (kv & (SPAWN_CHUNK - 1));
     spawn_draw(j, step, seed, raw);
     float u[4];
 #pragma unroll
