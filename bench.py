@@ -52,8 +52,7 @@ def stage_bytes(nx, ny, nz, nslots, n_spawn):
         "advect": 24 * nslots + 12 * N,
     }
     for p in ("p1", "p2"):
-        b[f"{p}_div"] = 16 * N + 4 * N
-        b[f"{p}_r2c_x"] = 4 * N + half
+        b[f"{p}_div_r2c_x"] = 16 * N + half  # psi in, half spectrum out (f stays on chip)
         b[f"{p}_plane"] = 2 * half
     b["p1_c2r_x_gauge_restrict"] = half + 32 * N
     b["p2_c2r_x_gauge"] = half + 32 * N
@@ -278,9 +277,9 @@ PARTICLE_STAGES = ("spawn", "particle_order", "advect")
 # stage name -> kernel name pattern (flow planes are k_plane<NY, NZ, 0>, Poisson planes <.., 1>)
 STAGE_KERNELS = {
     "flow_x_fwd": r"k_flow_x_fwd<", "flow_plane": r"k_plane<\d+, \d+, 0>", "flow_x_inv_normalize": r"k_flow_x_inv<",
-    "p1_div": r"k_div_psi<", "p1_r2c_x": r"k_r2c_x<", "p1_plane": r"k_plane<\d+, \d+, 1>",
-    "p1_c2r_x_gauge_restrict": r"k_c2r_x<\d+, \d+, 2>", "spawn": r"k_spawn", "p2_div": r"k_div_psi<",
-    "p2_r2c_x": r"k_r2c_x<", "p2_plane": r"k_plane<\d+, \d+, 1>", "p2_c2r_x_gauge": r"k_c2r_x<\d+, \d+, 1>",
+    "p1_div_r2c_x": r"k_div_r2c<", "p1_plane": r"k_plane<\d+, \d+, 1>",
+    "p1_c2r_x_gauge_restrict": r"k_c2r_x<\d+, \d+, 2>", "spawn": r"k_spawn", "p2_div_r2c_x": r"k_div_r2c<",
+    "p2_plane": r"k_plane<\d+, \d+, 1>", "p2_c2r_x_gauge": r"k_c2r_x<\d+, \d+, 1>",
     "velocity_sharp": r"k_velocity<", "advect": r"k_advect\(",
 }
 

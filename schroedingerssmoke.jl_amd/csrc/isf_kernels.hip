@@ -721,6 +721,117 @@ k_velocity(const float4* __restrict__ psi, float* __restrict__ vel, GridInfo g, 
     }
 }
 
+// ---- fused divergence + R2C x pass
+
+// Threads of the fused pass: 8 cells of the TY x NX tile each, at least one wavefront and
+// at least the TY * T threads of the R2C transform.
+template <int NX, int TY> struct DivR2cCfg {
+    static constexpr int M = NX / 2;
+    static constexpr int TF = TY * XCfg<M>::T;
+    static constexpr int NC = TY * NX / 8;
+    static constexpr int NT0 = NC > TF ? NC : TF;
+    static constexpr int NT = NT0 > 64 ? NT0 : 64;
+};
+
+// pressure_project!'s f = div(velocity_one_form(psi, 1)) (src/parallel.jl:100-113,131-143,
+// 156-157) fused with the real-to-complex x transform of k_r2c_x, so f never goes to
+// memory.  A block computes f for TY x lines at one z: psi(c + e_x) from the neighbouring
+// lane, psi(c + e_y), psi(c +- e_z) and the row y0 - 1 through the caches.  The edge angles
+// and their differences are those of k_div_psi (same operands).  Tiles are mapped
+// XCD-aware like the stencils, so the z +- 1 rows a block reads are its neighbours' own
+// rows, in the same L2.
+template <int NX, int TY>
+__global__ void __launch_bounds__((DivR2cCfg<NX, TY>::NT))
+k_div_r2c(const float4* __restrict__ psi, float2* __restrict__ P, GridInfo g, const float2* __restrict__ tw,
+          Batch B) {
+    using C = DivR2cCfg<NX, TY>;
+    constexpr int M = NX / 2, NT = C::NT, CELLS = TY * NX;
+    constexpr int T = XCfg<M>::T, E = XCfg<M>::E, LS = XCfg<M>::LS;
+    static_assert(CELLS % NT == 0 && NT % 64 == 0, "div r2c tiling");
+    const StTile tl = stencil_tile();  // x: y tile, y: z, z: member
+    if (B.inst) {
+        const Inst& m = B.inst[tl.bz];
+        psi = m.psi;
+        P = poisson_P(m);
+        g = m.g;
+        tw = m.tb.tw;
+    }
+    __shared__ float vx_s[TY][NX];
+    __shared__ float vy_s[TY + 1][NX];  // row l + 1: v_y at y0 + l; row 0: at y0 - 1
+    __shared__ float2 lds[TY * LS];     // v_z differences, then f as packed complex lines
+    __shared__ float2 twm[FftTw<M, E>::SIZE];
+    __shared__ float2 twn[NX];
+    float* dz_s = reinterpret_cast<float*>(lds);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int ny = g.ny, nz = g.nz;
+    const int y0 = tl.bx * TY, z = tl.by;
+    const size_t sy = NX, sz = (size_t)NX * ny;
+    const size_t zo = sz * z, zpo = sz * (z + 1 < nz ? z + 1 : 0), zmo = sz * (z > 0 ? z - 1 : nz - 1);
+    FftTw<M, E>::template fill<NT>(twm, tw, tid);
+    load_twiddles<NX, NT>(twn, tw);
+    // v_y of the row below the tile
+    {
+        const int ym = y0 > 0 ? y0 - 1 : ny - 1;
+        for (int x = tid; x < NX; x += NT) vy_s[0][x] = edge_angle(psi[sy * ym + zo + x], psi[sy * y0 + zo + x]);
+    }
+    // edge angles of the tile's cells, G cells per thread at a time with their loads issued
+    // together; a wavefront holds consecutive x of one row
+    constexpr int K = CELLS / NT, G = K < 4 ? K : 4;
+#pragma unroll 1
+    for (int k0 = 0; k0 < K; k0 += G) {
+        float4 pc[G], py[G], pz[G], pzm[G], pxh[G];
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+            const int i = tid + (k0 + q) * NT, l = i / NX, x = i % NX, y = y0 + l;
+            const size_t row = sy * y;
+            pc[q] = psi[row + zo + x];
+            py[q] = psi[sy * (y + 1 < ny ? y + 1 : 0) + zo + x];
+            pz[q] = psi[row + zpo + x];
+            pzm[q] = psi[row + zmo + x];
+            pxh[q] = pc[q];
+            if (lane == 63 || x + 1 == NX) pxh[q] = psi[row + zo + (x + 1 < NX ? x + 1 : 0)];
+        }
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+            const int i = tid + (k0 + q) * NT, l = i / NX, x = i % NX;
+            float4 px = shfl4(pc[q], lane < 63 ? lane + 1 : lane);
+            if (lane == 63 || x + 1 == NX) px = pxh[q];
+            vx_s[l][x] = edge_angle(pc[q], px);
+            vy_s[l + 1][x] = edge_angle(pc[q], py[q]);
+            dz_s[i] = edge_angle(pc[q], pz[q]) - edge_angle(pzm[q], pc[q]);
+        }
+    }
+    __syncthreads();
+    // f(c) = sum_a (v_a(c) - v_a(c - e_a)) / d_a^2
+    const float ds0 = 1.0f / (g.dx * g.dx), ds1 = 1.0f / (g.dy * g.dy), ds2 = 1.0f / (g.dz * g.dz);
+    float fv[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT, l = i / NX, x = i % NX;
+        const float vx = vx_s[l][x], vxm = vx_s[l][x > 0 ? x - 1 : NX - 1];
+        const float vy = vy_s[l + 1][x], vym = vy_s[l][x];
+        fv[k] = (vx - vxm) * ds0 + (vy - vym) * ds1 + dz_s[i] * ds2;
+    }
+    __syncthreads();  // dz_s is read; lds now receives f
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT, l = i / NX, x = i % NX;
+        reinterpret_cast<float*>(lds + l * LS + pad16(x >> 1))[x & 1] = fv[k];
+    }
+    __syncthreads();
+    // the R2C transform of k_r2c_x
+    fft_line<M, E, -1>(LayContig{lds + (tid / T) * LS}, tid % T, tid < TY * T, twm);
+    for (int i = tid; i < (M + 1) * TY; i += NT) {
+        const int ly = i % TY, k = i / TY;
+        const float2 zk = lds[ly * LS + pad16(k % M)];
+        const float2 zm = cconj(lds[ly * LS + pad16((M - k) % M)]);
+        const float2 e = cscale(cadd(zk, zm), 0.5f);
+        const float2 dd = csub(zk, zm);
+        const float2 o = make_float2(0.5f * dd.y, -0.5f * dd.x);  // (zk - zm) / (2i)
+        P[((size_t)k * nz + z) * ny + y0 + ly] = cadd(e, cmul(twn[k], o));
+    }
+}
+
 // div! from a velocity field (src/parallel.jl:131-152).
 __global__ void __launch_bounds__(256) k_div_vel(const float* __restrict__ vel, float* __restrict__ f, GridInfo g) {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1025,6 +1136,14 @@ int launch_flow_x_inv_split(const GridInfo& g, const float2* S, float2* oa, floa
 }
 int launch_r2c_x(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s, Batch b) {
     ISF_X_DISPATCH(r2c_x_t, g, f, P, t, s, b);
+}
+template <int NX, int TY>
+static int div_r2c_t(const GridInfo& g, const float4* psi, float2* P, const DevTables& t, hipStream_t s, Batch b) {
+    dim3 grid(g.ny / TY, g.nz, members_of(b));
+    ISF_LAUNCH((k_div_r2c<NX, TY>), grid, dim3(DivR2cCfg<NX, TY>::NT), s, psi, P, g, t.tw, b);
+}
+int launch_div_r2c(const GridInfo& g, const float4* psi, float2* P, const DevTables& t, hipStream_t s, Batch b) {
+    ISF_X_DISPATCH(div_r2c_t, g, psi, P, t, s, b);
 }
 int launch_c2r_x(const GridInfo& g, const float2* P, int mode, float* q, float4* psi, const JetParams* jet,
                  double* norm_partials, const DevTables& t, hipStream_t s, Batch b) {

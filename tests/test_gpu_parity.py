@@ -135,6 +135,14 @@ def test_pointwise_and_stencils(isf_mod, n, L):
     err = rel_l2(dp.to_host(), ref)
     print(f"  pressure_project rel L2 = {err:.3e}")
     assert err < 2e-6
+    # pressure_project! runs div(one_form) fused into the R2C x pass; the separate operators
+    # (one-form with hbar 1, div!, poisson_solve, gauge_transform) give the same psi
+    dc = isf.psi_array(s)
+    v1 = isf.velocity_one_form_(dc, hbar=1.0, sharp=False, out=isf_mod.DeviceArray(isf.shape3 + (3,), np.float32))
+    f1 = isf_mod.DeviceArray(isf.shape3, np.float32)
+    isf.div_(f1, v1)
+    isf.gauge_transform_(dc, isf.poisson_solve(f1))
+    assert rel_l2(dc.to_host(), dp.to_host()) < 1e-6
 
 
 @pytest.mark.parametrize("mode", [0, 1])
