@@ -152,15 +152,18 @@ struct LayGlobal {
     __device__ __forceinline__ void st(int e, float2 v) const { p[e] = v; }
 };
 
-// Every thread owns E = 16 elements of one y line during the y transforms and of one z
-// line during the z transforms, so NT = NY * NZ / 16 threads.  With 1024 threads a thread
-// has 128 VGPRs, too few for a second 16-element buffer, so the next-plane prefetch is
-// used only when NT <= 512.
+// Every thread owns E elements of one y line during the y transforms and of one z line
+// during the z transforms, so NT = NY * NZ / E threads: E = 16, or 8 when both sides are
+// at most 64 (twice the wavefronts for the small planes, whose few workgroups are
+// latency-bound).  The first pass of each direction has radix R1 = E, the second
+// N / R1.  With 1024 threads a thread has 128 VGPRs, too few for a second 16-element
+// buffer, so the next-plane prefetch is used only when NT <= 512.
 template <int NY, int NZ> struct PlaneCfg {
-    static constexpr int E = 16;
+    static constexpr int E = (NY <= 64 && NZ <= 64) ? 8 : 16;
+    static constexpr int R1 = E;
     static constexpr int TY = NY / E, TZ = NZ / E;   // threads per y line / z line
     static constexpr int NT = NZ * TY;
-    static constexpr int RY2 = NY / 16, RZ2 = NZ / 16;  // second radix (1: single pass)
+    static constexpr int RY2 = NY / R1, RZ2 = NZ / R1;  // second radix (1: single pass)
     static constexpr int LSY = padded_len(NY);
     static constexpr int LDS_BYTES = NZ * LSY * 8 + 16 * (NY + NZ);
     static constexpr bool PREFETCH = NT <= 512;
@@ -173,8 +176,8 @@ enum { PLANE_FLOW = 0, PLANE_POISSON = 1, PLANE_FLOW_ADJ = 2 };
 
 // y and z transforms of one kx plane, spectral multiplier, inverse z and y; the block
 // walks planes blockIdx.x, blockIdx.x + gridDim.x, ... and prefetches the next plane into
-// registers while it transforms the current one.  Schedules: forward [16, n/16],
-// inverse [n/16, 16]; the multiplier and the first inverse z pass act on the registers of
+// registers while it transforms the current one.  Schedules: forward [R1, n/R1],
+// inverse [n/R1, R1]; the multiplier and the first inverse z pass act on the registers of
 // the last forward z pass (same elements), the first forward y pass reads global memory
 // and the last inverse y pass writes it.
 //   PLANE_FLOW   : plane = c*nx + kx, multiplier mask_x[kx] mask_y[ky] mask_z[kz]
@@ -194,17 +197,18 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
     }
     constexpr int E = C::E, NT = C::NT, LSY = C::LSY;
     constexpr int RY2 = C::RY2, RZ2 = C::RZ2;
-    using TY2 = PassTw<NY, RY2, 16>;   // forward y pass 2
-    using TY2I = PassTw<NY, 16, RY2>;  // inverse y pass 2
-    using TZ2 = PassTw<NZ, RZ2, 16>;
-    using TZ2I = PassTw<NZ, 16, RZ2>;
+    constexpr int R1 = C::R1;
+    using TY2 = PassTw<NY, RY2, R1>;   // forward y pass 2
+    using TY2I = PassTw<NY, R1, RY2>;  // inverse y pass 2
+    using TZ2 = PassTw<NZ, RZ2, R1>;
+    using TZ2I = PassTw<NZ, R1, RZ2>;
     __shared__ float2 lds[NZ * LSY];
     __shared__ float2 twy[TY2::SIZE + TY2I::SIZE + 1];
     __shared__ float2 twz[TZ2::SIZE + TZ2I::SIZE + 1];
     __shared__ double mtab[NY + NZ];  // FLOW: float2 mask_y, mask_z ; POISSON: s2y, s2z
     const int tid = threadIdx.x;
     const int zl = tid / C::TY, ty = tid % C::TY;  // y transforms: line z = zl
-    using Y1 = RPass<NY, 16, 1, E>;
+    using Y1 = RPass<NY, R1, 1, E>;
     float2 v[E], pf[C::PREFETCH ? E : 1];
     // the first plane's load is issued before the table loads
     if (blockIdx.x < nplanes)
@@ -221,7 +225,7 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
     const int yl = tid % NY, tz = tid / NY;        // z transforms: line y = yl
     const LayContig ly{lds + zl * LSY};
     const LayStrided lz{lds + pad16(yl), LSY};
-    using Z1 = RPass<NZ, 16, 1, E>;
+    using Z1 = RPass<NZ, R1, 1, E>;
 
     int plane = blockIdx.x;
 #pragma unroll 1
@@ -241,8 +245,8 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
         __syncthreads();  // the previous plane's LDS reads are done, tables loaded
         Y1::store(v, ly, ty);
         __syncthreads();
-        if constexpr (NY > 16) {
-            using Y2 = RPass<NY, RY2, 16, E>;
+        if constexpr (RY2 > 1) {
+            using Y2 = RPass<NY, RY2, R1, E>;
             Y2::load(v, ly, ty);
             Y2::template compute<-1>(v, ty, twy);  // TY2 table
             __syncthreads();
@@ -252,9 +256,9 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
         // ---- forward z, multiplier, inverse z
         Z1::load(v, lz, tz);
         Z1::template compute<-1>(v, tz, twz);
-        using ZL = std::conditional_t<(NZ > 16), RPass<NZ, RZ2, 16, E>, Z1>;   // last forward pass
-        using ZI = std::conditional_t<(NZ > 16), RPass<NZ, RZ2, 1, E>, Z1>;    // first inverse pass
-        if constexpr (NZ > 16) {
+        using ZL = std::conditional_t<(RZ2 > 1), RPass<NZ, RZ2, R1, E>, Z1>;   // last forward pass
+        using ZI = std::conditional_t<(RZ2 > 1), RPass<NZ, RZ2, 1, E>, Z1>;    // first inverse pass
+        if constexpr (RZ2 > 1) {
             __syncthreads();
             Z1::store(v, lz, tz);
             __syncthreads();
@@ -293,8 +297,8 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
         __syncthreads();
         ZI::store(v, lz, tz);
         __syncthreads();
-        if constexpr (NZ > 16) {
-            using Z2I = RPass<NZ, 16, RZ2, E>;
+        if constexpr (RZ2 > 1) {
+            using Z2I = RPass<NZ, R1, RZ2, E>;
             Z2I::load(v, lz, tz);
             Z2I::template compute<1>(v, tz, twz + TZ2::SIZE);
             __syncthreads();
@@ -302,9 +306,9 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
             __syncthreads();
         }
         // ---- inverse y; the last pass writes global memory
-        if constexpr (NY > 16) {
+        if constexpr (RY2 > 1) {
             using Y1I = RPass<NY, RY2, 1, E>;
-            using Y2I = RPass<NY, 16, RY2, E>;
+            using Y2I = RPass<NY, R1, RY2, E>;
             Y1I::load(v, ly, ty);
             Y1I::template compute<1>(v, ty, twy);
             __syncthreads();
