@@ -29,6 +29,13 @@ template <int NX> struct XCfg {
     static constexpr int T = NX / E;
     static constexpr int LS = padded_len(NX);
 };
+// The C2R gauge pass's half-length transform: 8 elements per thread for M = 128 (twice
+// the threads of XCfg; that pass also streams psi in and out, and more wavefronts hide
+// those loads: 33 -> 29.5 us at 256x128x128).
+template <int M> struct C2rCfg {
+    static constexpr int E = M == 128 ? 8 : DefaultE<M>::value;
+    static constexpr int T = M / E;
+};
 
 // ======================================================================== Schroedinger x passes
 
@@ -457,7 +464,7 @@ enum { C2R_Q = 0, C2R_GAUGE = 1, C2R_GAUGE_RESTRICT = 2 };
 // constraint applied afterwards in C2R_GAUGE_RESTRICT (src/particles.jl:12-27).
 // Optional FP64 partial sums of |psi1|^2 + |psi2|^2 per block.
 template <int NX, int TY, int MODE>
-__global__ void __launch_bounds__(TY * XCfg<NX / 2>::T)
+__global__ void __launch_bounds__(TY * C2rCfg<NX / 2>::T)
 k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restrict__ psi, GridInfo g, JetParams jet,
         double* __restrict__ partials, const float2* __restrict__ tw, Batch B) {
     constexpr int M = NX / 2;
@@ -471,7 +478,7 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
         partials = partials ? m.norm_parts : nullptr;
         tw = m.tb.tw;
     }
-    constexpr int T = XCfg<M>::T, E = XCfg<M>::E;
+    constexpr int T = C2rCfg<M>::T, E = C2rCfg<M>::E;
     constexpr int LS = padded_len(M + 1);
     constexpr int NT = TY * T;
     __shared__ float2 lds[TY * LS];
@@ -1094,7 +1101,7 @@ template <int NX, int TY>
 static int c2r_x_t(const GridInfo& g, const float2* P, int mode, float* q, float4* psi, const JetParams* jet,
                    double* parts, const DevTables& t, hipStream_t s, Batch b) {
     dim3 grid(g.ny / TY, g.nz, members_of(b));
-    dim3 block(TY * XCfg<NX / 2>::T);
+    dim3 block(TY * C2rCfg<NX / 2>::T);
     JetParams j{};
     if (jet) j = *jet;
     if (mode == C2R_Q) ISF_LAUNCH((k_c2r_x<NX, TY, C2R_Q>), grid, block, s, P, q, psi, g, j, parts, t.tw, b);
