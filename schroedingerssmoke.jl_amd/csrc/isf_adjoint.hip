@@ -348,32 +348,26 @@ __device__ __forceinline__ void stag_eval(const float* __restrict__ vel, const f
                w1[a] * ((1.0f - w2[a]) * e.pp2[a] + w2[a] * e.vn[a]);
 }
 
-// Workgroup-local accumulation of the velocity cotangent: four LDS windows of ADJ_W
-// cells (3 components each) starting at cells cmin + {0, nx, nx ny, nx + nx ny}, cmin =
-// the lowest base cell among the workgroup's evaluations.  Every target of an evaluation
-// at base cell c is c + {0, 1, nx, nx + 1, nx ny, nx ny + 1, nx + nx ny} away from the
-// periodic wrap, so with cell-sorted slots almost all targets fall in a window; the rest
-// (and wrapped ones) go straight to global atomics.  The windows are added to velbar once
-// per workgroup, one atomic per touched (cell, component).
-constexpr int ADJ_W = 256;
+// Workgroup-local accumulation of the velocity cotangent: an LDS box of ADJ_WX x ADJ_WY x
+// ADJ_WZ cells (3 components each) whose corner is one cell below the lowest x, y and z of
+// the workgroup's particle cells.  Cell-sorted slots put a workgroup's particles in a few
+// neighbouring cells; the RK stages move at most about a cell and every target is a base
+// cell or its +1 neighbour, so nearly all targets fall in the box.  The rest (and targets
+// wrapped by the periodic boundary) go straight to global atomics.  The box is added to
+// velbar once per workgroup, one atomic per touched (cell, component).
+constexpr int ADJ_WX = 32, ADJ_WY = 8, ADJ_WZ = 8, ADJ_BOX = ADJ_WX * ADJ_WY * ADJ_WZ;
 typedef __attribute__((address_space(3))) float lds_float;  // an LDS address (ds_* instructions)
 struct AdjAcc {
-    lds_float* lds;      // [4][ADJ_W][3]
-    long long cmin;      // first cell of window 0 (workgroup-uniform)
-    long long sy, sz;    // window w starts at cmin + (w & 1) sy + (w >> 1) sz
+    lds_float* lds;      // [ADJ_WZ][ADJ_WY][ADJ_WX][3]
+    int ox, oy, oz;      // box corner (cell coordinates, workgroup-uniform)
     float* velbar;
-    __device__ __forceinline__ long long base(int w) const { return cmin + (w & 1) * sy + (w >> 1) * sz; }
-    __device__ __forceinline__ void add(size_t t, int a, float v) const {
-#pragma unroll
-        for (int w = 0; w < 4; w++) {
-            const long long d = (long long)t - base(w);
-            if (d >= 0 && d < ADJ_W) {
-                __hip_atomic_fetch_add(lds + (w * ADJ_W + (int)d) * 3 + a, v, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                return;
-            }
-        }
-        unsafeAtomicAdd(velbar + 3 * t + a, v);
+    __device__ __forceinline__ void add(int x, int y, int z, size_t t, int a, float v) const {
+        const unsigned lx = (unsigned)(x - ox), ly = (unsigned)(y - oy), lz = (unsigned)(z - oz);
+        if ((lx < (unsigned)ADJ_WX) & (ly < (unsigned)ADJ_WY) & (lz < (unsigned)ADJ_WZ))
+            __hip_atomic_fetch_add(lds + ((lz * ADJ_WY + ly) * ADJ_WX + lx) * 3 + a, v, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            unsafeAtomicAdd(velbar + 3 * t + a, v);
     }
 };
 
@@ -406,17 +400,18 @@ __device__ __forceinline__ void wave_scatter12(const AdjAcc& acc, const StagEval
     const unsigned knext = __shfl_down(key, 1u, 64);
     if (!act || (lane != 63 && knext == key)) return;
     const int *i = e.i, *ip = e.ip;
-    const size_t c000 = cidx(i[0], i[1], i[2], g);
-    const size_t cv[3][3] = {// (pp, pp2, vn) cells per component
-                             {cidx(i[0], ip[1], i[2], g), cidx(i[0], i[1], ip[2], g), cidx(i[0], ip[1], ip[2], g)},
-                             {cidx(ip[0], i[1], i[2], g), cidx(i[0], i[1], ip[2], g), cidx(ip[0], i[1], ip[2], g)},
-                             {cidx(ip[0], i[1], i[2], g), cidx(i[0], ip[1], i[2], g), cidx(ip[0], ip[1], i[2], g)}};
+    // (pp, pp2, vn) cells of each component after the base cell, as (x, y, z)
+    const int cv[3][3][3] = {{{i[0], ip[1], i[2]}, {i[0], i[1], ip[2]}, {i[0], ip[1], ip[2]}},
+                             {{ip[0], i[1], i[2]}, {i[0], i[1], ip[2]}, {ip[0], i[1], ip[2]}},
+                             {{ip[0], i[1], i[2]}, {i[0], ip[1], i[2]}, {ip[0], ip[1], i[2]}}};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        acc.add(c000, a, val[4 * a]);
-        acc.add(cv[a][0], a, val[4 * a + 1]);
-        acc.add(cv[a][1], a, val[4 * a + 2]);
-        acc.add(cv[a][2], a, val[4 * a + 3]);
+        acc.add(i[0], i[1], i[2], cidx(i[0], i[1], i[2], g), a, val[4 * a]);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int* c = cv[a][q];
+            acc.add(c[0], c[1], c[2], cidx(c[0], c[1], c[2], g), a, val[4 * a + 1 + q]);
+        }
     }
 }
 
@@ -456,10 +451,10 @@ k_advect_adj(const float* __restrict__ parts0, long long nslots, const int* __re
     // every lane runs to the end (the scatter works across the wavefront and the
     // workgroup); lanes past the last slot or on a dead slot do the arithmetic on a
     // stand-in position and add nothing
-    __shared__ float win[4 * ADJ_W * 3];
-    __shared__ unsigned cmin_s;
-    for (int i = threadIdx.x; i < 4 * ADJ_W * 3; i += blockDim.x) win[i] = 0.0f;
-    if (threadIdx.x == 0) cmin_s = 0xFFFFFFFFu;
+    __shared__ float win[ADJ_BOX * 3];
+    __shared__ int cmin_s[3];
+    for (int i = threadIdx.x; i < ADJ_BOX * 3; i += blockDim.x) win[i] = 0.0f;
+    if (threadIdx.x < 3) cmin_s[threadIdx.x] = 0x7FFFFFFF;
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool slot = k < nslots;
     const long long s = slot ? (order ? (long long)order[k] : k) : 0;
@@ -493,28 +488,33 @@ k_advect_adj(const float* __restrict__ parts0, long long nslots, const int* __re
 #pragma unroll
         for (int a = 0; a < 3; a++) P[3][a] = p[a] + kk[a] * dt;
     }
-    // lowest base cell of the workgroup's evaluations
-    unsigned m = 0xFFFFFFFFu;
+    // lowest x, y and z of the workgroup's particle cells
+    int m[3] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF};
     if (in) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            int i[3], ip[3];
-            float w[3];
-            stag_index(P[q], g, i, ip, w);
-            m = min(m, (unsigned)cidx(i[0], i[1], i[2], g));
-        }
+        int i[3], ip[3];
+        float w[3];
+        stag_index(P[0], g, i, ip, w);
+        m[0] = i[0];
+        m[1] = i[1];
+        m[2] = i[2];
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = min(m, (unsigned)__shfl_xor((int)m, off, 64));
-    __syncthreads();  // zeroed windows and cmin_s
-    if ((threadIdx.x & 63u) == 0) atomicMin(&cmin_s, m);
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m[a] = min(m[a], __shfl_xor(m[a], off, 64));
+    __syncthreads();  // zeroed box and cmin_s
+    if ((threadIdx.x & 63u) == 0) {
+        atomicMin(&cmin_s[0], m[0]);
+        atomicMin(&cmin_s[1], m[1]);
+        atomicMin(&cmin_s[2], m[2]);
+    }
     __syncthreads();
     AdjAcc acc;
     acc.lds = (lds_float*)win;
     acc.velbar = velbar;
-    acc.cmin = (long long)__builtin_amdgcn_readfirstlane(cmin_s);
-    acc.sy = g.nx;
-    acc.sz = (long long)g.nx * g.ny;
+    acc.ox = __builtin_amdgcn_readfirstlane(cmin_s[0]) - 1;
+    acc.oy = __builtin_amdgcn_readfirstlane(cmin_s[1]) - 1;
+    acc.oz = __builtin_amdgcn_readfirstlane(cmin_s[2]) - 1;
     float pb[3] = {0.0f, 0.0f, 0.0f};
     if (in) {
         pb[0] = pbar[3 * s];
@@ -545,13 +545,14 @@ k_advect_adj(const float* __restrict__ parts0, long long nslots, const int* __re
 #pragma unroll
         for (int a = 0; a < 3; a++) pbar[3 * s + a] = in ? pacc[a] : 0.0f;
     }
-    // add the windows to velbar (entries never touched are zero and skipped)
+    // add the box to velbar (entries never touched are zero and skipped; a touched entry
+    // is a valid cell, since only targets inside the grid were added)
     __syncthreads();
-    for (int i = threadIdx.x; i < 4 * ADJ_W * 3; i += blockDim.x) {
+    for (int i = threadIdx.x; i < ADJ_BOX * 3; i += blockDim.x) {
         const float v = win[i];
         if (v != 0.0f) {
-            const int w = i / (ADJ_W * 3), r = i - w * (ADJ_W * 3);
-            unsafeAtomicAdd(velbar + 3 * (acc.base(w) + r / 3) + r % 3, v);
+            const int c = i / 3, lx = c % ADJ_WX, ly = (c / ADJ_WX) % ADJ_WY, lz = c / (ADJ_WX * ADJ_WY);
+            unsafeAtomicAdd(velbar + 3 * cidx(acc.ox + lx, acc.oy + ly, acc.oz + lz, g) + i % 3, v);
         }
     }
 }
