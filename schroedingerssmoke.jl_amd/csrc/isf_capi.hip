@@ -342,7 +342,7 @@ double isf::poisson_scale(isf_t h) { return -0.25 * 2.0 / (double)h->N; }
 // velocity one-form (hbar=1) + div + Poisson + gauge; mode C2R_GAUGE(_RESTRICT)
 int isf::project(isf_t h, float* psi, int c2r_mode, const JetParams* jet, double* parts) {
     float4* p = reinterpret_cast<float4*>(psi);
-    LAUNCH_CHECK(launch_div_r2c(h->g, p, P_of(h), h->tb, h->stream), "div(one_form(psi)) + poisson x forward");
+    LAUNCH_CHECK(launch_div_r2c(h->g, p, f_of(h), P_of(h), h->tb, h->stream), "div(one_form(psi)) + poisson x forward");
     LAUNCH_CHECK(launch_poisson_plane(h->g, P_of(h), poisson_scale(h), h->tb, h->stream), "poisson plane");
     LAUNCH_CHECK(launch_c2r_x(h->g, P_of(h), c2r_mode, nullptr, p, jet, parts, h->tb, h->stream), "poisson x inverse + gauge");
     return ISF_OK;
@@ -569,12 +569,12 @@ static int step_impl(isf_t h, float* psi, float* parts, long long nslots, int it
     STAGE(launch_flow_x_inv(h->g, h->S, p, true, h->tb, s), "flow x inverse + normalize");
     // :50 pressure_project! with the restrict of :57 fused into its gauge pass
     const double sc = poisson_scale(h);
-    STAGE(launch_div_r2c(h->g, p, P_of(h), h->tb, s), "div(one_form(psi)) + poisson x forward");
+    STAGE(launch_div_r2c(h->g, p, f_of(h), P_of(h), h->tb, s), "div(one_form(psi)) + poisson x forward");
     STAGE(launch_poisson_plane(h->g, P_of(h), sc, h->tb, s), "poisson plane");
     STAGE(launch_c2r_x(h->g, P_of(h), 2, nullptr, p, &j, nullptr, h->tb, s), "poisson x inverse + gauge + restrict");
     if (!side) STAGE(launch_spawn(parts, nslots, first0, nsp, h->jet, h->seed, (unsigned)iter, s), "add_particles");
     // :58 pressure_project! (FP64 norm partials accumulated in its gauge pass)
-    STAGE(launch_div_r2c(h->g, p, P_of(h), h->tb, s), "div(one_form(psi)) + poisson x forward");
+    STAGE(launch_div_r2c(h->g, p, f_of(h), P_of(h), h->tb, s), "div(one_form(psi)) + poisson x forward");
     STAGE(launch_poisson_plane(h->g, P_of(h), sc, h->tb, s), "poisson plane");
     STAGE(launch_c2r_x(h->g, P_of(h), 1, nullptr, p, nullptr, h->norm_parts, h->tb, s), "poisson x inverse + gauge");
     // :59-62 velocity_one_form!(isf, psi, hbar) and the in-place StaggeredSharp; the
@@ -875,7 +875,8 @@ static int group_stages(const BatchGroup& gr, const Inst* inst, const int* count
     LAUNCH_CHECK(launch_flow_plane(g, nullptr, t0, s, bt), "batch flow plane");
     LAUNCH_CHECK(launch_flow_x_inv(g, nullptr, nullptr, true, t0, s, bt), "batch flow x inverse + normalize");
     for (int pass = 0; pass < 2; pass++) {
-        LAUNCH_CHECK(launch_div_r2c(g, nullptr, nullptr, t0, s, bt), "batch div(one_form(psi)) + poisson x forward");
+        LAUNCH_CHECK(launch_div_r2c(g, nullptr, nullptr, nullptr, t0, s, bt),
+                     "batch div(one_form(psi)) + poisson x forward");
         LAUNCH_CHECK(launch_poisson_plane(g, nullptr, 0.0, t0, s, bt), "batch poisson plane");
         LAUNCH_CHECK(launch_c2r_x(g, nullptr, pass == 0 ? 2 : 1, nullptr, nullptr, nullptr,
                                   pass == 0 ? nullptr : norm_flag, t0, s, bt),
@@ -1086,7 +1087,7 @@ static int time_x_tile(isf_batch_t b, const BatchGroup& gr, int ty, const Inst* 
     auto once = [&]() -> int {
         LAUNCH_CHECK(launch_flow_x_fwd(g, nullptr, nullptr, t0, b->stream, bt), "tune x forward");
         LAUNCH_CHECK(launch_flow_x_inv(g, nullptr, nullptr, true, t0, b->stream, bt), "tune x inverse");
-        LAUNCH_CHECK(launch_div_r2c(g, nullptr, nullptr, t0, b->stream, bt), "tune div + r2c");
+        LAUNCH_CHECK(launch_div_r2c(g, nullptr, nullptr, nullptr, t0, b->stream, bt), "tune div + r2c");
         LAUNCH_CHECK(launch_c2r_x(g, nullptr, 1, nullptr, nullptr, nullptr, nullptr, t0, b->stream, bt), "tune c2r");
         return ISF_OK;
     };

@@ -82,8 +82,9 @@ int launch_flow_x_fwd_split(const GridInfo& g, const float2* sa, const float2* s
 int launch_flow_x_inv_split(const GridInfo& g, const float2* S, float2* oa, float2* ob, const DevTables& t,
                             hipStream_t s);
 int launch_r2c_x(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s, Batch b = Batch{});
-// div(one_form(psi), hbar = 1) and the R2C x pass in one kernel: f is not stored
-int launch_div_r2c(const GridInfo& g, const float4* psi, float2* P, const DevTables& t, hipStream_t s,
+// div(one_form(psi), hbar = 1) and the R2C x pass, in one kernel (f not stored) while a z
+// plane of psi is at most 1 MB, else as k_div_psi into f (4N bytes of scratch) and R2C
+int launch_div_r2c(const GridInfo& g, const float4* psi, float* f, float2* P, const DevTables& t, hipStream_t s,
                    Batch b = Batch{});
 int launch_poisson_plane(const GridInfo& g, float2* P, double scale, const DevTables& t, hipStream_t s,
                          Batch b = Batch{});

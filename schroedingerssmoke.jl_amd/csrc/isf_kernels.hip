@@ -1153,7 +1153,17 @@ static int div_r2c_t(const GridInfo& g, const float4* psi, float2* P, const DevT
     dim3 grid(g.ny / TY, g.nz, members_of(b));
     ISF_LAUNCH((k_div_r2c<NX, TY>), grid, dim3(DivR2cCfg<NX, TY>::NT), s, psi, P, g, t.tw, b);
 }
-int launch_div_r2c(const GridInfo& g, const float4* psi, float2* P, const DevTables& t, hipStream_t s, Batch b) {
+// Fused while a z plane of psi (nx ny 16 bytes) is at most 1 MB: the z +- 1 rows a block
+// re-reads then stay in its XCD's L2.  Larger grids run k_div_psi (z-marching, each row
+// read once) into f and the R2C pass.
+static bool div_r2c_fused(const GridInfo& g) { return (size_t)g.nx * g.ny * 16 <= (size_t)1 << 20; }
+int launch_div_r2c(const GridInfo& g, const float4* psi, float* f, float2* P, const DevTables& t, hipStream_t s,
+                   Batch b) {
+    if (!div_r2c_fused(g)) {
+        const int e = launch_div_psi(g, psi, f, s, b);
+        if (e) return e;
+        return launch_r2c_x(g, f, P, t, s, b);
+    }
     ISF_X_DISPATCH(div_r2c_t, g, psi, P, t, s, b);
 }
 int launch_c2r_x(const GridInfo& g, const float2* P, int mode, float* q, float4* psi, const JetParams* jet,

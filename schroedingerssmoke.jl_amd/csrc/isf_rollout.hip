@@ -117,7 +117,7 @@ static int poisson(isf_rollout_t r, const float* fin, float* fout) {
 // pressure_project! in place keeping the gauge phase q (src/parallel.jl:155-160)
 static int project_q(isf_rollout_t r, float4* psi, float* q) {
     isf_t h = r->h;
-    LAUNCH_CHECK(launch_div_r2c(r->g, psi, P_of(r), h->tb, h->stream), "div(one_form(psi)) + poisson x forward");
+    LAUNCH_CHECK(launch_div_r2c(r->g, psi, r->f, P_of(r), h->tb, h->stream), "div(one_form(psi)) + poisson x forward");
     LAUNCH_CHECK(launch_poisson_plane(r->g, P_of(r), poisson_scale(r), h->tb, h->stream), "poisson plane");
     LAUNCH_CHECK(launch_c2r_x(r->g, P_of(r), 1, q, psi, nullptr, nullptr, h->tb, h->stream), "poisson x inverse + gauge");
     return ISF_OK;
@@ -542,7 +542,7 @@ int isf_vjp_pressure_project(isf_t h, const float* psi_in, float* psibar) {
     float4* out = vjp_psi(h);
     // forward again, keeping q and the output
     HIP_CHECK(hipMemcpyAsync(out, in, 16 * h->N, hipMemcpyDeviceToDevice, st), "copy");
-    LAUNCH_CHECK(launch_div_r2c(h->g, out, hP(h), h->tb, st), "div(one_form(psi)) + poisson x forward");
+    LAUNCH_CHECK(launch_div_r2c(h->g, out, vjp_f(h), hP(h), h->tb, st), "div(one_form(psi)) + poisson x forward");
     LAUNCH_CHECK(launch_poisson_plane(h->g, hP(h), sc, h->tb, st), "poisson plane");
     LAUNCH_CHECK(launch_c2r_x(h->g, hP(h), 1, vjp_q(h), out, nullptr, nullptr, h->tb, st), "poisson x inverse + gauge");
     // reverse

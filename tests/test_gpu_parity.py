@@ -145,6 +145,22 @@ def test_pointwise_and_stencils(isf_mod, n, L):
     assert rel_l2(dc.to_host(), dp.to_host()) < 1e-6
 
 
+# z planes of psi above 1 MB (nx ny > 65536): div(one_form) runs as its own z-marching
+# stencil into f before the R2C x pass instead of fused into it
+@pytest.mark.parametrize("n,L", [((512, 256, 16), (8, 4, 1)), ((256, 512, 16), (4, 8, 1))])
+def test_pressure_project_wide_planes(isf_mod, n, L):
+    print(f"[project wide] n={n}", flush=True)
+    isf = isf_mod.ISF(L, n, 0.1, 1 / 48)
+    d = np.array(isf.d, np.float64)
+    s = smooth_psi(n, L, 6)
+    dp = isf.psi_array(s)
+    isf.pressure_project_(dp)
+    fac, _ = O64.tables(n, L, 0.1, 1 / 48)
+    err = rel_l2(dp.to_host(), O64.pressure_project(s.astype(np.float64), fac, d))
+    print(f"  pressure_project rel L2 = {err:.3e}")
+    assert err < 2e-6
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_isjet_and_restrict(isf_mod, mode):
     n, L = (64, 32, 32), (4, 2, 2)
