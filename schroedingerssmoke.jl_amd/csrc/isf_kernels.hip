@@ -25,7 +25,7 @@
 namespace isf {
 
 template <int NX> struct XCfg {
-    static constexpr int E = DefaultE<NX>::value;
+    static constexpr int E = NX == 512 ? 16 : DefaultE<NX>::value;
     static constexpr int T = NX / E;
     static constexpr int LS = padded_len(NX);
 };
@@ -36,6 +36,29 @@ template <int M> struct C2rCfg {
     static constexpr int E = M == 128 ? 8 : DefaultE<M>::value;
     static constexpr int T = M / E;
 };
+
+// Tile of this block: x tile, y tile and z slab (member-major in batch mode).  Blocks are
+// mapped XCD-aware: the hardware deals consecutive block ids to the 8 XCDs in turn, so
+// tile t is given to id (t mod C) * 8 + t / C, C = blocks / 8; each XCD then walks one
+// contiguous range of tiles, and rows or cache lines shared by neighbouring tiles stay in
+// that XCD's L2.
+struct StTile {
+    int bx, by, bz;
+};
+__device__ __forceinline__ StTile stencil_tile() {
+    const unsigned T = gridDim.x * gridDim.y * gridDim.z;
+    unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if ((T & 7u) == 0u) {
+        const unsigned C = T >> 3;
+        lin = (lin & 7u) * C + (lin >> 3);
+    }
+    StTile t;
+    t.bx = (int)(lin % gridDim.x);
+    lin /= gridDim.x;
+    t.by = (int)(lin % gridDim.y);
+    t.bz = (int)(lin / gridDim.y);
+    return t;
+}
 
 // ======================================================================== Schroedinger x passes
 
@@ -48,8 +71,9 @@ __global__ void __launch_bounds__(2 * TY * XCfg<NX>::T)
 k_flow_x_fwd(const float4* __restrict__ psi, const float2* __restrict__ sa, const float2* __restrict__ sb,
              float2* __restrict__ S, int ny, int nz, const float2* __restrict__ tw, Batch B) {
     constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
+    const StTile tl = stencil_tile();  // x: y tile, y: z, z: member
     if (B.inst) {
-        const Inst& m = B.inst[blockIdx.z];
+        const Inst& m = B.inst[tl.bz];
         psi = m.psi;
         S = m.S;
         tw = m.tb.tw;
@@ -58,7 +82,7 @@ k_flow_x_fwd(const float4* __restrict__ psi, const float2* __restrict__ sa, cons
     __shared__ float2 lds[2 * TY * LS];
     __shared__ float2 twl[FftTw<NX, E>::SIZE];
     const int tid = threadIdx.x;
-    const int y0 = blockIdx.x * TY, z = blockIdx.y;
+    const int y0 = tl.bx * TY, z = tl.by;
     const size_t line0 = ((size_t)z * ny + y0) * NX;
     // the block's lines: all loads issued before the twiddle-table loads and the LDS stores
     constexpr int K = TY * NX / NT;
@@ -101,8 +125,9 @@ __global__ void __launch_bounds__(2 * TY * XCfg<NX>::T)
 k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, float2* __restrict__ oa, float2* __restrict__ ob,
              int ny, int nz, const float2* __restrict__ tw, Batch B) {
     constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
+    const StTile tl = stencil_tile();  // x: y tile, y: z, z: member
     if (B.inst) {
-        const Inst& m = B.inst[blockIdx.z];
+        const Inst& m = B.inst[tl.bz];
         psi = m.psi;
         S = m.S;
         tw = m.tb.tw;
@@ -111,7 +136,7 @@ k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, float2* __r
     __shared__ float2 lds[2 * TY * LS];
     __shared__ float2 twl[FftTw<NX, E>::SIZE];
     const int tid = threadIdx.x;
-    const int y0 = blockIdx.x * TY, z = blockIdx.y;
+    const int y0 = tl.bx * TY, z = tl.by;
     constexpr int K = 2 * NX * TY / NT;
     float2 v[K];
 #pragma unroll
@@ -341,8 +366,9 @@ template <int NX, int TY>
 __global__ void __launch_bounds__(TY * XCfg<NX / 2>::T)
 k_r2c_x(const float* __restrict__ f, float2* __restrict__ P, int ny, int nz, const float2* __restrict__ tw, Batch B) {
     constexpr int M = NX / 2;
+    const StTile tl = stencil_tile();  // x: y tile, y: z, z: member
     if (B.inst) {
-        const Inst& m = B.inst[blockIdx.z];
+        const Inst& m = B.inst[tl.bz];
         f = reinterpret_cast<const float*>(m.S);
         P = poisson_P(m);
         tw = m.tb.tw;
@@ -353,7 +379,7 @@ k_r2c_x(const float* __restrict__ f, float2* __restrict__ P, int ny, int nz, con
     __shared__ float2 twm[FftTw<M, E>::SIZE];
     __shared__ float2 twn[NX];
     const int tid = threadIdx.x;
-    const int y0 = blockIdx.x * TY, z = blockIdx.y;
+    const int y0 = tl.bx * TY, z = tl.by;
     const float2* src = reinterpret_cast<const float2*>(f + ((size_t)z * ny + y0) * NX);
     constexpr int K = TY * M / NT;
     float2 v[K];
@@ -468,8 +494,9 @@ __global__ void __launch_bounds__(TY * C2rCfg<NX / 2>::T)
 k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restrict__ psi, GridInfo g, JetParams jet,
         double* __restrict__ partials, const float2* __restrict__ tw, Batch B) {
     constexpr int M = NX / 2;
+    const StTile tl = stencil_tile();  // x: y tile, y: z, z: member
     if (B.inst) {
-        const Inst& m = B.inst[blockIdx.z];
+        const Inst& m = B.inst[tl.bz];
         P = poisson_P(m);
         psi = m.psi;
         g = m.g;
@@ -486,7 +513,7 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
     __shared__ float2 twn[NX];
     const int tid = threadIdx.x;
     const int ny = g.ny, nz = g.nz;
-    const int y0 = blockIdx.x * TY, z = blockIdx.y;
+    const int y0 = tl.bx * TY, z = tl.by;
     {
         constexpr int CNT = (M + 1) * TY, K = (CNT + NT - 1) / NT;
         float2 v[K];
@@ -556,7 +583,7 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
     }
     if (MODE != C2R_Q && partials) {
         const double s = block_sum<NT>(acc);
-        if (tid == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
+        if (tid == 0) partials[tl.by * gridDim.x + tl.bx] = s;  // the tile within its member
     }
 }
 
@@ -570,33 +597,12 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
 // psi row is fetched once per block (plus one halo row on each side in y and z) and
 // every forward-edge angle
 // v_a(c) = angle(conj psi1(c) psi1(c+e_a) + conj psi2(c) psi2(c+e_a)) (src/parallel.jl:100-113)
-// is evaluated once per cell (plus a thin halo).  Blocks are mapped XCD-aware: the
-// hardware deals consecutive block ids to the 8 XCDs in turn, so tile t is given to id
-// (t mod C) * 8 + t / C, C = blocks / 8; each XCD then walks one contiguous range of
-// tiles, and the halo rows shared by neighbouring tiles stay in that XCD's L2.
+// is evaluated once per cell (plus a thin halo).  Blocks are mapped XCD-aware
+// (stencil_tile), so the halo rows shared by neighbouring tiles stay in one L2.
 constexpr int ST_BY = 4, ST_TZ = 16;  // rows per block, longest z walk
 
 __device__ __forceinline__ float4 shfl4(float4 v, int src) {
     return make_float4(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64), __shfl(v.w, src, 64));
-}
-
-// Tile of this block: x tile, y tile and z slab (member-major in batch mode).
-struct StTile {
-    int bx, by, bz;
-};
-__device__ __forceinline__ StTile stencil_tile() {
-    const unsigned T = gridDim.x * gridDim.y * gridDim.z;
-    unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    if ((T & 7u) == 0u) {
-        const unsigned C = T >> 3;
-        lin = (lin & 7u) * C + (lin >> 3);
-    }
-    StTile t;
-    t.bx = (int)(lin % gridDim.x);
-    lin /= gridDim.x;
-    t.by = (int)(lin % gridDim.y);
-    t.bz = (int)(lin / gridDim.y);
-    return t;
 }
 
 // Fused velocity_one_form (hbar = 1) + div (src/parallel.jl:100-113,131-143,156-157):
@@ -1058,14 +1064,13 @@ bool plane_in_lds(int ny, int nz) { return ny <= 128 && nz <= 128; }
 // y lines per block of the x passes: g.ty when set (8, 16 or 32, not above ny; 32 only
 // for nx <= 256), else 16 (8 for nx = 512)
 int x_tile_lines(int nx, int ny) {
-    const int ty = nx >= 512 ? 8 : 16;
+    const int ty = 16;
     return ty < ny ? ty : ny;
 }
 static int x_tile(const GridInfo& g) {
     int ty = g.ty > 0 ? g.ty : x_tile_lines(g.nx, g.ny);
     if (ty > g.ny) ty = g.ny;
     if (ty == 32 && g.nx > 256) ty = 16;
-    if (g.nx >= 512) ty = 8;
     return ty;
 }
 
@@ -1125,6 +1130,7 @@ static int c2r_x_t(const GridInfo& g, const float2* P, int mode, float* q, float
             ISF_X_CASE(128, FN, g, __VA_ARGS__)                                                 \
             ISF_X_CASE(256, FN, g, __VA_ARGS__)                                                 \
             case 512:                                                                           \
+                if (ty == 16) return FN<512, 16>(g, __VA_ARGS__);                               \
                 return FN<512, 8>(g, __VA_ARGS__);                                              \
         }                                                                                       \
         return (int)hipErrorInvalidValue;                                                       \
