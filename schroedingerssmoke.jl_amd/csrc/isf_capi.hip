@@ -68,6 +68,7 @@ int isf::join(isf_t h) {
         hipError_t e = hipStreamWaitEvent(h->stream, h->ev_adv, 0);
         if (e != hipSuccess) return fail(ISF_EHIP, "hipStreamWaitEvent: %s", hipGetErrorString(e));
         h->side_pending = false;
+        h->vread_pending = false;  // ev_adv is recorded after the velocity pass
     }
     return ISF_OK;
 }
@@ -252,6 +253,8 @@ int isf_create(isf_t* out, const int res[3], const int box[3], float hbar, float
     if ((e = hipStreamCreate(&h->pstream)) != hipSuccess) return cleanup(hip_fail(e, "hipStreamCreate"));
     if ((e = hipEventCreateWithFlags(&h->ev_vel, hipEventDisableTiming)) != hipSuccess)
         return cleanup(hip_fail(e, "hipEventCreate"));
+    if ((e = hipEventCreateWithFlags(&h->ev_vread, hipEventDisableTiming)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipEventCreate"));
     if ((e = hipEventCreateWithFlags(&h->ev_adv, hipEventDisableTiming)) != hipSuccess)
         return cleanup(hip_fail(e, "hipEventCreate"));
     if ((e = hipEventCreateWithFlags(&h->ext_ev, hipEventDisableTiming)) != hipSuccess)
@@ -269,6 +272,7 @@ int isf_destroy(isf_t h) {
     if (h->pstream) (void)hipStreamSynchronize(h->pstream);
     if (h->ev_vel) (void)hipEventDestroy(h->ev_vel);
     if (h->ev_adv) (void)hipEventDestroy(h->ev_adv);
+    if (h->ev_vread) (void)hipEventDestroy(h->ev_vread);
     if (h->ext_ev) (void)hipEventDestroy(h->ext_ev);
     if (h->pstream) (void)hipStreamDestroy(h->pstream);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -517,20 +521,26 @@ static const char* kStageNames[] = {
     "velocity_sharp", "particle_order", "advect"};                               // :59-63
 static constexpr int kNumStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 
-// One simloop body (test/runtests.jl:45-64).  The grid part runs on the handle's stream;
-// the particle part (spawn, processing order, advect) runs on pstream: the spawn of step
-// k after the advect of step k-1, the advect of step k after the velocity of step k, and
-// the velocity of step k+1 after the advect of step k, so the latency-bound particle work
-// overlaps the next step's transforms.  If ev is non-null (profiling, kNumStages + 1
-// events) everything runs on the handle's stream in stage order, ev[0] recorded before
-// the first stage and ev[k + 1] after stage k.
+// One simloop body (test/runtests.jl:45-64).  The transforms and projections run on the
+// handle's stream; the velocity pass and the particle part (spawn, processing order,
+// advect) run on pstream, in the order velocity(k-1), order, advect(k-1), spawn(k): the
+// velocity of step k after the last projection of step k (event), and the inverse flow
+// pass of step k+1, the next write of psi, after that velocity pass (event).  The
+// bandwidth-bound velocity pass and the latency-bound particle work so overlap the next
+// step's forward flow transforms.  If ev is non-null (profiling, kNumStages + 1 events)
+// everything runs on the handle's stream in stage order, ev[0] recorded before the first
+// stage and ev[k + 1] after stage k.
 static int step_impl(isf_t h, float* psi, float* parts, long long nslots, int iter, hipEvent_t* ev) {
     const float t = (float)iter * h->dt;  // test/runtests.jl:46
     JetParams j = h->jet;
     j.omgterm = h->omega * t;             // test/runtests.jl:57
     float4* p = reinterpret_cast<float4*>(psi);
     hipStream_t s = h->stream;
-    const bool side = ev == nullptr && nslots > 0;
+    // Outside profiling the particle work and the velocity pass run on the side stream:
+    // the velocity field is read only by the advect, and the next step's forward flow pass
+    // only reads psi, so both overlap it; its inverse pass, the next write of psi, waits
+    // for the velocity pass (ev_vread).
+    const bool side = ev == nullptr;
     hipStream_t ps = side ? h->pstream : s;
     {
         int r = wait_ext(h);
@@ -562,10 +572,15 @@ static int step_impl(isf_t h, float* psi, float* parts, long long nslots, int it
         first0 = start1 == 0 ? nslots - 1 : start1 - 1;
         nsp = h->n_spawn;
     }
-    if (side) LAUNCH_CHECK(launch_spawn(parts, nslots, first0, nsp, h->jet, h->seed, (unsigned)iter, ps), "add_particles");
+    if (side && nslots > 0)
+        LAUNCH_CHECK(launch_spawn(parts, nslots, first0, nsp, h->jet, h->seed, (unsigned)iter, ps), "add_particles");
     // :48-49 schroedinger_flow! + normalize
     STAGE(launch_flow_x_fwd(h->g, p, h->S, h->tb, s), "flow x forward");
     STAGE(launch_flow_plane(h->g, h->S, h->tb, s), "flow plane");
+    if (h->vread_pending) {
+        HIP_CHECK(hipStreamWaitEvent(s, h->ev_vread, 0), "hipStreamWaitEvent");
+        h->vread_pending = false;
+    }
     STAGE(launch_flow_x_inv(h->g, h->S, p, true, h->tb, s), "flow x inverse + normalize");
     // :50 pressure_project! with the restrict of :57 fused into its gauge pass
     const double sc = poisson_scale(h);
@@ -577,24 +592,26 @@ static int step_impl(isf_t h, float* psi, float* parts, long long nslots, int it
     STAGE(launch_div_r2c(h->g, p, f_of(h), P_of(h), h->tb, s), "div(one_form(psi)) + poisson x forward");
     STAGE(launch_poisson_plane(h->g, P_of(h), sc, h->tb, s), "poisson plane");
     STAGE(launch_c2r_x(h->g, P_of(h), 1, nullptr, p, nullptr, h->norm_parts, h->tb, s), "poisson x inverse + gauge");
-    // :59-62 velocity_one_form!(isf, psi, hbar) and the in-place StaggeredSharp; the
-    // previous step's advect must be done reading vel
-    {
-        int r = join(h);
-        if (r) return r;
-    }
-    STAGE(launch_velocity(h->g, p, h->hbar, 1, h->vel, s), "velocity_one_form");
-    // :63 staggered_advect!, slots processed in a spatial order when the slot order is
-    // scattered (particles move less than a cell per step)
+    // :59-62 velocity_one_form!(isf, psi, hbar) and the in-place StaggeredSharp, after the
+    // previous step's advect is done reading vel (same side stream, or joined above)
     if (side) {
         HIP_CHECK(hipEventRecord(h->ev_vel, s), "hipEventRecord");
         HIP_CHECK(hipStreamWaitEvent(ps, h->ev_vel, 0), "hipStreamWaitEvent");
+        LAUNCH_CHECK(launch_velocity(h->g, p, h->hbar, 1, h->vel, ps), "velocity_one_form");
+        HIP_CHECK(hipEventRecord(h->ev_vread, ps), "hipEventRecord");
+        h->vread_pending = true;
+    } else {
+        STAGE(launch_velocity(h->g, p, h->hbar, 1, h->vel, s), "velocity_one_form");
     }
-    const int* order = nullptr;
-    int r = ensure_order(h, parts, nslots, iter, &order, ps);
-    if (r) return r;
-    if ((r = mark())) return r;
-    STAGE(launch_advect(h->g, parts, nslots, order, h->vel, h->dt, ps), "staggered_advect");
+    // :63 staggered_advect!, slots processed in a spatial order when the slot order is
+    // scattered (particles move less than a cell per step)
+    if (nslots > 0 || !side) {
+        const int* order = nullptr;
+        int r = ensure_order(h, parts, nslots, iter, &order, ps);
+        if (r) return r;
+        if ((r = mark())) return r;
+        STAGE(launch_advect(h->g, parts, nslots, order, h->vel, h->dt, ps), "staggered_advect");
+    }
     if (side) {
         HIP_CHECK(hipEventRecord(h->ev_adv, ps), "hipEventRecord");
         h->side_pending = true;

@@ -44,10 +44,11 @@ struct isf_handle {
     float hbar, dt;
     size_t N;
     hipStream_t stream;   // the handle's stream: every operator, the grid part of isf_step
-    hipStream_t pstream;  // particle side of isf_step (spawn, order, advect), joined by events
+    hipStream_t pstream;  // side of isf_step (velocity, spawn, order, advect), joined by events
     hipEvent_t ev0, ev1;
-    hipEvent_t ev_vel, ev_adv;
-    bool side_pending;    // an advect on pstream may still read vel / write particles
+    hipEvent_t ev_vel, ev_adv, ev_vread;
+    bool side_pending;    // side work of isf_step may still read psi / vel or write particles
+    bool vread_pending;   // the velocity pass on pstream may still read psi (ev_vread)
     hipEvent_t ext_ev;    // recorded on the handle's stream when a batched step must wait for it
     hipEvent_t batch_ev;  // (borrowed) end of the last batched step that used this handle's data
     bool ext_pending;     // the handle's streams must wait for batch_ev before new work

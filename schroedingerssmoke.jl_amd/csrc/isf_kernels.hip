@@ -1062,7 +1062,7 @@ bool plane_supported(int ny, int nz) { return yz_len_ok(ny) && yz_len_ok(nz); }
 bool plane_in_lds(int ny, int nz) { return ny <= 128 && nz <= 128; }
 
 // y lines per block of the x passes: g.ty when set (8, 16 or 32, not above ny; 32 only
-// for nx <= 256), else 16 (8 for nx = 512)
+// for nx <= 256), else 16
 int x_tile_lines(int nx, int ny) {
     const int ty = 16;
     return ty < ny ? ty : ny;
