@@ -200,9 +200,9 @@ int isf_mp_staggered_advect(isf_t h, isf_particles_t p, const float* vx, const f
 
 /* ---- asynchronous particle export (examples/jet.jl:83, Array(particles) per frame) ------ */
 /* Double-buffered copies of the particle positions (nslots x 3 floats) into pinned host
- * memory.  push stages the positions on the handle's stream after all queued work and
- * copies them to the host on a separate stream, so later steps do not wait for the PCIe
- * transfer.  previous returns push k-1 after push k (waiting for its copy if needed);
+ * memory.  push stages the positions after all work queued on the handle, on the stream
+ * where isf_step advects particles, and copies them to the host on a separate stream, so
+ * later steps wait neither for the staging copy nor for the PCIe transfer.  previous returns push k-1 after push k (waiting for its copy if needed);
  * latest waits for the last push.  A returned buffer stays valid until the next push. */
 typedef struct isf_export* isf_export_t;
 int isf_export_create(isf_export_t* out, isf_t h, long long nslots);

@@ -60,6 +60,15 @@ static int wait_ext(isf_t h) {
     }
     return ISF_OK;
 }
+// Order the side stream after all work queued on the handle's stream (ev_vel is recorded
+// afresh wherever the side stream must follow the handle's stream).
+int isf::side_after_main(isf_t h) {
+    int r = wait_ext(h);
+    if (r) return r;
+    HIP_CHECK(hipEventRecord(h->ev_vel, h->stream), "hipEventRecord");
+    HIP_CHECK(hipStreamWaitEvent(h->pstream, h->ev_vel, 0), "hipStreamWaitEvent");
+    return ISF_OK;
+}
 // Order the handle's stream after a pending particle-side advect of isf_step.
 int isf::join(isf_t h) {
     int r = wait_ext(h);

@@ -2,11 +2,13 @@
 // memory, the device-side form of the visualisation driver's per-frame copy
 // `Array(particles)` (examples/jet.jl:83).
 //
-// push(k): on the handle's stream, after every step queued so far, the positions are
-// copied device-to-device into staging buffer k mod 2 (the stream first waits until the
-// host copy of push k-2 has drained that buffer); a second stream then copies the staging
-// buffer into pinned host buffer k mod 2.  Later steps therefore wait only for the
-// device-to-device copy, never for the PCIe transfer.  previous() waits for the host copy
+// push(k): on the handle's side stream (where isf_step advects particles), after all work
+// queued on the handle so far, the positions are copied device-to-device into staging
+// buffer k mod 2 (the stream first waits until the host copy of push k-2 has drained that
+// buffer); a second stream then copies the staging buffer into pinned host buffer k mod 2.
+// The handle's stream, which carries the grid part of later steps, waits for neither copy;
+// work that writes particles after the push is ordered after the staging copy like after
+// an advect (join).  previous() waits for the host copy
 // of push k-1 and returns its buffer, valid until the next push.
 #include <hip/hip_runtime.h>
 
@@ -89,11 +91,13 @@ int isf_export_push(isf_export_t e, const float* parts) {
     if (r) return r;
     CHECK_PTR(parts, "parts");
     isf_t h = e->h;
-    if ((r = join(h))) return r;  // after the side stream's advect of the last isf_step
+    if ((r = side_after_main(h))) return r;  // the side stream already follows the last advect
     const int b = (int)(e->pushed & 1);
-    HIP_CHECK(hipStreamWaitEvent(h->stream, e->drained[b], 0), "hipStreamWaitEvent");
-    HIP_CHECK(hipMemcpyAsync(e->stage[b], parts, e->bytes, hipMemcpyDeviceToDevice, h->stream), "export stage copy");
-    HIP_CHECK(hipEventRecord(e->staged[b], h->stream), "hipEventRecord");
+    HIP_CHECK(hipStreamWaitEvent(h->pstream, e->drained[b], 0), "hipStreamWaitEvent");
+    HIP_CHECK(hipMemcpyAsync(e->stage[b], parts, e->bytes, hipMemcpyDeviceToDevice, h->pstream), "export stage copy");
+    HIP_CHECK(hipEventRecord(e->staged[b], h->pstream), "hipEventRecord");
+    HIP_CHECK(hipEventRecord(h->ev_adv, h->pstream), "hipEventRecord");
+    h->side_pending = true;
     HIP_CHECK(hipStreamWaitEvent(e->copy, e->staged[b], 0), "hipStreamWaitEvent");
     HIP_CHECK(hipMemcpyAsync(e->host[b], e->stage[b], e->bytes, hipMemcpyDeviceToHost, e->copy), "export host copy");
     HIP_CHECK(hipEventRecord(e->drained[b], e->copy), "hipEventRecord");

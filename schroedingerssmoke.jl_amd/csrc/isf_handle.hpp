@@ -101,4 +101,6 @@ double poisson_scale(isf_t h);
 int project(isf_t h, float* psi, int c2r_mode, const JetParams* jet, double* parts);
 // order the handle's streams after its pending side-stream and batched work
 int join(isf_t h);
+// order the side stream (pstream) after all work queued on the handle's stream
+int side_after_main(isf_t h);
 }  // namespace isf
