@@ -124,6 +124,28 @@ def test_batched_member_matches_oracle(ens):
     b.close()
 
 
+def test_tiles_give_identical_results_through_isf_step(ens, tmp_path):
+    """Every x tile the autotuner can choose (8, 16, 32 lines per block), applied to the
+    member handles through the tile cache file and stepped with isf_step one member after
+    another, gives the same psi and particles within FP32 rounding."""
+    ms = members(ens, 4, seed=14, grids=((64, 32, 32),))
+    results = {}
+    for forced in (8, 16, 32):
+        path = str(tmp_path / f"tile{forced}.txt")
+        with open(path, "w") as f:
+            f.write(f"64 32 32 4 {forced}\n")
+        b = ens.build_batch(ms, n_init=500)
+        b.tune(path)
+        assert b.tile((64, 32, 32)) == forced
+        print(f"[tiles via isf_step] tile {forced}", flush=True)
+        for it in range(1, 4):
+            b.step_sequential_(it)
+        results[forced] = state(b)
+        b.close()
+    assert_close(results[8], results[16], "tile 8 vs 16")
+    assert_close(results[8], results[32], "tile 8 vs 32")
+
+
 @pytest.mark.skip(reason="aborted on the host (SIGABRT inside isf_batch_step, no GPU fault reported) in one run "
                          "of the suite; the cause has not been found, so the case stays off the GPU until it is")
 def test_autotuned_tiles_give_identical_results(ens, tmp_path):
