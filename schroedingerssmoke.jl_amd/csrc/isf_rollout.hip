@@ -9,7 +9,9 @@
 // inputs can be kept) and psi / particles are checkpointed every K steps.  Backward:
 // segments are processed last to first; each is recomputed from its checkpoint keeping
 // the per-step tape (step input psi0, flow output psi1, first gauge output psi3, the two
-// gauge phases q1 q2, particles after the spawn) and then reversed step by step.
+// gauge phases q1 q2, particles after the spawn) and then reversed step by step.  The
+// recompute of segment s-1 runs on a second stream, with its own scratch and the other of
+// two tape sets, while segment s is reversed on the handle's stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,6 +25,21 @@
 using namespace isf;
 
 static constexpr uint32_t kRolloutMagic = 0x2011A0D7u;
+
+// Per-step tape of one segment: step outputs inside the segment (mid), flow output psi1,
+// first gauge output psi3, gauge phases q1 q2, particles after the spawn p0.
+struct Tape {
+    std::vector<float4*> mid, psi1, psi3;
+    std::vector<float*> q1, q2, p0;
+};
+// Scratch and stream of one chain of forward steps.
+struct Work {
+    float2* S;     // spectrum (16N bytes); f at byte 0 and P at byte 4N in projections
+    float* f;      // div scratch of the wide-plane projection path
+    float* vel;    // velocity field
+    float* parts;  // particles being stepped
+    hipStream_t st;
+};
 
 struct isf_rollout {
     uint32_t magic;
@@ -43,9 +60,12 @@ struct isf_rollout {
     // checkpoints at steps 0, K, 2K, ... (ck_psi[0] = cpsi[C], ck_parts[0] = parts_init)
     std::vector<float4*> ck_psi;
     std::vector<float*> ck_parts;
-    // segment tape
-    std::vector<float4*> t_mid, t_psi1, t_psi3;
-    std::vector<float*> t_q1, t_q2, t_p0;
+    // segment tapes: set s % 2 holds segment s during the backward
+    Tape tape[2];
+    // recompute stream of the backward with its own scratch (spectrum / Poisson, f,
+    // velocity, particles); rec_done[i] / adj_done[i]: tape set i recomputed / reversed
+    Work rw;
+    hipEvent_t rec_done[2], adj_done[2], start;
     // work
     float4 *W, *psibar;
     float *f, *fbar, *vel, *velbar, *parts, *parts_final, *pbar, *phibar, *target, *resid;
@@ -73,6 +93,13 @@ static int check_rollout(isf_rollout_t r) {
 
 static void rollout_free(isf_rollout_t r) {
     if (r->h && r->h->stream) (void)hipStreamSynchronize(r->h->stream);
+    if (r->rw.st) (void)hipStreamSynchronize(r->rw.st);
+    for (int i = 0; i < 2; i++) {
+        if (r->rec_done[i]) (void)hipEventDestroy(r->rec_done[i]);
+        if (r->adj_done[i]) (void)hipEventDestroy(r->adj_done[i]);
+    }
+    if (r->start) (void)hipEventDestroy(r->start);
+    if (r->rw.st) (void)hipStreamDestroy(r->rw.st);
     for (void* p : r->allocs) (void)hipFree(p);
     r->magic = 0;
     delete r;
@@ -101,9 +128,12 @@ static float driver_omega(const float v[3], float hbar) {
 
 // ------------------------------------------------------------------ forward pieces
 
-static float2* P_of(isf_rollout_t r) {
-    return reinterpret_cast<float2*>(reinterpret_cast<char*>(r->h->S) + 4 * r->N);
+static float2* P_in(isf_rollout_t r, float2* S) {
+    return reinterpret_cast<float2*>(reinterpret_cast<char*>(S) + 4 * r->N);
 }
+static float2* P_of(isf_rollout_t r) { return P_in(r, r->h->S); }
+// the handle's stream with the handle's scratch and the rollout's work buffers
+static Work main_work(isf_rollout_t r) { return Work{r->h->S, r->f, r->vel, r->parts, r->h->stream}; }
 static double poisson_scale(isf_rollout_t r) { return -0.25 * 2.0 / (double)r->N; }
 
 // fout = Poisson(fin) (src/parallel.jl:168-174)
@@ -115,19 +145,20 @@ static int poisson(isf_rollout_t r, const float* fin, float* fout) {
     return ISF_OK;
 }
 // pressure_project! in place keeping the gauge phase q (src/parallel.jl:155-160)
-static int project_q(isf_rollout_t r, float4* psi, float* q) {
+static int project_q(isf_rollout_t r, float4* psi, float* q, const Work& w) {
     isf_t h = r->h;
-    LAUNCH_CHECK(launch_div_r2c(r->g, psi, r->f, P_of(r), h->tb, h->stream), "div(one_form(psi)) + poisson x forward");
-    LAUNCH_CHECK(launch_poisson_plane(r->g, P_of(r), poisson_scale(r), h->tb, h->stream), "poisson plane");
-    LAUNCH_CHECK(launch_c2r_x(r->g, P_of(r), 1, q, psi, nullptr, nullptr, h->tb, h->stream), "poisson x inverse + gauge");
+    float2* P = P_in(r, w.S);
+    LAUNCH_CHECK(launch_div_r2c(r->g, psi, w.f, P, h->tb, w.st), "div(one_form(psi)) + poisson x forward");
+    LAUNCH_CHECK(launch_poisson_plane(r->g, P, poisson_scale(r), h->tb, w.st), "poisson plane");
+    LAUNCH_CHECK(launch_c2r_x(r->g, P, 1, q, psi, nullptr, nullptr, h->tb, w.st), "poisson x inverse + gauge");
     return ISF_OK;
 }
 
 static int seg_steps(isf_rollout_t r, int s) { return std::min(r->K, r->T - s * r->K); }
-static float4* psi0_of(isf_rollout_t r, int s, int k) {
+static float4* psi0_of(isf_rollout_t r, const Tape& tp, int s, int k) {
     if (k == 0) return r->ck_psi[s];
     if (k == seg_steps(r, s)) return r->ck_psi[s + 1];
-    return r->t_mid[k - 1];
+    return tp.mid[k - 1];
 }
 static JetParams jet_at(isf_rollout_t r, int it) {
     JetParams j = r->jet;
@@ -143,58 +174,61 @@ static void spawn_slots(isf_rollout_t r, int it, long long* first0, long long* n
     *n = r->h->n_spawn;
 }
 
-// Steps of segment s from its checkpoint, keeping the tape.
-static int seg_forward(isf_rollout_t r, int s) {
+// Steps of segment s from its checkpoint into tape tp, on w's stream and scratch.  The
+// forward (checkpoint = true) also writes the next checkpoint; a recompute writes only
+// the tape (its last step output is the next psi checkpoint again, rewritten with the
+// same values).
+static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bool checkpoint) {
     isf_t h = r->h;
-    hipStream_t st = h->stream;
+    hipStream_t st = w.st;
     const size_t pb = sizeof(float) * 3 * (size_t)r->P;
     if (r->P > 0)
-        HIP_CHECK(hipMemcpyAsync(r->parts, r->ck_parts[s], pb, hipMemcpyDeviceToDevice, st), "particles checkpoint");
+        HIP_CHECK(hipMemcpyAsync(w.parts, r->ck_parts[s], pb, hipMemcpyDeviceToDevice, st), "particles checkpoint");
     const int ks = seg_steps(r, s);
     for (int k = 0; k < ks; k++) {
         const int it = s * r->K + k + 1;
-        const float4* in = psi0_of(r, s, k);
-        float4* out = psi0_of(r, s, k + 1);
+        const float4* in = psi0_of(r, tp, s, k);
+        float4* out = psi0_of(r, tp, s, k + 1);
         long long first0, nsp;
         spawn_slots(r, it, &first0, &nsp);
         if (nsp > 0) {
-            LAUNCH_CHECK(launch_spawn(r->parts, r->P, first0, nsp, h->jet, h->seed, (unsigned)it, st), "add_particles");
+            LAUNCH_CHECK(launch_spawn(w.parts, r->P, first0, nsp, h->jet, h->seed, (unsigned)it, st), "add_particles");
         }
         if (r->P > 0)
-            HIP_CHECK(hipMemcpyAsync(r->t_p0[k], r->parts, pb, hipMemcpyDeviceToDevice, st), "particles tape");
-        LAUNCH_CHECK(launch_flow_x_fwd(r->g, in, h->S, h->tb, st), "flow x forward");
-        LAUNCH_CHECK(launch_flow_plane(r->g, h->S, h->tb, st), "flow plane");
-        LAUNCH_CHECK(launch_flow_x_inv(r->g, h->S, r->t_psi1[k], false, h->tb, st), "flow x inverse");
-        LAUNCH_CHECK(launch_normalize_to(r->g, r->t_psi1[k], r->t_psi3[k], st), "normalize");
-        int e = project_q(r, r->t_psi3[k], r->t_q1[k]);
+            HIP_CHECK(hipMemcpyAsync(tp.p0[k], w.parts, pb, hipMemcpyDeviceToDevice, st), "particles tape");
+        LAUNCH_CHECK(launch_flow_x_fwd(r->g, in, w.S, h->tb, st), "flow x forward");
+        LAUNCH_CHECK(launch_flow_plane(r->g, w.S, h->tb, st), "flow plane");
+        LAUNCH_CHECK(launch_flow_x_inv(r->g, w.S, tp.psi1[k], false, h->tb, st), "flow x inverse");
+        LAUNCH_CHECK(launch_normalize_to(r->g, tp.psi1[k], tp.psi3[k], st), "normalize");
+        int e = project_q(r, tp.psi3[k], tp.q1[k], w);
         if (e) return e;
-        LAUNCH_CHECK(launch_restrict_to(r->g, r->t_psi3[k], out, jet_at(r, it), st), "restrict_velocity");
-        if ((e = project_q(r, out, r->t_q2[k]))) return e;
-        LAUNCH_CHECK(launch_velocity(r->g, out, h->hbar, 1, r->vel, st), "velocity_one_form");
-        if (r->P > 0) LAUNCH_CHECK(launch_advect(r->g, r->parts, r->P, nullptr, r->vel, h->dt, st), "staggered_advect");
+        LAUNCH_CHECK(launch_restrict_to(r->g, tp.psi3[k], out, jet_at(r, it), st), "restrict_velocity");
+        if ((e = project_q(r, out, tp.q2[k], w))) return e;
+        LAUNCH_CHECK(launch_velocity(r->g, out, h->hbar, 1, w.vel, st), "velocity_one_form");
+        if (r->P > 0) LAUNCH_CHECK(launch_advect(r->g, w.parts, r->P, nullptr, w.vel, h->dt, st), "staggered_advect");
     }
-    if (s + 1 < r->nseg && r->P > 0)
-        HIP_CHECK(hipMemcpyAsync(r->ck_parts[s + 1], r->parts, pb, hipMemcpyDeviceToDevice, st), "particles checkpoint");
+    if (checkpoint && s + 1 < r->nseg && r->P > 0)
+        HIP_CHECK(hipMemcpyAsync(r->ck_parts[s + 1], w.parts, pb, hipMemcpyDeviceToDevice, st), "particles checkpoint");
     return ISF_OK;
 }
 
-// Reverse of step k of segment s (the tape of segment s is current).
-static int step_adjoint(isf_rollout_t r, int s, int k) {
+// Reverse of step k of segment s from its tape tp, on the handle's stream.
+static int step_adjoint(isf_rollout_t r, const Tape& tp, int s, int k) {
     isf_t h = r->h;
     hipStream_t st = h->stream;
     const GridInfo& g = r->g;
     const int it = s * r->K + k + 1;
-    const float4* psi5 = psi0_of(r, s, k + 1);
+    const float4* psi5 = psi0_of(r, tp, s, k + 1);
     const JetParams j = jet_at(r, it);
     // :59-63 velocity and advection
     LAUNCH_CHECK(launch_velocity(g, psi5, h->hbar, 1, r->vel, st), "velocity_one_form");
     LAUNCH_CHECK(launch_fill_f32(r->velbar, 0.0f, 3 * r->N, st), "fill");
     if (r->P > 0) {
         // cell-sorted slot order: long same-cell runs for the scatter's wavefront reduction
-        LAUNCH_CHECK(launch_cell_order(g, r->t_p0[k], r->P, r->adj_order, r->adj_order_scratch,
+        LAUNCH_CHECK(launch_cell_order(g, tp.p0[k], r->P, r->adj_order, r->adj_order_scratch,
                                                  r->adj_order_bytes, st),
                      "particle order");
-        LAUNCH_CHECK(launch_advect_adj(g, r->t_p0[k], r->P, r->adj_order, r->vel, h->dt, r->pbar, r->velbar, st),
+        LAUNCH_CHECK(launch_advect_adj(g, tp.p0[k], r->P, r->adj_order, r->vel, h->dt, r->pbar, r->velbar, st),
                      "advect adjoint");
         long long first0, nsp;
         spawn_slots(r, it, &first0, &nsp);
@@ -205,18 +239,18 @@ static int step_adjoint(isf_rollout_t r, int s, int k) {
     LAUNCH_CHECK(launch_gauge_qbar(g, r->psibar, psi5, r->f, st), "gauge adjoint");
     int e = poisson(r, r->f, r->fbar);
     if (e) return e;
-    LAUNCH_CHECK(launch_restrict_to(g, r->t_psi3[k], r->W, j, st), "restrict_velocity");
-    LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, r->W, r->t_q2[k], r->fbar, st), "projection adjoint");
+    LAUNCH_CHECK(launch_restrict_to(g, tp.psi3[k], r->W, j, st), "restrict_velocity");
+    LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, r->W, tp.q2[k], r->fbar, st), "projection adjoint");
     // :57 jet constraint
-    LAUNCH_CHECK(launch_restrict_adj(g, r->psibar, r->t_psi3[k], j, r->partials, r->vacc + 4 * (size_t)(it - 1), st),
+    LAUNCH_CHECK(launch_restrict_adj(g, r->psibar, tp.psi3[k], j, r->partials, r->vacc + 4 * (size_t)(it - 1), st),
                  "restrict adjoint");
     // :50 first projection (gauge output psi3, one-form at psi2 = normalize(psi1))
-    LAUNCH_CHECK(launch_gauge_qbar(g, r->psibar, r->t_psi3[k], r->f, st), "gauge adjoint");
+    LAUNCH_CHECK(launch_gauge_qbar(g, r->psibar, tp.psi3[k], r->f, st), "gauge adjoint");
     if ((e = poisson(r, r->f, r->fbar))) return e;
-    LAUNCH_CHECK(launch_normalize_to(g, r->t_psi1[k], r->W, st), "normalize");
-    LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, r->W, r->t_q1[k], r->fbar, st), "projection adjoint");
+    LAUNCH_CHECK(launch_normalize_to(g, tp.psi1[k], r->W, st), "normalize");
+    LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, r->W, tp.q1[k], r->fbar, st), "projection adjoint");
     // :49 normalize, :48 schroedinger_flow
-    LAUNCH_CHECK(launch_normalize_adj(g, r->psibar, r->t_psi1[k], st), "normalize adjoint");
+    LAUNCH_CHECK(launch_normalize_adj(g, r->psibar, tp.psi1[k], st), "normalize adjoint");
     LAUNCH_CHECK(launch_flow_x_fwd(g, r->psibar, h->S, h->tb, st), "flow adjoint x forward");
     LAUNCH_CHECK(launch_flow_plane_adj(g, h->S, h->tb, st), "flow adjoint plane");
     LAUNCH_CHECK(launch_flow_x_inv(g, h->S, r->psibar, false, h->tb, st), "flow adjoint x inverse");
@@ -292,20 +326,26 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
     for (int s = 1; s <= r->nseg; s++) RA(&r->ck_psi[s], N);
     for (int s = 1; s < r->nseg; s++) RA(&r->ck_parts[s], P3);
     r->ck_parts[r->nseg] = nullptr;
-    r->t_mid.resize(r->K - 1);
-    for (auto& p : r->t_mid) RA(&p, N);
-    r->t_psi1.resize(r->K);
-    r->t_psi3.resize(r->K);
-    r->t_q1.resize(r->K);
-    r->t_q2.resize(r->K);
-    r->t_p0.resize(r->K);
-    for (int k = 0; k < r->K; k++) {
-        RA(&r->t_psi1[k], N);
-        RA(&r->t_psi3[k], N);
-        RA(&r->t_q1[k], N);
-        RA(&r->t_q2[k], N);
-        RA(&r->t_p0[k], P3);
+    for (Tape& tp : r->tape) {
+        tp.mid.resize(r->K - 1);
+        for (auto& p : tp.mid) RA(&p, N);
+        tp.psi1.resize(r->K);
+        tp.psi3.resize(r->K);
+        tp.q1.resize(r->K);
+        tp.q2.resize(r->K);
+        tp.p0.resize(r->K);
+        for (int k = 0; k < r->K; k++) {
+            RA(&tp.psi1[k], N);
+            RA(&tp.psi3[k], N);
+            RA(&tp.q1[k], N);
+            RA(&tp.q2[k], N);
+            RA(&tp.p0[k], P3);
+        }
     }
+    RA(&r->rw.S, 2 * N);
+    RA(&r->rw.f, N);
+    RA(&r->rw.vel, 3 * N);
+    RA(&r->rw.parts, P3);
     RA(&r->W, N);
     RA(&r->psibar, N);
     RA(&r->f, N);
@@ -329,6 +369,15 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
         RA(&r->adj_order_scratch, r->adj_order_bytes);
     }
 #undef RA
+    {
+        hipError_t he = hipStreamCreate(&r->rw.st);
+        for (int i = 0; i < 2 && he == hipSuccess; i++) {
+            he = hipEventCreateWithFlags(&r->rec_done[i], hipEventDisableTiming);
+            if (he == hipSuccess) he = hipEventCreateWithFlags(&r->adj_done[i], hipEventDisableTiming);
+        }
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&r->start, hipEventDisableTiming);
+        if (he != hipSuccess) return bail(hip_fail(he, "rollout streams"));
+    }
     hipStream_t st = h->stream;
     if ((e = join(h))) return bail(e);
     if (launch_fill_f32(r->target, 0.0f, (size_t)r->M, st)) return bail(fail(ISF_EHIP, "fill"));
@@ -393,12 +442,13 @@ int isf_rollout_forward(isf_rollout_t r, const float velocity[3], const float* t
         LAUNCH_CHECK(launch_apply_phase(r->g, r->cpsi[0], r->theta, r->d.phase_res, st), "apply phase");
     }
     // test/runtests.jl:27-30
+    const Work mw = main_work(r);
     for (int j = 0; j < r->C; j++) {
         LAUNCH_CHECK(launch_restrict_to(r->g, r->cpsi[j], r->cpsi[j + 1], jet_at(r, 0), st), "restrict_velocity");
-        if ((e = project_q(r, r->cpsi[j + 1], r->cq[j]))) return e;
+        if ((e = project_q(r, r->cpsi[j + 1], r->cq[j], mw))) return e;
     }
     for (int s = 0; s < r->nseg; s++)
-        if ((e = seg_forward(r, s))) return e;
+        if ((e = seg_forward(r, s, r->tape[0], mw, true))) return e;
     if (r->P > 0)
         HIP_CHECK(hipMemcpyAsync(r->parts_final, r->parts, sizeof(float) * 3 * r->P, hipMemcpyDeviceToDevice, st),
                   "copy");
@@ -426,10 +476,27 @@ int isf_rollout_backward(isf_rollout_t r, double grad_velocity[3], float* grad_t
     LAUNCH_CHECK(launch_fill_psi(r->psibar, make_float4(0.0f, 0.0f, 0.0f, 0.0f), r->N, st), "fill");
     LAUNCH_CHECK(launch_fill_f64(r->vacc, 0.0, 4 * (size_t)(r->T + r->C), st), "fill");
     LAUNCH_CHECK(launch_splat_adj(r->parts_final, r->P, r->sc, r->resid, r->pbar, st), "loss adjoint");
+    // segments last to first: segment s is recomputed into tape set s % 2 on the recompute
+    // stream (after the adjoint of segment s + 2 has released that set) and reversed on
+    // the handle's stream, so the recompute of s - 1 overlaps the reverse of s
+    HIP_CHECK(hipEventRecord(r->start, st), "hipEventRecord");
+    HIP_CHECK(hipStreamWaitEvent(r->rw.st, r->start, 0), "hipStreamWaitEvent");
+    auto recompute = [&](int s) -> int {
+        const int b = s % 2;
+        if (s + 2 < r->nseg) HIP_CHECK(hipStreamWaitEvent(r->rw.st, r->adj_done[b], 0), "hipStreamWaitEvent");
+        int e2 = seg_forward(r, s, r->tape[b], r->rw, false);
+        if (e2) return e2;
+        HIP_CHECK(hipEventRecord(r->rec_done[b], r->rw.st), "hipEventRecord");
+        return ISF_OK;
+    };
+    if ((e = recompute(r->nseg - 1))) return e;
     for (int s = r->nseg - 1; s >= 0; s--) {
-        if ((e = seg_forward(r, s))) return e;
+        const int b = s % 2;
+        HIP_CHECK(hipStreamWaitEvent(st, r->rec_done[b], 0), "hipStreamWaitEvent");
+        if (s > 0 && (e = recompute(s - 1))) return e;
         for (int k = seg_steps(r, s) - 1; k >= 0; k--)
-            if ((e = step_adjoint(r, s, k))) return e;
+            if ((e = step_adjoint(r, r->tape[b], s, k))) return e;
+        HIP_CHECK(hipEventRecord(r->adj_done[b], st), "hipEventRecord");
     }
     // set-up constraint iterations in reverse
     for (int j = r->C - 1; j >= 0; j--) {
