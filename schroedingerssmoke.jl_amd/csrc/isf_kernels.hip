@@ -982,7 +982,11 @@ k_advect(float* __restrict__ parts, long long nslots, const int* __restrict__ or
 // Draw j of the batch uses Philox4x32-10 with counter (j, step, 0, 0).  Each chunk of
 // 1024 consecutive draws is stored in the chunk's slots first + chunk*1024 + k (mod
 // nslots) in increasing order of key = (top 22 bits of raw word 0, i.e. the ring angle)
-// << 10 | (j mod 1024), so neighbouring slots hold neighbouring points of the ring.
+// << 10 | (j mod 1024), so neighbouring slots hold neighbouring points of the ring.  A
+// block sorts blockDim.x keys (a power of two from 64 to 1024, at least the draw count
+// when that is below 1024), padded with keys that sort last: with fewer than 1024 draws
+// the order is the same as with 1024-key blocks, and at most 64 draws sort within one
+// wavefront, without LDS stages.
 constexpr int SPAWN_CHUNK = 1024;
 
 __device__ __forceinline__ void spawn_draw(uint32_t j, unsigned step, unsigned long long seed, uint32_t raw[4]) {
@@ -1010,10 +1014,10 @@ k_spawn(float* __restrict__ parts, long long nslots, long long first, long long 
         const long long start1 = nslots > 0 ? ((long long)(it - 1) * m.n_spawn + 1) % nslots : 0;
         first = start1 == 0 ? nslots - 1 : start1 - 1;
     }
-    const int t = threadIdx.x;
-    const long long base = (long long)blockIdx.x * SPAWN_CHUNK;
+    const int t = threadIdx.x, C = blockDim.x;
+    const long long base = (long long)blockIdx.x * C;
     if (base >= n) return;  // uniform per block (batch members with fewer draws)
-    const int m = (int)min((long long)SPAWN_CHUNK, n - base);
+    const int m = (int)min((long long)C, n - base);
     uint32_t raw[4];
     uint32_t kv = 0xFFFFFFFFu;  // unused entries sort last
     if (t < m) {
@@ -1022,7 +1026,7 @@ k_spawn(float* __restrict__ parts, long long nslots, long long first, long long 
     }
     // bitonic sort of the 1024 keys, one key per thread: compare-exchange partners closer
     // than a wavefront trade through cross-lane shuffles, the rest through LDS
-    for (int size = 2; size <= SPAWN_CHUNK; size <<= 1) {
+    for (int size = 2; size <= C; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
             uint32_t other;
             if (stride >= 64) {
@@ -1038,7 +1042,7 @@ k_spawn(float* __restrict__ parts, long long nslots, long long first, long long 
         }
     }
     if (t >= m) return;
-    const uint32_t j = (uint32_t)base + (kv & (SPAWN_CHUNK - 1));
+    const uint32_t j = (uint32_t)base + (kv & (SPAWN_CHUNK - 1));  // t < C <= 1024
     spawn_draw(j, step, seed, raw);
     float u[4];
 #pragma unroll
@@ -1349,8 +1353,10 @@ int launch_spawn(float* parts, long long nslots, long long first, long long n, c
                  unsigned long long seed, unsigned step, hipStream_t s, Batch b, long long max_n) {
     const long long nn = b.inst ? max_n : n;
     if (nn <= 0) return 0;
-    const dim3 grid((unsigned)((nn + SPAWN_CHUNK - 1) / SPAWN_CHUNK), members_of(b));
-    ISF_LAUNCH(k_spawn, grid, dim3(SPAWN_CHUNK), s, parts, nslots, first, n, jet, seed, step, b);
+    int chunk = 64;  // the sort size: the draw count rounded up to a power of two, 64..1024
+    while (chunk < SPAWN_CHUNK && chunk < nn) chunk <<= 1;
+    const dim3 grid((unsigned)((nn + chunk - 1) / chunk), members_of(b));
+    ISF_LAUNCH(k_spawn, grid, dim3(chunk), s, parts, nslots, first, n, jet, seed, step, b);
 }
 // batch mode: max_slots = the largest nslots of the members (grid size)
 int launch_advect(const GridInfo& g, float* parts, long long nslots, const int* order, const float* vel, float dt,
