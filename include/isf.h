@@ -14,8 +14,9 @@
  *       f, q      1 float per cell
  *       isjet     1 byte per cell
  *       particles 3 floats per slot  {x, y, z}              NTuple{3,Float32}
- *   - All work is enqueued asynchronously on the handle's HIP stream; isf_synchronize
- *     waits for it (GPUArrays.synchronize, test/runtests.jl:65).
+ *   - All work is enqueued asynchronously on the handle's HIP stream (isf_step also uses a
+ *     second internal stream, ordered by events); isf_synchronize waits for both
+ *     (GPUArrays.synchronize, test/runtests.jl:65).
  *   - One host thread per handle.
  */
 #ifndef ISF_H
