@@ -204,6 +204,15 @@ template <int NY, int NZ> struct PlaneCfg {
 
 // PLANE_FLOW_ADJ: the adjoint of PLANE_FLOW, conj(mask) (the adjoint of F^-1 M F with an
 // unnormalised forward transform F is F^-1 conj(M) F).
+
+// Orders a wavefront's LDS accesses before the call before those after it, for exchanges
+// among the lanes of one wavefront (a wavefront's LDS instructions execute in order; the
+// fences keep the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 enum { PLANE_FLOW = 0, PLANE_POISSON = 1, PLANE_FLOW_ADJ = 2 };
 
 // y and z transforms of one kx plane, spectral multiplier, inverse z and y; the block
@@ -211,7 +220,10 @@ enum { PLANE_FLOW = 0, PLANE_POISSON = 1, PLANE_FLOW_ADJ = 2 };
 // registers while it transforms the current one.  Schedules: forward [R1, n/R1],
 // inverse [n/R1, R1]; the multiplier and the first inverse z pass act on the registers of
 // the last forward z pass (same elements), the first forward y pass reads global memory
-// and the last inverse y pass writes it.
+// and the last inverse y pass writes it.  The threads of a y line (y passes) and of a z
+// column (z passes) are lanes of one wavefront, so the exchanges between the two passes of
+// a direction stay inside a wavefront (wave_lds_sync); only the row-to-column and
+// column-to-row exchanges need the workgroup barrier.
 //   PLANE_FLOW   : plane = c*nx + kx, multiplier mask_x[kx] mask_y[ky] mask_z[kz]
 //                  (src/parallel.jl:58-62,182-183; 1/N folded into mask_x)
 //   PLANE_POISSON: plane = kx in [0, nx/2], multiplier scale / (s2x+s2y+s2z), 0 at k = 0
@@ -255,6 +267,7 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
         else mtab[i] = i < NY ? tb.s2y[i] : tb.s2z[i - NY];
     }
     const int yl = tid % NY, tz = tid / NY;        // z transforms: line y = yl
+    __syncthreads();  // tables loaded
     const LayContig ly{lds + zl * LSY};
     const LayStrided lz{lds + pad16(yl), LSY};
     using Z1 = RPass<NZ, R1, 1, E>;
@@ -272,19 +285,19 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
             Y1::load(v, LayGlobal{P + zl * NY}, ty);
         }
 
-        // ---- forward y (the first pass of each direction has no twiddles)
+        // ---- forward y (the first pass of each direction has no twiddles); the row's
+        // previous readers (the last inverse y pass) are this wavefront
         Y1::template compute<-1>(v, ty, twy);
-        __syncthreads();  // the previous plane's LDS reads are done, tables loaded
+        wave_lds_sync();
         Y1::store(v, ly, ty);
-        __syncthreads();
         if constexpr (RY2 > 1) {
             using Y2 = RPass<NY, RY2, R1, E>;
+            wave_lds_sync();
             Y2::load(v, ly, ty);
             Y2::template compute<-1>(v, ty, twy);  // TY2 table
-            __syncthreads();
             Y2::store(v, ly, ty);
-            __syncthreads();
         }
+        __syncthreads();  // rows -> columns
         // ---- forward z, multiplier, inverse z
         Z1::load(v, lz, tz);
         Z1::template compute<-1>(v, tz, twz);
@@ -328,24 +341,23 @@ k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables 
         ZI::template compute<1>(v, tz, twz);
         __syncthreads();
         ZI::store(v, lz, tz);
-        __syncthreads();
         if constexpr (RZ2 > 1) {
             using Z2I = RPass<NZ, R1, RZ2, E>;
+            __syncthreads();
             Z2I::load(v, lz, tz);
             Z2I::template compute<1>(v, tz, twz + TZ2::SIZE);
             __syncthreads();
             Z2I::store(v, lz, tz);
-            __syncthreads();
         }
+        __syncthreads();  // columns -> rows
         // ---- inverse y; the last pass writes global memory
         if constexpr (RY2 > 1) {
             using Y1I = RPass<NY, RY2, 1, E>;
             using Y2I = RPass<NY, R1, RY2, E>;
             Y1I::load(v, ly, ty);
             Y1I::template compute<1>(v, ty, twy);
-            __syncthreads();
             Y1I::store(v, ly, ty);
-            __syncthreads();
+            wave_lds_sync();
             Y2I::load(v, ly, ty);
             Y2I::template compute<1>(v, ty, twy + TY2::SIZE);
             Y2I::store(v, LayGlobal{P + zl * NY}, ty);
