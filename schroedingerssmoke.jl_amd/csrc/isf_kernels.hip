@@ -202,9 +202,6 @@ template <int NY, int NZ> struct PlaneCfg {
     static_assert(NY >= 16 && NZ >= 16 && NT <= 1024, "plane too large for one workgroup");
 };
 
-// PLANE_FLOW_ADJ: the adjoint of PLANE_FLOW, conj(mask) (the adjoint of F^-1 M F with an
-// unnormalised forward transform F is F^-1 conj(M) F).
-
 // Orders a wavefront's LDS accesses before the call before those after it, for exchanges
 // among the lanes of one wavefront (a wavefront's LDS instructions execute in order; the
 // fences keep the compiler from moving them across).
@@ -213,6 +210,9 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+// PLANE_FLOW_ADJ: the adjoint of PLANE_FLOW, conj(mask) (the adjoint of F^-1 M F with an
+// unnormalised forward transform F is F^-1 conj(M) F).
 enum { PLANE_FLOW = 0, PLANE_POISSON = 1, PLANE_FLOW_ADJ = 2 };
 
 // y and z transforms of one kx plane, spectral multiplier, inverse z and y; the block
@@ -220,10 +220,13 @@ enum { PLANE_FLOW = 0, PLANE_POISSON = 1, PLANE_FLOW_ADJ = 2 };
 // registers while it transforms the current one.  Schedules: forward [R1, n/R1],
 // inverse [n/R1, R1]; the multiplier and the first inverse z pass act on the registers of
 // the last forward z pass (same elements), the first forward y pass reads global memory
-// and the last inverse y pass writes it.  The threads of a y line (y passes) and of a z
-// column (z passes) are lanes of one wavefront, so the exchanges between the two passes of
-// a direction stay inside a wavefront (wave_lds_sync); only the row-to-column and
-// column-to-row exchanges need the workgroup barrier.
+// and the last inverse y pass writes it.  The threads of a y line are neighbouring lanes
+// of one wavefront, so the exchanges between the two y passes of each direction stay
+// inside the wavefront (wave_lds_sync).  The threads of a z column are NY lanes apart, so
+// that a wavefront reads consecutive columns (no LDS bank conflicts); the z exchanges and
+// the row / column switches take the workgroup barrier.  Giving each z column one
+// wavefront (8 columns x 8 threads) would save those barriers too, but its strided column
+// accesses conflict in the LDS banks: measured 59 us against 43 us for the flow plane.
 //   PLANE_FLOW   : plane = c*nx + kx, multiplier mask_x[kx] mask_y[ky] mask_z[kz]
 //                  (src/parallel.jl:58-62,182-183; 1/N folded into mask_x)
 //   PLANE_POISSON: plane = kx in [0, nx/2], multiplier scale / (s2x+s2y+s2z), 0 at k = 0
