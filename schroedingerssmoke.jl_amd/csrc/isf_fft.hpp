@@ -181,14 +181,26 @@ struct PassTw {
 };
 
 // ------------------------------------------------------------------ Stockham passes
+// Orders a wavefront's LDS accesses before the call before those after it, for exchanges
+// among the lanes of one wavefront (a wavefront's LDS instructions execute in order; the
+// fences keep the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One radix-R pass over a line of N held in LDS by T = N/E threads (thread t of the
-// line).  All threads of the block call it (it contains block barriers); threads with
-// active == false only take part in the barriers.  tw is this pass's PassTw table.
+// line).  The T threads of a line are lanes of one wavefront (T divides 64 and the caller
+// gives line l to threads l*T .. l*T + T - 1), so the pass orders its reads and writes
+// with wavefront fences only; threads with active == false do nothing.  tw is this
+// pass's PassTw table.
 template <class Lay, int N, int R, int NS, int E, int DIR>
 __device__ __forceinline__ void stockham_pass(const Lay& lay, int t, bool active, const float2* tw) {
     constexpr int T = N / E;
     constexpr int NB = E / R;
     static_assert(E <= N && NB >= 1 && R * NB == E, "bad FFT schedule");
+    static_assert(64 % T == 0, "a line's threads must share a wavefront");
     float2 v[NB][R];
     if (active) {
 #pragma unroll
@@ -204,7 +216,7 @@ __device__ __forceinline__ void stockham_pass(const Lay& lay, int t, bool active
             Dft<R, DIR>::run(v[b]);
         }
     }
-    __syncthreads();
+    wave_lds_sync();
     if (active) {
 #pragma unroll
         for (int b = 0; b < NB; b++) {
@@ -215,7 +227,7 @@ __device__ __forceinline__ void stockham_pass(const Lay& lay, int t, bool active
             for (int r = 0; r < R; r++) lay.st(idxd + r * NS, v[b][r]);
         }
     }
-    __syncthreads();
+    wave_lds_sync();
 }
 
 // The passes of a length-N transform with E elements per thread: radix
@@ -258,10 +270,13 @@ struct FftTw {
 };
 
 // Full length-N transform of one LDS line (DIR = -1 forward, +1 inverse, unnormalised).
-// tw holds the schedule's tables (FftTw<N, E>).
+// tw holds the schedule's tables (FftTw<N, E>).  All threads of the block call it: the
+// line's data must be in LDS (block barrier) before, and it ends with a block barrier, so
+// the caller may read other lines afterwards.
 template <int N, int E, int DIR, class Lay>
 __device__ __forceinline__ void fft_line(const Lay& lay, int t, bool active, const float2* tw) {
     Sched<N, E, 1>::template run<Lay, DIR>(lay, t, active, tw);
+    __syncthreads();
 }
 
 // Copy the length-N twiddle table (at tw_all + N) into LDS; the caller synchronises.

@@ -202,15 +202,6 @@ template <int NY, int NZ> struct PlaneCfg {
     static_assert(NY >= 16 && NZ >= 16 && NT <= 1024, "plane too large for one workgroup");
 };
 
-// Orders a wavefront's LDS accesses before the call before those after it, for exchanges
-// among the lanes of one wavefront (a wavefront's LDS instructions execute in order; the
-// fences keep the compiler from moving them across).
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // PLANE_FLOW_ADJ: the adjoint of PLANE_FLOW, conj(mask) (the adjoint of F^-1 M F with an
 // unnormalised forward transform F is F^-1 conj(M) F).
 enum { PLANE_FLOW = 0, PLANE_POISSON = 1, PLANE_FLOW_ADJ = 2 };
