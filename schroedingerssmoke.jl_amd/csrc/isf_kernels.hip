@@ -1362,7 +1362,13 @@ int launch_spawn(float* parts, long long nslots, long long first, long long n, c
     int chunk = 64;  // the sort size: the draw count rounded up to a power of two, 64..1024
     while (chunk < SPAWN_CHUNK && chunk < nn) chunk <<= 1;
     const dim3 grid((unsigned)((nn + chunk - 1) / chunk), members_of(b));
-    ISF_LAUNCH(k_spawn, grid, dim3(chunk), s, parts, nslots, first, n, jet, seed, step, b);
+    // 16 KB of LDS reserved (unused): with its 4 KB the block cannot share a CU with a
+    // 128 x 128 plane-pass workgroup (147 KB of the 160 KB).  Beside one it slows that
+    // workgroup, and the plane pass ends with its slowest workgroup (flow plane 46 -> 68 us
+    // in a trace where the spawn of the side stream landed there).
+    constexpr unsigned kSpawnLdsReserve = 16384;
+    hipLaunchKernelGGL(k_spawn, grid, dim3(chunk), kSpawnLdsReserve, s, parts, nslots, first, n, jet, seed, step, b);
+    return (int)hipGetLastError();
 }
 // batch mode: max_slots = the largest nslots of the members (grid size)
 int launch_advect(const GridInfo& g, float* parts, long long nslots, const int* order, const float* vel, float dt,
