@@ -236,6 +236,35 @@ def test_constraint_init_matches_oracle(isf_mod):
     assert err < 1e-5
 
 
+def test_step_side_stream_ordering(isf_mod):
+    """isf_step computes the velocity field and advects on a second stream.  A gauge
+    transform queued right after the step (no synchronisation) rewrites psi with a phase
+    that varies in x, which changes every edge angle: the step's velocity field and
+    particles must still be those of the stepped psi, as in a run synchronised first."""
+    cfg = JetConfig(n_spawn=200, n_slots=20_000, n_init=10_000, constraint_iters=4, seed=5)
+    runs = []
+    for sync_first in (True, False):
+        g = GpuRun(cfg)
+        g.step(3)
+        if sync_first:
+            g.isf.synchronize()
+        nx = g.isf.grid_res[0]
+        q = np.broadcast_to((0.4 * np.sin(2 * np.pi * np.arange(nx) / nx)).astype(np.float32), g.isf.shape3)
+        g.isf.gauge_transform_(g.psi, isf_mod.DeviceArray.from_host(np.ascontiguousarray(q)))
+        g.isf.synchronize()
+        runs.append((g.isf.velocity.to_host(), g.host_parts(), g.host_psi()))
+    (va, pa, sa), (vb, pb, sb) = runs
+    assert np.array_equal(va, vb)
+    assert np.array_equal(pa, pb)
+    assert np.array_equal(sa, sb)
+    # and the field is velocity_one_form of the psi the step produced (before the gauge)
+    g = GpuRun(cfg)
+    g.step(3)
+    ref = g.isf.velocity_one_form_(g.psi, hbar=cfg.hbar, sharp=True,
+                                   out=isf_mod.DeviceArray(g.isf.shape3 + (3,), np.float32)).to_host()
+    assert np.array_equal(va, ref)
+
+
 @pytest.mark.parametrize("phase_mode", [0, 1])
 def test_full_steps_c1(isf_mod, phase_mode):
     """The CPU-runnable jet (64x32x32, 48 steps, 50 particles per step).
