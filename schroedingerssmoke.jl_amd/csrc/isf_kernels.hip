@@ -489,6 +489,67 @@ k_line_z(float2* __restrict__ base, int nx, int ny, double scale, DevTables tb) 
     }
 }
 
+// The same column transforms with the data in registers (columns of at most E^2 points,
+// two radix passes per direction): the first forward pass reads global memory and the
+// last inverse pass writes it, the multiplier and the first inverse pass act on the
+// registers of the last forward pass, so each direction has one LDS exchange.  Same
+// schedules and multipliers as the z passes of k_plane.
+template <int NZ, int MODE>
+__global__ void __launch_bounds__(ZColCfg<NZ>::NT)
+k_line_z_reg(float2* __restrict__ base, int nx, int ny, double scale, DevTables tb) {
+    using C = ZColCfg<NZ>;
+    constexpr int E = C::E, CW = C::CW, LS = C::LS, NT = C::NT;
+    constexpr int R1 = E, R2 = NZ / E;
+    static_assert(R2 > 1 && R2 <= E, "two radix passes");
+    using Z1 = RPass<NZ, R1, 1, E>;    // forward pass 1
+    using ZL = RPass<NZ, R2, R1, E>;   // forward pass 2
+    using ZI = RPass<NZ, R2, 1, E>;    // inverse pass 1 (reads ZL's outputs in place)
+    using Z2I = RPass<NZ, R1, R2, E>;  // inverse pass 2
+    using TF = PassTw<NZ, R2, R1>;
+    using TI = PassTw<NZ, R1, R2>;
+    __shared__ float2 lds[CW * LS];
+    __shared__ float2 tw[TF::SIZE + TI::SIZE + 1];
+    const int tid = threadIdx.x, c = tid % CW, t = tid / CW;
+    const int plane = blockIdx.y, y0 = blockIdx.x * CW, ky = y0 + c;
+    float2* P = base + (size_t)plane * ny * NZ;
+    const LayStrided col{P + ky, ny};
+    const LayContig lc{lds + c * LS};
+    float2 v[E];
+    Z1::load(v, col, t);  // issued before the table loads
+    TF::template fill<NT>(tw, tb.tw + NZ, tid);
+    TI::template fill<NT>(tw + TF::SIZE, tb.tw + NZ, tid);
+    Z1::template compute<-1>(v, t, tw);
+    Z1::store(v, lc, t);
+    __syncthreads();  // column exchange, tables loaded
+    ZL::load(v, lc, t);
+    ZL::template compute<-1>(v, t, tw);
+    constexpr int RL = E / ZL::NB;
+#pragma unroll
+    for (int b = 0; b < ZL::NB; b++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            const int kz = ZL::out(t, b, r);
+            float2& x = v[b * RL + r];
+            if (MODE != PLANE_POISSON) {
+                const int kx = plane % nx;
+                const float2 m = cmul(cmul(tb.mask_x[kx], tb.mask_y[ky]), tb.mask_z[kz]);
+                x = cmul(x, MODE == PLANE_FLOW_ADJ ? cconj(m) : m);
+            } else {
+                const int kx = plane;
+                const float den = (float)((tb.s2x[kx] + tb.s2y[ky]) + tb.s2z[kz]);
+                const float m = (kx == 0 && ky == 0 && kz == 0) ? 0.0f : (float)scale * __builtin_amdgcn_rcpf(den);
+                x = cscale(x, m);
+            }
+        }
+    ZI::template compute<1>(v, t, tw);
+    __syncthreads();  // the forward exchange's reads are done
+    ZI::store(v, lc, t);
+    __syncthreads();
+    Z2I::load(v, lc, t);
+    Z2I::template compute<1>(v, t, tw + TF::SIZE);
+    Z2I::store(v, col, t);
+}
+
 enum { C2R_Q = 0, C2R_GAUGE = 1, C2R_GAUGE_RESTRICT = 2 };
 
 // Complex-to-real inverse x transform of TY lines of P (src/parallel.jl:172) and, in the
@@ -1250,6 +1311,12 @@ static int line_z_t(int mode, float2* base, int nplanes, int nx, int ny, double 
     using C = ZColCfg<NZ>;
     if (ny % C::CW) return (int)hipErrorInvalidValue;
     const dim3 grid(ny / C::CW, nplanes);
+    if constexpr (NZ / C::E > 1 && NZ / C::E <= C::E) {
+        if (mode == PLANE_FLOW) ISF_LAUNCH((k_line_z_reg<NZ, PLANE_FLOW>), grid, dim3(C::NT), s, base, nx, ny, scale, t);
+        if (mode == PLANE_FLOW_ADJ)
+            ISF_LAUNCH((k_line_z_reg<NZ, PLANE_FLOW_ADJ>), grid, dim3(C::NT), s, base, nx, ny, scale, t);
+        ISF_LAUNCH((k_line_z_reg<NZ, PLANE_POISSON>), grid, dim3(C::NT), s, base, nx, ny, scale, t);
+    }
     if (mode == PLANE_FLOW) ISF_LAUNCH((k_line_z<NZ, PLANE_FLOW>), grid, dim3(C::NT), s, base, nx, ny, scale, t);
     if (mode == PLANE_FLOW_ADJ) ISF_LAUNCH((k_line_z<NZ, PLANE_FLOW_ADJ>), grid, dim3(C::NT), s, base, nx, ny, scale, t);
     ISF_LAUNCH((k_line_z<NZ, PLANE_POISSON>), grid, dim3(C::NT), s, base, nx, ny, scale, t);
