@@ -128,6 +128,7 @@ def bench_ensemble(steps, warmup, count=48, seed=20261203):
     setup_s = time.perf_counter() - t0
     shapes = sorted({m["res"] for m in ms})
     tiles = {"x".join(map(str, r)): b.tile(r) for r in shapes}
+    plane_elems = {"x".join(map(str, r)): b.plane_elems(r) for r in shapes}
     counts = {"x".join(map(str, r)): sum(1 for m in ms if m["res"] == r) for r in shapes}
 
     def run(fn, it0):
@@ -163,6 +164,7 @@ def bench_ensemble(steps, warmup, count=48, seed=20261203):
         "achieved_GB_s": round(balg / t_bat / 1e9, 1),
         "frac": round(balg / t_bat / 1e9 / HBM_PEAK_GBS, 4),
         "x_tiles": tiles,
+        "plane_elements_per_thread": plane_elems,
         "norm2_over_N_range": [round(float(min(norms)), 9), round(float(max(norms)), 9)],
         "steps": steps,
         "warmup": warmup,

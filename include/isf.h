@@ -243,12 +243,16 @@ int isf_batch_set_graph(isf_batch_t b, int use_graph);
 /* The same step as n separate isf_step calls, one member after another (baseline). */
 int isf_batch_step_sequential(isf_batch_t b, int iter);
 int isf_batch_synchronize(isf_batch_t b);
-/* Pick the x-pass tile (y lines per block: 8, 16, 32) of every grid shape of the batch by
- * timing it on scratch data; results are read from / appended to the text file
- * cache_path when it is not NULL. */
+/* Pick the FFT schedule of every grid shape of the batch by timing it on scratch data:
+ * the x-pass tile (y lines per block: 8, 16, 32) and, for planes of at most 64 x 64, the
+ * plane pass's elements per thread (8: radix-8 passes, 16: radix-16 passes).  Results are
+ * read from / appended to the text file cache_path when it is not NULL, one line per
+ * shape: "nx ny nz members tile [plane_elements]". */
 int isf_batch_tune(isf_batch_t b, const char* cache_path);
 /* The tile chosen for a grid shape (0 when none was tuned). */
 int isf_batch_tile(isf_batch_t b, const int res[3], int* ty);
+/* The plane pass's elements per thread in use for a grid shape (8 or 16). */
+int isf_batch_plane_elems(isf_batch_t b, const int res[3], int* pe);
 
 /* ---- vector-Jacobian products of the operators (no reference counterpart) ------------- */
 /* Each maps the cotangent of an operator's output to the cotangent of its input, under

@@ -1139,20 +1139,62 @@ static int time_x_tile(isf_batch_t b, const BatchGroup& gr, int ty, const Inst* 
     return ISF_OK;
 }
 
+// Time the flow and Poisson plane passes of one step for one group with plane elements
+// per thread pe (0: default), on the members' spectra; milliseconds of `reps` rounds.
+static int time_planes(isf_batch_t b, const BatchGroup& gr, int pe, const Inst* inst, float* ms) {
+    GridInfo g = gr.g;
+    g.pe = pe;
+    const Batch bt{inst, b->counter, (int)gr.idx.size()};
+    const DevTables& t0 = b->mem[gr.idx[0]]->tb;
+    auto once = [&]() -> int {
+        LAUNCH_CHECK(launch_flow_plane(g, nullptr, t0, b->stream, bt), "tune flow plane");
+        LAUNCH_CHECK(launch_poisson_plane(g, nullptr, 1.0, t0, b->stream, bt), "tune poisson plane");
+        return ISF_OK;
+    };
+    int r = once();
+    if (r) return r;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_CHECK(hipEventCreate(&e0), "hipEventCreate");
+    hipError_t e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(e, "hipEventCreate");
+    }
+    const int reps = 5;
+    e = hipEventRecord(e0, b->stream);
+    for (int k = 0; !r && e == hipSuccess && k < reps; k++) r = once();
+    if (!r && e == hipSuccess) e = hipEventRecord(e1, b->stream);
+    if (!r && e == hipSuccess) e = hipEventSynchronize(e1);
+    if (!r && e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r) return r;
+    if (e != hipSuccess) return hip_fail(e, "tune timing");
+    return ISF_OK;
+}
+
+// planes whose pass can run with 8 (default) or 16 elements per thread
+static bool plane_pe_choice(const GridInfo& g) { return g.ny <= 64 && g.nz <= 64; }
+
 int isf_batch_tune(isf_batch_t b, const char* cache_path) {
     CHECK_B(b);
     HIP_CHECK(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    // cache lines: nx ny nz members x_tile [plane_elements]; a line without the last
+    // column keeps the default plane schedule
     struct Entry {
-        int nx, ny, nz, members, ty;
+        int nx, ny, nz, members, ty, pe;
     };
     std::vector<Entry> cache;
     if (cache_path) {
         if (FILE* f = std::fopen(cache_path, "r")) {
-            Entry en;
             char line[256];
-            while (std::fgets(line, sizeof line, f))
-                if (std::sscanf(line, "%d %d %d %d %d", &en.nx, &en.ny, &en.nz, &en.members, &en.ty) == 5)
-                    cache.push_back(en);
+            while (std::fgets(line, sizeof line, f)) {
+                Entry en{};
+                const int k = std::sscanf(line, "%d %d %d %d %d %d", &en.nx, &en.ny, &en.nz, &en.members, &en.ty,
+                                          &en.pe);
+                if (k == 5) en.pe = 0;
+                if (k == 5 || k == 6) cache.push_back(en);
+            }
             std::fclose(f);
         }
     }
@@ -1160,17 +1202,20 @@ int isf_batch_tune(isf_batch_t b, const char* cache_path) {
     int rc = ISF_OK;
     for (BatchGroup& gr : b->groups) {
         const int members = (int)gr.idx.size();
-        int best = 0;
+        const bool pe_choice = plane_pe_choice(gr.g);
+        int best = 0, best_pe = 0;
         for (const Entry& en : cache)
             if (en.nx == gr.g.nx && en.ny == gr.g.ny && en.nz == gr.g.nz && en.members == members &&
-                (en.ty == 8 || en.ty == 16 || en.ty == 32))
+                (en.ty == 8 || en.ty == 16 || en.ty == 32)) {
                 best = en.ty;
+                best_pe = (pe_choice && en.pe == 16) ? 16 : 0;
+            }
         if (!best) {
             std::vector<int> cand;
             for (int ty : {8, 16, 32})
                 if (ty <= gr.g.ny && (ty == 8 || gr.g.nx <= 256) && gr.g.nx < 512) cand.push_back(ty);
             if (cand.empty()) cand.push_back(8);
-            if (cand.size() == 1) {
+            if (cand.size() == 1 && !pe_choice) {
                 best = cand[0];
             } else {
                 // scratch psi per member; the members' own spectra are scratch already
@@ -1190,10 +1235,21 @@ int isf_batch_tune(isf_batch_t b, const char* cache_path) {
                 for (int ty : cand) {
                     if (rc) break;
                     float ms = 0.0f;
-                    rc = time_x_tile(b, gr, ty, dinst, &ms);
+                    rc = cand.size() == 1 ? ISF_OK : time_x_tile(b, gr, ty, dinst, &ms);
                     if (!rc && (best == 0 || ms < best_ms)) {
                         best = ty;
                         best_ms = ms;
+                    }
+                }
+                // plane schedule: radix 8 (8 elements per thread) or radix 16 (16)
+                float pe_ms = 0.0f;
+                for (int pe : {0, 16}) {
+                    if (rc || !pe_choice) break;
+                    float ms = 0.0f;
+                    rc = time_planes(b, gr, pe, dinst, &ms);
+                    if (!rc && (pe == 0 || ms < pe_ms)) {
+                        best_pe = pe;
+                        pe_ms = ms;
                     }
                 }
                 (void)hipStreamSynchronize(b->stream);
@@ -1201,19 +1257,36 @@ int isf_batch_tune(isf_batch_t b, const char* cache_path) {
                 if (dinst) (void)hipFree(dinst);
                 if (rc) return rc;
             }
-            fresh.push_back(Entry{gr.g.nx, gr.g.ny, gr.g.nz, members, best});
+            fresh.push_back(Entry{gr.g.nx, gr.g.ny, gr.g.nz, members, best, best_pe});
         }
         gr.g.ty = best;
-        for (int i : gr.idx) b->mem[i]->g.ty = best;
+        gr.g.pe = best_pe;
+        for (int i : gr.idx) {
+            b->mem[i]->g.ty = best;
+            b->mem[i]->g.pe = best_pe;
+        }
     }
     if (cache_path && !fresh.empty()) {
         if (FILE* f = std::fopen(cache_path, "a")) {
-            for (const Entry& en : fresh) std::fprintf(f, "%d %d %d %d %d\n", en.nx, en.ny, en.nz, en.members, en.ty);
+            for (const Entry& en : fresh)
+                std::fprintf(f, "%d %d %d %d %d %d\n", en.nx, en.ny, en.nz, en.members, en.ty, en.pe ? en.pe : 8);
             std::fclose(f);
         }
     }
     b->dirty = true;
     return ISF_OK;
+}
+
+int isf_batch_plane_elems(isf_batch_t b, const int res[3], int* pe) {
+    CHECK_B(b);
+    CHECK_PTR(res, "res");
+    CHECK_PTR(pe, "pe");
+    for (const BatchGroup& gr : b->groups)
+        if (gr.g.nx == res[0] && gr.g.ny == res[1] && gr.g.nz == res[2]) {
+            *pe = gr.g.pe ? gr.g.pe : (plane_pe_choice(gr.g) ? 8 : 16);
+            return ISF_OK;
+        }
+    return fail(ISF_EINVAL, "no member of this batch has grid (%d, %d, %d)", res[0], res[1], res[2]);
 }
 
 int isf_batch_tile(isf_batch_t b, const int res[3], int* ty) {

@@ -15,6 +15,8 @@ struct GridInfo {
     float idx, idy, idz;  // 1/d, used instead of a division only when d is a power of two
     int d_pow2;           // all three d are powers of two (then p * (1/d) == p / d exactly)
     int ty;               // y lines per block of the x passes (0: default choice)
+    int pe;               // elements per thread of the plane pass (0: default choice; 16 also
+                          // for planes up to 64 x 64, whose default is 8)
 };
 
 // Jet constraint parameters (src/particles.jl:12-27, test/runtests.jl:7-12,23-25).

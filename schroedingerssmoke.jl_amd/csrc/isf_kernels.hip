@@ -187,11 +187,11 @@ struct LayGlobal {
 // Every thread owns E elements of one y line during the y transforms and of one z line
 // during the z transforms, so NT = NY * NZ / E threads: E = 16, or 8 when both sides are
 // at most 64 (twice the wavefronts for the small planes, whose few workgroups are
-// latency-bound).  The first pass of each direction has radix R1 = E, the second
+// latency-bound); EO != 0 sets E (the batch autotuner may pick 16 for the small planes).  The first pass of each direction has radix R1 = E, the second
 // N / R1.  With 1024 threads a thread has 128 VGPRs, too few for a second 16-element
 // buffer, so the next-plane prefetch is used only when NT <= 512.
-template <int NY, int NZ> struct PlaneCfg {
-    static constexpr int E = (NY <= 64 && NZ <= 64) ? 8 : 16;
+template <int NY, int NZ, int EO = 0> struct PlaneCfg {
+    static constexpr int E = EO ? EO : (NY <= 64 && NZ <= 64) ? 8 : 16;
     static constexpr int R1 = E;
     static constexpr int TY = NY / E, TZ = NZ / E;   // threads per y line / z line
     static constexpr int NT = NZ * TY;
@@ -223,10 +223,10 @@ enum { PLANE_FLOW = 0, PLANE_POISSON = 1, PLANE_FLOW_ADJ = 2 };
 //   PLANE_POISSON: plane = kx in [0, nx/2], multiplier scale / (s2x+s2y+s2z), 0 at k = 0
 //                  (src/parallel.jl:53-55,63,171; scale folds -0.25 and the 2/N of the
 //                  half-length real inverse)
-template <int NY, int NZ, int MODE>
-__global__ void __launch_bounds__((PlaneCfg<NY, NZ>::NT))
+template <int NY, int NZ, int MODE, int EO>
+__global__ void __launch_bounds__((PlaneCfg<NY, NZ, EO>::NT))
 k_plane(float2* __restrict__ base, int nplanes, int nx, double scale, DevTables tb, Batch B) {
-    using C = PlaneCfg<NY, NZ>;
+    using C = PlaneCfg<NY, NZ, EO>;
     if (B.inst) {
         const Inst& m = B.inst[blockIdx.y];
         base = MODE == PLANE_POISSON ? poisson_P(m) : m.S;
@@ -1261,10 +1261,10 @@ static int cu_count() {
     return cached[dev];
 }
 
-template <int NY, int NZ>
+template <int NY, int NZ, int EO>
 static int plane_t(int mode, float2* base, int nplanes, int nx, double scale, const DevTables& t, hipStream_t s,
                    Batch b) {
-    using C = PlaneCfg<NY, NZ>;
+    using C = PlaneCfg<NY, NZ, EO>;
     const int by_lds = (160 * 1024) / C::LDS_BYTES, by_threads = 2048 / C::NT;
     const int per_cu = std::max(1, std::min(by_lds, by_threads));
     const int mem = (int)members_of(b);
@@ -1272,19 +1272,29 @@ static int plane_t(int mode, float2* base, int nplanes, int nx, double scale, co
     const int blocks = std::max(1, std::min(nplanes, (cu_count() * per_cu + mem - 1) / mem));
     const dim3 grid(blocks, mem);
     if (mode == PLANE_FLOW)
-        ISF_LAUNCH((k_plane<NY, NZ, PLANE_FLOW>), grid, dim3(C::NT), s, base, nplanes, nx, scale, t, b);
+        ISF_LAUNCH((k_plane<NY, NZ, PLANE_FLOW, EO>), grid, dim3(C::NT), s, base, nplanes, nx, scale, t, b);
     if (mode == PLANE_FLOW_ADJ)
-        ISF_LAUNCH((k_plane<NY, NZ, PLANE_FLOW_ADJ>), grid, dim3(C::NT), s, base, nplanes, nx, scale, t, b);
-    ISF_LAUNCH((k_plane<NY, NZ, PLANE_POISSON>), grid, dim3(C::NT), s, base, nplanes, nx, scale, t, b);
+        ISF_LAUNCH((k_plane<NY, NZ, PLANE_FLOW_ADJ, EO>), grid, dim3(C::NT), s, base, nplanes, nx, scale, t, b);
+    ISF_LAUNCH((k_plane<NY, NZ, PLANE_POISSON, EO>), grid, dim3(C::NT), s, base, nplanes, nx, scale, t, b);
+}
+// pe: elements per thread (0: the default; 16 for planes up to 64 x 64 selects the radix
+// 16 schedule instead of radix 8)
+template <int NY, int NZ>
+static int plane_pe(int pe, int mode, float2* base, int nplanes, int nx, double scale, const DevTables& t,
+                    hipStream_t s, Batch b) {
+    if constexpr (NY <= 64 && NZ <= 64) {
+        if (pe == 16) return plane_t<NY, NZ, 16>(mode, base, nplanes, nx, scale, t, s, b);
+    }
+    return plane_t<NY, NZ, 0>(mode, base, nplanes, nx, scale, t, s, b);
 }
 
-#define ISF_PLANE_Z(NY)                                                            \
-    switch (nz) {                                                                  \
-        case 16: return plane_t<NY, 16>(mode, base, nplanes, nx, scale, t, s, b);  \
-        case 32: return plane_t<NY, 32>(mode, base, nplanes, nx, scale, t, s, b);  \
-        case 64: return plane_t<NY, 64>(mode, base, nplanes, nx, scale, t, s, b);  \
-        case 128: return plane_t<NY, 128>(mode, base, nplanes, nx, scale, t, s, b); \
-    }                                                                              \
+#define ISF_PLANE_Z(NY)                                                                     \
+    switch (nz) {                                                                           \
+        case 16: return plane_pe<NY, 16>(pe, mode, base, nplanes, nx, scale, t, s, b);     \
+        case 32: return plane_pe<NY, 32>(pe, mode, base, nplanes, nx, scale, t, s, b);     \
+        case 64: return plane_pe<NY, 64>(pe, mode, base, nplanes, nx, scale, t, s, b);     \
+        case 128: return plane_pe<NY, 128>(pe, mode, base, nplanes, nx, scale, t, s, b);   \
+    }                                                                                       \
     return (int)hipErrorInvalidValue;
 
 template <int NY, int DIR>
@@ -1340,7 +1350,7 @@ static int plane_split(int mode, int ny, int nz, float2* base, int nplanes, int 
 }
 
 static int plane_dispatch(int mode, int ny, int nz, float2* base, int nplanes, int nx, double scale,
-                          const DevTables& t, hipStream_t s, Batch b) {
+                          const DevTables& t, hipStream_t s, Batch b, int pe) {
     if (!plane_supported(ny, nz)) return (int)hipErrorInvalidValue;
     if (!plane_in_lds(ny, nz)) {
         if (b.inst) return (int)hipErrorInvalidValue;  // batches hold LDS-sized planes only
@@ -1356,13 +1366,13 @@ static int plane_dispatch(int mode, int ny, int nz, float2* base, int nplanes, i
 }
 
 int launch_flow_plane(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s, Batch b) {
-    return plane_dispatch(PLANE_FLOW, g.ny, g.nz, S, 2 * g.nx, g.nx, 0.0, t, s, b);
+    return plane_dispatch(PLANE_FLOW, g.ny, g.nz, S, 2 * g.nx, g.nx, 0.0, t, s, b, g.pe);
 }
 int launch_flow_plane_adj(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s) {
-    return plane_dispatch(PLANE_FLOW_ADJ, g.ny, g.nz, S, 2 * g.nx, g.nx, 0.0, t, s, Batch{});
+    return plane_dispatch(PLANE_FLOW_ADJ, g.ny, g.nz, S, 2 * g.nx, g.nx, 0.0, t, s, Batch{}, g.pe);
 }
 int launch_poisson_plane(const GridInfo& g, float2* P, double scale, const DevTables& t, hipStream_t s, Batch b) {
-    return plane_dispatch(PLANE_POISSON, g.ny, g.nz, P, g.nx / 2 + 1, g.nx, scale, t, s, b);
+    return plane_dispatch(PLANE_POISSON, g.ny, g.nz, P, g.nx / 2 + 1, g.nx, scale, t, s, b, g.pe);
 }
 
 static inline dim3 grid1d(size_t n, int nt) { return dim3((unsigned)((n + nt - 1) / nt)); }

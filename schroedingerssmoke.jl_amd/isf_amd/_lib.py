@@ -111,6 +111,7 @@ SIGNATURES = {
     "isf_batch_synchronize": ([C.c_void_p], C.c_int),
     "isf_batch_tune": ([C.c_void_p, C.c_char_p], C.c_int),
     "isf_batch_tile": ([C.c_void_p, c_int3, C.POINTER(C.c_int)], C.c_int),
+    "isf_batch_plane_elems": ([C.c_void_p, c_int3, C.POINTER(C.c_int)], C.c_int),
     "isf_vjp_schroedinger_flow": ([C.c_void_p, C.c_void_p], C.c_int),
     "isf_vjp_normalize": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "isf_vjp_pressure_project": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),

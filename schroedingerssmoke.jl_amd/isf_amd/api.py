@@ -351,6 +351,13 @@ class Batch:
         check(self.lib.isf_batch_tile(self.h, c_int3(*[int(v) for v in res]), C.byref(ty)), "isf_batch_tile")
         return ty.value
 
+    def plane_elems(self, res):
+        """Elements per thread of the plane pass for grid shape res (8 or 16)."""
+        pe = C.c_int()
+        check(self.lib.isf_batch_plane_elems(self.h, c_int3(*[int(v) for v in res]), C.byref(pe)),
+              "isf_batch_plane_elems")
+        return pe.value
+
     def close(self):
         if getattr(self, "h", None):
             for m in self.members:

@@ -125,25 +125,45 @@ def test_batched_member_matches_oracle(ens):
 
 
 def test_tiles_give_identical_results_through_isf_step(ens, tmp_path):
-    """Every x tile the autotuner can choose (8, 16, 32 lines per block), applied to the
-    member handles through the tile cache file and stepped with isf_step one member after
-    another, gives the same psi and particles within FP32 rounding."""
+    """Every schedule the autotuner can choose -- x tiles of 8, 16, 32 lines per block and,
+    for these 32 x 32 planes, the plane pass with 8 or 16 elements per thread -- applied to
+    the member handles through the tile cache file and stepped with isf_step one member
+    after another, gives the same psi and particles within FP32 rounding."""
     ms = members(ens, 4, seed=14, grids=((64, 32, 32),))
     results = {}
-    for forced in (8, 16, 32):
-        path = str(tmp_path / f"tile{forced}.txt")
+    for forced, pe in ((8, 8), (16, 8), (32, 8), (16, 16)):
+        path = str(tmp_path / f"tile{forced}_{pe}.txt")
         with open(path, "w") as f:
-            f.write(f"64 32 32 4 {forced}\n")
+            f.write(f"64 32 32 4 {forced} {pe}\n")
         b = ens.build_batch(ms, n_init=500)
         b.tune(path)
         assert b.tile((64, 32, 32)) == forced
-        print(f"[tiles via isf_step] tile {forced}", flush=True)
+        assert b.plane_elems((64, 32, 32)) == pe
+        print(f"[schedules via isf_step] tile {forced}, plane elements {pe}", flush=True)
         for it in range(1, 4):
             b.step_sequential_(it)
-        results[forced] = state(b)
+        results[(forced, pe)] = state(b)
         b.close()
-    assert_close(results[8], results[16], "tile 8 vs 16")
-    assert_close(results[8], results[32], "tile 8 vs 32")
+    assert_close(results[(8, 8)], results[(16, 8)], "tile 8 vs 16")
+    assert_close(results[(8, 8)], results[(32, 8)], "tile 8 vs 32")
+    assert_close(results[(16, 8)], results[(16, 16)], "plane radix 8 vs 16")
+
+
+def test_tuner_writes_both_schedule_choices(ens, tmp_path):
+    """A fresh tune times the x tiles and the plane schedules of each small-plane shape and
+    appends one cache line per shape with both choices; a second tune reads them back."""
+    ms = members(ens, 3, seed=21, grids=((32, 32, 32),))
+    cache = str(tmp_path / "sched.txt")
+    b = ens.build_batch(ms, n_init=200)
+    b.tune(cache)
+    ty, pe = b.tile((32, 32, 32)), b.plane_elems((32, 32, 32))
+    b.close()
+    assert ty in (8, 16, 32) and pe in (8, 16)
+    assert open(cache).read().split() == ["32", "32", "32", "3", str(ty), str(pe)]
+    b = ens.build_batch(ms, n_init=200)
+    b.tune(cache)
+    assert b.tile((32, 32, 32)) == ty and b.plane_elems((32, 32, 32)) == pe
+    b.close()
 
 
 @pytest.mark.skip(reason="aborted on the host (SIGABRT inside isf_batch_step, no GPU fault reported) in one run "
@@ -156,7 +176,7 @@ def test_autotuned_tiles_give_identical_results(ens, tmp_path):
     ty = t.tile((64, 32, 32))
     assert ty in (8, 16, 32)
     lines = open(cache).read().split()
-    assert lines == ["64", "32", "32", "4", str(ty)]
+    assert lines == ["64", "32", "32", "4", str(ty), str(t.plane_elems((64, 32, 32)))]
     t.close()
     results = {}
     for forced in (8, 16, 32):
