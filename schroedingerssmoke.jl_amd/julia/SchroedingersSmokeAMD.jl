@@ -275,6 +275,16 @@ tune!(b::ISFBatch, cache::AbstractString = "") =
     check(ccall((:isf_batch_tune, libisf), Cint, (Ptr{Cvoid}, Cstring), b.h, isempty(cache) ? C_NULL : cache),
           "isf_batch_tune")
 synchronize(b::ISFBatch) = check(ccall((:isf_batch_synchronize, libisf), Cint, (Ptr{Cvoid},), b.h), "isf_batch_synchronize")
+"The FFT schedule in use for grid `res`: x-pass tile and plane-pass elements per thread."
+function schedule(b::ISFBatch, res::NTuple{3,Integer})
+    r = Cint[res...]
+    ty = Ref{Cint}(0)
+    pe = Ref{Cint}(0)
+    check(ccall((:isf_batch_tile, libisf), Cint, (Ptr{Cvoid}, Ptr{Cint}, Ref{Cint}), b.h, r, ty), "isf_batch_tile")
+    check(ccall((:isf_batch_plane_elems, libisf), Cint, (Ptr{Cvoid}, Ptr{Cint}, Ref{Cint}), b.h, r, pe),
+          "isf_batch_plane_elems")
+    return (tile = Int(ty[]), plane_elements = Int(pe[]))
+end
 
 # ---------------------------------------------------------------- vector-Jacobian products
 # One per operator (isf_vjp_*): the cotangent of the output becomes that of the input, in place.
