@@ -276,12 +276,12 @@ def _counter_pass(counter, timeout_s=150):
 
 PARTICLE_STAGES = ("spawn", "particle_order", "advect")
 
-# stage name -> kernel name pattern (flow planes are k_plane<NY, NZ, 0>, Poisson planes <.., 1>)
+# stage name -> kernel name pattern (flow planes are k_plane<NY, NZ, 0, E>, Poisson planes <.., 1, E>)
 STAGE_KERNELS = {
-    "flow_x_fwd": r"k_flow_x_fwd<", "flow_plane": r"k_plane<\d+, \d+, 0>", "flow_x_inv_normalize": r"k_flow_x_inv<",
-    "p1_div_r2c_x": r"k_div_r2c<", "p1_plane": r"k_plane<\d+, \d+, 1>",
+    "flow_x_fwd": r"k_flow_x_fwd<", "flow_plane": r"k_plane<\d+, \d+, 0[,>]", "flow_x_inv_normalize": r"k_flow_x_inv<",
+    "p1_div_r2c_x": r"k_div_r2c<", "p1_plane": r"k_plane<\d+, \d+, 1[,>]",
     "p1_c2r_x_gauge_restrict": r"k_c2r_x<\d+, \d+, 2>", "spawn": r"k_spawn", "p2_div_r2c_x": r"k_div_r2c<",
-    "p2_plane": r"k_plane<\d+, \d+, 1>", "p2_c2r_x_gauge": r"k_c2r_x<\d+, \d+, 1>",
+    "p2_plane": r"k_plane<\d+, \d+, 1[,>]", "p2_c2r_x_gauge": r"k_c2r_x<\d+, \d+, 1>",
     "velocity_sharp": r"k_velocity<", "advect": r"k_advect\(",
 }
 
