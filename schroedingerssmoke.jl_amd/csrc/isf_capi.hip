@@ -1126,7 +1126,7 @@ static int time_x_tile(isf_batch_t b, const BatchGroup& gr, int ty, const Inst* 
         (void)hipEventDestroy(e0);
         return hip_fail(e, "hipEventCreate");
     }
-    const int reps = 5;
+    const int reps = 20;
     e = hipEventRecord(e0, b->stream);
     for (int k = 0; !r && e == hipSuccess && k < reps; k++) r = once();
     if (!r && e == hipSuccess) e = hipEventRecord(e1, b->stream);
@@ -1160,7 +1160,7 @@ static int time_planes(isf_batch_t b, const BatchGroup& gr, int pe, const Inst* 
         (void)hipEventDestroy(e0);
         return hip_fail(e, "hipEventCreate");
     }
-    const int reps = 5;
+    const int reps = 20;
     e = hipEventRecord(e0, b->stream);
     for (int k = 0; !r && e == hipSuccess && k < reps; k++) r = once();
     if (!r && e == hipSuccess) e = hipEventRecord(e1, b->stream);
