@@ -11,7 +11,10 @@
 // the per-step tape (step input psi0, flow output psi1, first gauge output psi3, the two
 // gauge phases q1 q2, particles after the spawn) and then reversed step by step.  The
 // recompute of segment s-1 runs on a second stream, with its own scratch and the other of
-// two tape sets, while segment s is reversed on the handle's stream.
+// two tape sets, while segment s is reversed on the handle's stream.  The particle part
+// of the reverse (velocity, advection adjoint, spawn adjoint) only feeds each step's
+// velocity cotangent into the psi chain, so it runs ahead on a third stream, into one
+// velocity-cotangent buffer per step of the tape set.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -66,9 +69,15 @@ struct isf_rollout {
     // velocity, particles); rec_done[i] / adj_done[i]: tape set i recomputed / reversed
     Work rw;
     hipEvent_t rec_done[2], adj_done[2], start;
+    // particle stream of the backward: vbar[i][k] = velocity cotangent of step k of the
+    // segment in tape set i (ready at pev[i][k]); pdone[i]: that segment's particle part done
+    hipStream_t pst;
+    std::vector<float*> vbar[2];
+    std::vector<hipEvent_t> pev[2];
+    hipEvent_t pdone[2];
     // work
     float4 *W, *psibar;
-    float *f, *fbar, *vel, *velbar, *parts, *parts_final, *pbar, *phibar, *target, *resid;
+    float *f, *fbar, *vel, *parts, *parts_final, *pbar, *phibar, *target, *resid;
     double *rho, *partials, *loss_dev, *vacc;
     double* splat_part;  // per-block density grids (null: global atomics)
     int* adj_order;      // cell-sorted slot order of the advection adjoint
@@ -94,11 +103,18 @@ static int check_rollout(isf_rollout_t r) {
 static void rollout_free(isf_rollout_t r) {
     if (r->h && r->h->stream) (void)hipStreamSynchronize(r->h->stream);
     if (r->rw.st) (void)hipStreamSynchronize(r->rw.st);
+    if (r->pst) (void)hipStreamSynchronize(r->pst);
     for (int i = 0; i < 2; i++) {
         if (r->rec_done[i]) (void)hipEventDestroy(r->rec_done[i]);
         if (r->adj_done[i]) (void)hipEventDestroy(r->adj_done[i]);
     }
     if (r->start) (void)hipEventDestroy(r->start);
+    for (int i = 0; i < 2; i++) {
+        for (hipEvent_t e : r->pev[i])
+            if (e) (void)hipEventDestroy(e);
+        if (r->pdone[i]) (void)hipEventDestroy(r->pdone[i]);
+    }
+    if (r->pst) (void)hipStreamDestroy(r->pst);
     if (r->rw.st) (void)hipStreamDestroy(r->rw.st);
     for (void* p : r->allocs) (void)hipFree(p);
     r->magic = 0;
@@ -212,29 +228,42 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
     return ISF_OK;
 }
 
-// Reverse of step k of segment s from its tape tp, on the handle's stream.
-static int step_adjoint(isf_rollout_t r, const Tape& tp, int s, int k) {
+// Particle part of the reverse of step k of segment s (tape tp), on the particle stream:
+// the step's velocity field, the advection adjoint (pbar in place, the velocity cotangent
+// into vb) and the spawn adjoint.
+static int step_adjoint_particles(isf_rollout_t r, const Tape& tp, int s, int k, float* vb) {
+    isf_t h = r->h;
+    hipStream_t st = r->pst;
+    const GridInfo& g = r->g;
+    const int it = s * r->K + k + 1;
+    const float4* psi5 = psi0_of(r, tp, s, k + 1);
+    // :59-63 velocity and advection
+    LAUNCH_CHECK(launch_fill_f32(vb, 0.0f, 3 * r->N, st), "fill");
+    if (r->P > 0) {
+        LAUNCH_CHECK(launch_velocity(g, psi5, h->hbar, 1, r->vel, st), "velocity_one_form");
+        // cell-sorted slot order: long same-cell runs for the scatter's wavefront reduction
+        LAUNCH_CHECK(launch_cell_order(g, tp.p0[k], r->P, r->adj_order, r->adj_order_scratch,
+                                                 r->adj_order_bytes, st),
+                     "particle order");
+        LAUNCH_CHECK(launch_advect_adj(g, tp.p0[k], r->P, r->adj_order, r->vel, h->dt, r->pbar, vb, st),
+                     "advect adjoint");
+        long long first0, nsp;
+        spawn_slots(r, it, &first0, &nsp);
+        LAUNCH_CHECK(launch_zero_slots(r->pbar, r->P, first0, nsp, st), "spawn adjoint");
+    }
+    return ISF_OK;
+}
+
+// The psi part of the reverse of step k of segment s from its tape tp, on the handle's
+// stream, with the step's velocity cotangent vb.
+static int step_adjoint(isf_rollout_t r, const Tape& tp, int s, int k, const float* vb) {
     isf_t h = r->h;
     hipStream_t st = h->stream;
     const GridInfo& g = r->g;
     const int it = s * r->K + k + 1;
     const float4* psi5 = psi0_of(r, tp, s, k + 1);
     const JetParams j = jet_at(r, it);
-    // :59-63 velocity and advection
-    LAUNCH_CHECK(launch_velocity(g, psi5, h->hbar, 1, r->vel, st), "velocity_one_form");
-    LAUNCH_CHECK(launch_fill_f32(r->velbar, 0.0f, 3 * r->N, st), "fill");
-    if (r->P > 0) {
-        // cell-sorted slot order: long same-cell runs for the scatter's wavefront reduction
-        LAUNCH_CHECK(launch_cell_order(g, tp.p0[k], r->P, r->adj_order, r->adj_order_scratch,
-                                                 r->adj_order_bytes, st),
-                     "particle order");
-        LAUNCH_CHECK(launch_advect_adj(g, tp.p0[k], r->P, r->adj_order, r->vel, h->dt, r->pbar, r->velbar, st),
-                     "advect adjoint");
-        long long first0, nsp;
-        spawn_slots(r, it, &first0, &nsp);
-        LAUNCH_CHECK(launch_zero_slots(r->pbar, r->P, first0, nsp, st), "spawn adjoint");
-    }
-    LAUNCH_CHECK(launch_oneform_adj_vel(g, r->psibar, psi5, r->velbar, h->hbar, 1, st), "velocity adjoint");
+    LAUNCH_CHECK(launch_oneform_adj_vel(g, r->psibar, psi5, vb, h->hbar, 1, st), "velocity adjoint");
     // :58 second projection (gauge output psi5, one-form at psi4 = restrict(psi3))
     LAUNCH_CHECK(launch_gauge_qbar(g, r->psibar, psi5, r->f, st), "gauge adjoint");
     int e = poisson(r, r->f, r->fbar);
@@ -351,7 +380,10 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
     RA(&r->f, N);
     RA(&r->fbar, N);
     RA(&r->vel, 3 * N);
-    RA(&r->velbar, 3 * N);
+    for (int i = 0; i < 2; i++) {
+        r->vbar[i].resize(r->K);
+        for (auto& p : r->vbar[i]) RA(&p, 3 * N);
+    }
     RA(&r->parts, P3);
     RA(&r->parts_final, P3);
     RA(&r->pbar, P3);
@@ -376,6 +408,13 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
             if (he == hipSuccess) he = hipEventCreateWithFlags(&r->adj_done[i], hipEventDisableTiming);
         }
         if (he == hipSuccess) he = hipEventCreateWithFlags(&r->start, hipEventDisableTiming);
+        if (he == hipSuccess) he = hipStreamCreate(&r->pst);
+        for (int i = 0; i < 2 && he == hipSuccess; i++) {
+            r->pev[i].assign(r->K, nullptr);
+            for (int k = 0; k < r->K && he == hipSuccess; k++)
+                he = hipEventCreateWithFlags(&r->pev[i][k], hipEventDisableTiming);
+            if (he == hipSuccess) he = hipEventCreateWithFlags(&r->pdone[i], hipEventDisableTiming);
+        }
         if (he != hipSuccess) return bail(hip_fail(he, "rollout streams"));
     }
     hipStream_t st = h->stream;
@@ -479,11 +518,18 @@ int isf_rollout_backward(isf_rollout_t r, double grad_velocity[3], float* grad_t
     // segments last to first: segment s is recomputed into tape set s % 2 on the recompute
     // stream (after the adjoint of segment s + 2 has released that set) and reversed on
     // the handle's stream, so the recompute of s - 1 overlaps the reverse of s
+    // The particle part of segment s runs on the particle stream after that recompute (and
+    // after the reverse of segment s + 2 has read the cotangent buffers of the set); each
+    // step's psi part waits for its cotangent.
     HIP_CHECK(hipEventRecord(r->start, st), "hipEventRecord");
     HIP_CHECK(hipStreamWaitEvent(r->rw.st, r->start, 0), "hipStreamWaitEvent");
+    HIP_CHECK(hipStreamWaitEvent(r->pst, r->start, 0), "hipStreamWaitEvent");
     auto recompute = [&](int s) -> int {
         const int b = s % 2;
-        if (s + 2 < r->nseg) HIP_CHECK(hipStreamWaitEvent(r->rw.st, r->adj_done[b], 0), "hipStreamWaitEvent");
+        if (s + 2 < r->nseg) {
+            HIP_CHECK(hipStreamWaitEvent(r->rw.st, r->adj_done[b], 0), "hipStreamWaitEvent");
+            HIP_CHECK(hipStreamWaitEvent(r->rw.st, r->pdone[b], 0), "hipStreamWaitEvent");
+        }
         int e2 = seg_forward(r, s, r->tape[b], r->rw, false);
         if (e2) return e2;
         HIP_CHECK(hipEventRecord(r->rec_done[b], r->rw.st), "hipEventRecord");
@@ -492,12 +538,22 @@ int isf_rollout_backward(isf_rollout_t r, double grad_velocity[3], float* grad_t
     if ((e = recompute(r->nseg - 1))) return e;
     for (int s = r->nseg - 1; s >= 0; s--) {
         const int b = s % 2;
+        HIP_CHECK(hipStreamWaitEvent(r->pst, r->rec_done[b], 0), "hipStreamWaitEvent");
+        if (s + 2 < r->nseg) HIP_CHECK(hipStreamWaitEvent(r->pst, r->adj_done[b], 0), "hipStreamWaitEvent");
+        for (int k = seg_steps(r, s) - 1; k >= 0; k--) {
+            if ((e = step_adjoint_particles(r, r->tape[b], s, k, r->vbar[b][k]))) return e;
+            HIP_CHECK(hipEventRecord(r->pev[b][k], r->pst), "hipEventRecord");
+        }
+        HIP_CHECK(hipEventRecord(r->pdone[b], r->pst), "hipEventRecord");
         HIP_CHECK(hipStreamWaitEvent(st, r->rec_done[b], 0), "hipStreamWaitEvent");
         if (s > 0 && (e = recompute(s - 1))) return e;
-        for (int k = seg_steps(r, s) - 1; k >= 0; k--)
-            if ((e = step_adjoint(r, r->tape[b], s, k))) return e;
+        for (int k = seg_steps(r, s) - 1; k >= 0; k--) {
+            HIP_CHECK(hipStreamWaitEvent(st, r->pev[b][k], 0), "hipStreamWaitEvent");
+            if ((e = step_adjoint(r, r->tape[b], s, k, r->vbar[b][k]))) return e;
+        }
         HIP_CHECK(hipEventRecord(r->adj_done[b], st), "hipEventRecord");
     }
+    HIP_CHECK(hipStreamWaitEvent(st, r->pdone[0], 0), "hipStreamWaitEvent");  // pbar complete
     // set-up constraint iterations in reverse
     for (int j = r->C - 1; j >= 0; j--) {
         const JetParams j0 = jet_at(r, 0);
