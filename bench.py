@@ -430,7 +430,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--batch-steps", type=int, default=60)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint rollout leg")
-    ap.add_argument("--adjoint-reps", type=int, default=3)
+    ap.add_argument("--adjoint-reps", type=int, default=5)
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--no-large", action="store_true", help="skip the 512x256x256 and 128^3 adjoint-batch legs")
     ap.add_argument("--no-export", action="store_true", help="skip the particle export timing")

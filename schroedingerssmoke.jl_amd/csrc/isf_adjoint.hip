@@ -197,17 +197,34 @@ __device__ __forceinline__ void add_mul(float4& acc, float2 zb, float4 a) {
     acc.w += zb.x * a.w + zb.y * a.z;
 }
 
-// Adjoint of a one-form evaluated at psi, fused with the adjoint of what consumed it:
+// The psi a one-form was evaluated at, formed from the stored psi on the fly: as stored
+// (PRE_NONE), normalized (PRE_NORMALIZE: the first projection's input normalize(psi1)) or
+// jet-restricted (PRE_RESTRICT: the second projection's input restrict(psi3)).  Same
+// arithmetic as k_normalize_to and k_restrict_to.
+template <int PRE>
+__device__ __forceinline__ float4 psi_pre(const float4* __restrict__ psi, size_t i, int x, int y, int z,
+                                          const GridInfo& g, const JetParams& j) {
+    const float4 p = psi[i];
+    if (PRE == PRE_NORMALIZE) return normalized(p);
+    if (PRE == PRE_RESTRICT) return is_jet_cell(x, y, z, g, j) ? jet_phase(p, x, y, z, g, j) : p;
+    return p;
+}
+
+// Adjoint of a one-form evaluated at psi (formed per PRE), fused with the adjoint of what
+// consumed it:
 //   OF_DIV: div then a Poisson solve produced q; w = fbar (Poisson-solved qbar),
 //           edge weight (fbar(c) - fbar(c + e_a)) / d_a^2, hbar = 1 (src/parallel.jl:156)
 //   OF_VEL: velocity_one_form(hbar), with 1/d when sharp (test/runtests.jl:59-62); w =
 //           velbar (3 per cell), edge weight velbar_a(c) hbar (/ d_a)
 // psibar(c) <- rot(psibar(c), +q(c)) (the gauge's own adjoint, when q != nullptr) plus the
-// one-form contributions of the six edges that touch c.
-template <int MODE>
+// one-form contributions of the six edges that touch c.  With qbar_out, also the gauge
+// cotangent of the new psibar against psi (k_gauge_qbar with psi_out = psi), for a gauge
+// transform whose output psi is.
+template <int MODE, int PRE>
 __global__ void __launch_bounds__(256)
 k_oneform_adj(float4* __restrict__ psibar, const float4* __restrict__ psi, const float* __restrict__ q,
-              const float* __restrict__ w, GridInfo g, float hbar, int sharp) {
+              const float* __restrict__ w, GridInfo g, float hbar, int sharp, JetParams j,
+              float* __restrict__ qbar_out) {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells(g)) return;
     int x, y, z;
@@ -220,7 +237,7 @@ k_oneform_adj(float4* __restrict__ psibar, const float4* __restrict__ psi, const
         sincosf(q[c], &s, &co);
         acc = make_float4(acc.x * co - acc.y * s, acc.x * s + acc.y * co, acc.z * co - acc.w * s, acc.z * s + acc.w * co);
     }
-    const float4 pc = psi[c];
+    const float4 pc = psi_pre<PRE>(psi, c, x, y, z, g, j);
     const float wc = MODE == OF_DIV ? w[c] : 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -228,7 +245,8 @@ k_oneform_adj(float4* __restrict__ psibar, const float4* __restrict__ psi, const
         cp[a] = cp[a] + 1 == n3[a] ? 0 : cp[a] + 1;
         cm[a] = cm[a] == 0 ? n3[a] - 1 : cm[a] - 1;
         const size_t ip = cidx(cp[0], cp[1], cp[2], g), im = cidx(cm[0], cm[1], cm[2], g);
-        const float4 pp = psi[ip], pm = psi[im];
+        const float4 pp = psi_pre<PRE>(psi, ip, cp[0], cp[1], cp[2], g, j);
+        const float4 pm = psi_pre<PRE>(psi, im, cm[0], cm[1], cm[2], g, j);
         float wf, wb;  // weights of the forward edge (c, c+e_a) and the backward edge (c-e_a, c)
         if (MODE == OF_DIV) {
             const float id2 = 1.0f / (d[a] * d[a]);
@@ -246,20 +264,28 @@ k_oneform_adj(float4* __restrict__ psibar, const float4* __restrict__ psi, const
         add_mul(acc, angle_zbar(re, imz, wb), pm);       // c is the second end of (c-e_a, c)
     }
     psibar[c] = acc;
+    if (qbar_out) qbar_out[c] = (acc.x * pc.y - acc.y * pc.x) + (acc.z * pc.w - acc.w * pc.z);
 }
 
 // Adjoint of restrict_velocity (jet cells) at its input psi_in, in place on psibar; per
 // block FP64 partial sums of phibar and phibar * pos_a (4 per block) for the gradient with
-// respect to kvec and omega t.
+// respect to kvec and omega t.  With qbar_out, also the gauge cotangent of the new psibar
+// against psi_in (k_gauge_qbar with psi_out = psi_in), for a gauge transform whose output
+// psi_in is.
 __global__ void __launch_bounds__(256)
 k_restrict_adj(float4* __restrict__ psibar, const float4* __restrict__ psi_in, GridInfo g, JetParams j,
-               double* __restrict__ partials) {
+               double* __restrict__ partials, float* __restrict__ qbar_out) {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     if (c < ncells(g)) {
         int x, y, z;
         cell_xyz(c, g, x, y, z);
-        if (is_jet_cell(x, y, z, g, j)) {
+        const bool jet = is_jet_cell(x, y, z, g, j);
+        if (qbar_out && !jet) {
+            const float4 b = psibar[c], p = psi_in[c];
+            qbar_out[c] = (b.x * p.y - b.y * p.x) + (b.z * p.w - b.w * p.z);
+        }
+        if (jet) {
             const float4 p = psi_in[c], b = psibar[c];
             const float4 o = jet_phase(p, x, y, z, g, j);
             const float a1 = hypotf(p.x, p.y), a2 = hypotf(p.z, p.w);
@@ -269,7 +295,9 @@ k_restrict_adj(float4* __restrict__ psibar, const float4* __restrict__ psi_in, G
             const float r1 = b.x * e1r + b.y * e1i, r2 = b.z * e2r + b.w * e2i;  // Re(conj(e) bar)
             const float phib = a1 * (b.y * e1r - b.x * e1i) + a2 * (b.w * e2r - b.z * e2i);
             const float i1 = a1 > 0.0f ? 1.0f / a1 : 0.0f, i2 = a2 > 0.0f ? 1.0f / a2 : 0.0f;
-            psibar[c] = make_float4(r1 * p.x * i1, r1 * p.y * i1, r2 * p.z * i2, r2 * p.w * i2);
+            const float4 nb = make_float4(r1 * p.x * i1, r1 * p.y * i1, r2 * p.z * i2, r2 * p.w * i2);
+            psibar[c] = nb;
+            if (qbar_out) qbar_out[c] = (nb.x * p.y - nb.y * p.x) + (nb.z * p.w - nb.w * p.z);
             const float px = __fmul_rn((float)x, g.dx), py = __fmul_rn((float)y, g.dy), pz = __fmul_rn((float)z, g.dz);
             s0 = phib;
             s1 = (double)phib * px;
@@ -742,17 +770,29 @@ int launch_gauge_qbar(const GridInfo& g, const float4* psibar, const float4* psi
     ISF_LAUNCH(k_gauge_qbar, grid1d(ncells(g), 256), dim3(256), s, psibar, psi_out, qbar, ncells(g));
 }
 int launch_oneform_adj_div(const GridInfo& g, float4* psibar, const float4* psi, const float* q, const float* fbar,
-                           hipStream_t s) {
-    ISF_LAUNCH(k_oneform_adj<OF_DIV>, grid1d(ncells(g), 256), dim3(256), s, psibar, psi, q, fbar, g, 1.0f, 0);
+                           hipStream_t s, int pre, const JetParams* j) {
+    const JetParams jp = j ? *j : JetParams{};
+    if (pre == PRE_NORMALIZE)
+        ISF_LAUNCH((k_oneform_adj<OF_DIV, PRE_NORMALIZE>), grid1d(ncells(g), 256), dim3(256), s, psibar, psi, q, fbar,
+                   g, 1.0f, 0, jp, nullptr);
+    if (pre == PRE_RESTRICT) {
+        if (!j) return (int)hipErrorInvalidValue;
+        ISF_LAUNCH((k_oneform_adj<OF_DIV, PRE_RESTRICT>), grid1d(ncells(g), 256), dim3(256), s, psibar, psi, q, fbar,
+                   g, 1.0f, 0, jp, nullptr);
+    }
+    ISF_LAUNCH((k_oneform_adj<OF_DIV, PRE_NONE>), grid1d(ncells(g), 256), dim3(256), s, psibar, psi, q, fbar, g, 1.0f,
+               0, jp, nullptr);
 }
 int launch_oneform_adj_vel(const GridInfo& g, float4* psibar, const float4* psi, const float* velbar, float hbar,
-                           int sharp, hipStream_t s) {
-    ISF_LAUNCH(k_oneform_adj<OF_VEL>, grid1d(ncells(g), 256), dim3(256), s, psibar, psi, nullptr, velbar, g, hbar, sharp);
+                           int sharp, hipStream_t s, float* qbar_out) {
+    ISF_LAUNCH((k_oneform_adj<OF_VEL, PRE_NONE>), grid1d(ncells(g), 256), dim3(256), s, psibar, psi, nullptr, velbar,
+               g, hbar, sharp, JetParams{}, qbar_out);
 }
 size_t restrict_adj_blocks(const GridInfo& g) { return (ncells(g) + 255) / 256; }
 int launch_restrict_adj(const GridInfo& g, float4* psibar, const float4* psi_in, const JetParams& j, double* partials,
-                        double* out4, hipStream_t s) {
-    hipLaunchKernelGGL(k_restrict_adj, grid1d(ncells(g), 256), dim3(256), 0, s, psibar, psi_in, g, j, partials);
+                        double* out4, hipStream_t s, float* qbar_out) {
+    hipLaunchKernelGGL(k_restrict_adj, grid1d(ncells(g), 256), dim3(256), 0, s, psibar, psi_in, g, j, partials,
+                       qbar_out);
     int e = (int)hipGetLastError();
     if (e) return e;
     ISF_LAUNCH(k_reduce4, dim3(1), dim3(256), s, partials, restrict_adj_blocks(g), out4);

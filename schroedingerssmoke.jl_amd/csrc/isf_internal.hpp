@@ -156,15 +156,20 @@ int launch_phase_gather(const GridInfo& g, const float* phibar, float* thbar, co
 int launch_normalize_to(const GridInfo& g, const float4* in, float4* out, hipStream_t s);
 int launch_restrict_to(const GridInfo& g, const float4* in, float4* out, const JetParams& j, hipStream_t s);
 int launch_gauge_qbar(const GridInfo& g, const float4* psibar, const float4* psi_out, float* qbar, hipStream_t s);
+// pre: how the one-form's psi is formed from the stored one (as stored, normalized, or
+// jet-restricted with *j)
+enum { PRE_NONE = 0, PRE_NORMALIZE = 1, PRE_RESTRICT = 2 };
 int launch_oneform_adj_div(const GridInfo& g, float4* psibar, const float4* psi, const float* q, const float* fbar,
-                           hipStream_t s);
+                           hipStream_t s, int pre = 0, const JetParams* j = nullptr);
+// qbar_out (optional): the gauge cotangent of the new psibar against psi
 int launch_oneform_adj_vel(const GridInfo& g, float4* psibar, const float4* psi, const float* velbar, float hbar,
-                           int sharp, hipStream_t s);
+                           int sharp, hipStream_t s, float* qbar_out = nullptr);
 // partials: restrict_adj_blocks(g) x 4 doubles; out4[k] += the block sums (sum phibar,
 // sum phibar x, sum phibar y, sum phibar z)
 size_t restrict_adj_blocks(const GridInfo& g);
+// qbar_out (optional): the gauge cotangent of the new psibar against psi_in
 int launch_restrict_adj(const GridInfo& g, float4* psibar, const float4* psi_in, const JetParams& j, double* partials,
-                        double* out4, hipStream_t s);
+                        double* out4, hipStream_t s, float* qbar_out = nullptr);
 int launch_normalize_adj(const GridInfo& g, float4* psibar, const float4* psi_in, hipStream_t s);
 // order: slot processing order (null: slot order); cell-sorted slots make the velocity
 // scatter of a wavefront hit few cells

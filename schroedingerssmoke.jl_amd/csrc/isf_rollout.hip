@@ -76,7 +76,7 @@ struct isf_rollout {
     std::vector<hipEvent_t> pev[2];
     hipEvent_t pdone[2];
     // work
-    float4 *W, *psibar;
+    float4* psibar;
     float *f, *fbar, *vel, *parts, *parts_final, *pbar, *phibar, *target, *resid;
     double *rho, *partials, *loss_dev, *vacc;
     double* splat_part;  // per-block density grids (null: global atomics)
@@ -263,21 +263,21 @@ static int step_adjoint(isf_rollout_t r, const Tape& tp, int s, int k, const flo
     const int it = s * r->K + k + 1;
     const float4* psi5 = psi0_of(r, tp, s, k + 1);
     const JetParams j = jet_at(r, it);
-    LAUNCH_CHECK(launch_oneform_adj_vel(g, r->psibar, psi5, vb, h->hbar, 1, st), "velocity adjoint");
+    // the velocity adjoint also forms the second projection's gauge cotangent (output psi5)
+    LAUNCH_CHECK(launch_oneform_adj_vel(g, r->psibar, psi5, vb, h->hbar, 1, st, r->f), "velocity adjoint");
     // :58 second projection (gauge output psi5, one-form at psi4 = restrict(psi3))
-    LAUNCH_CHECK(launch_gauge_qbar(g, r->psibar, psi5, r->f, st), "gauge adjoint");
     int e = poisson(r, r->f, r->fbar);
     if (e) return e;
-    LAUNCH_CHECK(launch_restrict_to(g, tp.psi3[k], r->W, j, st), "restrict_velocity");
-    LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, r->W, tp.q2[k], r->fbar, st), "projection adjoint");
-    // :57 jet constraint
-    LAUNCH_CHECK(launch_restrict_adj(g, r->psibar, tp.psi3[k], j, r->partials, r->vacc + 4 * (size_t)(it - 1), st),
+    LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, tp.psi3[k], tp.q2[k], r->fbar, st, PRE_RESTRICT, &j),
+                 "projection adjoint");
+    // :57 jet constraint, with the first projection's gauge cotangent (output psi3)
+    LAUNCH_CHECK(launch_restrict_adj(g, r->psibar, tp.psi3[k], j, r->partials, r->vacc + 4 * (size_t)(it - 1), st,
+                                     r->f),
                  "restrict adjoint");
     // :50 first projection (gauge output psi3, one-form at psi2 = normalize(psi1))
-    LAUNCH_CHECK(launch_gauge_qbar(g, r->psibar, tp.psi3[k], r->f, st), "gauge adjoint");
     if ((e = poisson(r, r->f, r->fbar))) return e;
-    LAUNCH_CHECK(launch_normalize_to(g, tp.psi1[k], r->W, st), "normalize");
-    LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, r->W, tp.q1[k], r->fbar, st), "projection adjoint");
+    LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, tp.psi1[k], tp.q1[k], r->fbar, st, PRE_NORMALIZE),
+                 "projection adjoint");
     // :49 normalize, :48 schroedinger_flow
     LAUNCH_CHECK(launch_normalize_adj(g, r->psibar, tp.psi1[k], st), "normalize adjoint");
     LAUNCH_CHECK(launch_flow_x_fwd(g, r->psibar, h->S, h->tb, st), "flow adjoint x forward");
@@ -375,7 +375,6 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
     RA(&r->rw.f, N);
     RA(&r->rw.vel, 3 * N);
     RA(&r->rw.parts, P3);
-    RA(&r->W, N);
     RA(&r->psibar, N);
     RA(&r->f, N);
     RA(&r->fbar, N);
@@ -555,14 +554,16 @@ int isf_rollout_backward(isf_rollout_t r, double grad_velocity[3], float* grad_t
     }
     HIP_CHECK(hipStreamWaitEvent(st, r->pdone[0], 0), "hipStreamWaitEvent");  // pbar complete
     // set-up constraint iterations in reverse
+    // (each restrict adjoint also forms the next iteration's gauge cotangent: the gauge
+    // output of iteration j - 1 is cpsi[j])
+    if (r->C > 0) LAUNCH_CHECK(launch_gauge_qbar(g, r->psibar, r->cpsi[r->C], r->f, st), "gauge adjoint");
     for (int j = r->C - 1; j >= 0; j--) {
         const JetParams j0 = jet_at(r, 0);
-        LAUNCH_CHECK(launch_gauge_qbar(g, r->psibar, r->cpsi[j + 1], r->f, st), "gauge adjoint");
         if ((e = poisson(r, r->f, r->fbar))) return e;
-        LAUNCH_CHECK(launch_restrict_to(g, r->cpsi[j], r->W, j0, st), "restrict_velocity");
-        LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, r->W, r->cq[j], r->fbar, st), "projection adjoint");
+        LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, r->cpsi[j], r->cq[j], r->fbar, st, PRE_RESTRICT, &j0),
+                     "projection adjoint");
         LAUNCH_CHECK(launch_restrict_adj(g, r->psibar, r->cpsi[j], j0, r->partials,
-                                         r->vacc + 4 * (size_t)(r->T + j), st),
+                                         r->vacc + 4 * (size_t)(r->T + j), st, j > 0 ? r->f : nullptr),
                      "restrict adjoint");
     }
     if (r->use_theta) {
