@@ -197,6 +197,15 @@ __device__ __forceinline__ void add_mul(float4& acc, float2 zb, float4 a) {
     acc.w += zb.x * a.w + zb.y * a.z;
 }
 
+// Cotangent of normalize's input u from the cotangent b of its output:
+// (b - n (n . b)) / |u| with n = u / |u|.
+__device__ __forceinline__ float4 normalize_adj_at(float4 u, float4 b) {
+    const float inv = 1.0f / sqrtf((u.x * u.x + u.y * u.y) + (u.z * u.z + u.w * u.w));
+    const float4 n = make_float4(u.x * inv, u.y * inv, u.z * inv, u.w * inv);
+    const float dn = (n.x * b.x + n.y * b.y) + (n.z * b.z + n.w * b.w);
+    return make_float4((b.x - n.x * dn) * inv, (b.y - n.y * dn) * inv, (b.z - n.z * dn) * inv, (b.w - n.w * dn) * inv);
+}
+
 // The psi a one-form was evaluated at, formed from the stored psi on the fly: as stored
 // (PRE_NONE), normalized (PRE_NORMALIZE: the first projection's input normalize(psi1)) or
 // jet-restricted (PRE_RESTRICT: the second projection's input restrict(psi3)).  Same
@@ -217,7 +226,8 @@ __device__ __forceinline__ float4 psi_pre(const float4* __restrict__ psi, size_t
 //   OF_VEL: velocity_one_form(hbar), with 1/d when sharp (test/runtests.jl:59-62); w =
 //           velbar (3 per cell), edge weight velbar_a(c) hbar (/ d_a)
 // psibar(c) <- rot(psibar(c), +q(c)) (the gauge's own adjoint, when q != nullptr) plus the
-// one-form contributions of the six edges that touch c.  With qbar_out, also the gauge
+// one-form contributions of the six edges that touch c.  With PRE_NORMALIZE the result is
+// the cotangent of the stored psi (the normalize adjoint applied last).  With qbar_out, also the gauge
 // cotangent of the new psibar against psi (k_gauge_qbar with psi_out = psi), for a gauge
 // transform whose output psi is.
 template <int MODE, int PRE>
@@ -263,6 +273,7 @@ k_oneform_adj(float4* __restrict__ psibar, const float4* __restrict__ psi, const
         edge_z(pm, pc, re, imz);
         add_mul(acc, angle_zbar(re, imz, wb), pm);       // c is the second end of (c-e_a, c)
     }
+    if (PRE == PRE_NORMALIZE) acc = normalize_adj_at(psi[c], acc);
     psibar[c] = acc;
     if (qbar_out) qbar_out[c] = (acc.x * pc.y - acc.y * pc.x) + (acc.z * pc.w - acc.w * pc.z);
 }
@@ -336,11 +347,7 @@ __global__ void __launch_bounds__(256)
 k_normalize_adj(float4* __restrict__ psibar, const float4* __restrict__ psi_in, size_t N) {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
-    const float4 u = psi_in[c], b = psibar[c];
-    const float inv = 1.0f / sqrtf((u.x * u.x + u.y * u.y) + (u.z * u.z + u.w * u.w));
-    const float4 n = make_float4(u.x * inv, u.y * inv, u.z * inv, u.w * inv);
-    const float dn = (n.x * b.x + n.y * b.y) + (n.z * b.z + n.w * b.w);
-    psibar[c] = make_float4((b.x - n.x * dn) * inv, (b.y - n.y * dn) * inv, (b.z - n.z * dn) * inv, (b.w - n.w * dn) * inv);
+    psibar[c] = normalize_adj_at(psi_in[c], psibar[c]);
 }
 
 // ---- particles

@@ -157,7 +157,8 @@ int launch_normalize_to(const GridInfo& g, const float4* in, float4* out, hipStr
 int launch_restrict_to(const GridInfo& g, const float4* in, float4* out, const JetParams& j, hipStream_t s);
 int launch_gauge_qbar(const GridInfo& g, const float4* psibar, const float4* psi_out, float* qbar, hipStream_t s);
 // pre: how the one-form's psi is formed from the stored one (as stored, normalized, or
-// jet-restricted with *j)
+// jet-restricted with *j); with PRE_NORMALIZE psibar comes out as the cotangent of the
+// stored psi (the normalize adjoint included)
 enum { PRE_NONE = 0, PRE_NORMALIZE = 1, PRE_RESTRICT = 2 };
 int launch_oneform_adj_div(const GridInfo& g, float4* psibar, const float4* psi, const float* q, const float* fbar,
                            hipStream_t s, int pre = 0, const JetParams* j = nullptr);

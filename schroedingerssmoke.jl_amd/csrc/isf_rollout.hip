@@ -274,12 +274,12 @@ static int step_adjoint(isf_rollout_t r, const Tape& tp, int s, int k, const flo
     LAUNCH_CHECK(launch_restrict_adj(g, r->psibar, tp.psi3[k], j, r->partials, r->vacc + 4 * (size_t)(it - 1), st,
                                      r->f),
                  "restrict adjoint");
-    // :50 first projection (gauge output psi3, one-form at psi2 = normalize(psi1))
+    // :50 first projection (gauge output psi3, one-form at psi2 = normalize(psi1)) and
+    // :49 normalize
     if ((e = poisson(r, r->f, r->fbar))) return e;
     LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, tp.psi1[k], tp.q1[k], r->fbar, st, PRE_NORMALIZE),
-                 "projection adjoint");
-    // :49 normalize, :48 schroedinger_flow
-    LAUNCH_CHECK(launch_normalize_adj(g, r->psibar, tp.psi1[k], st), "normalize adjoint");
+                 "projection and normalize adjoint");
+    // :48 schroedinger_flow
     LAUNCH_CHECK(launch_flow_x_fwd(g, r->psibar, h->S, h->tb, st), "flow adjoint x forward");
     LAUNCH_CHECK(launch_flow_plane_adj(g, h->S, h->tb, st), "flow adjoint plane");
     LAUNCH_CHECK(launch_flow_x_inv(g, h->S, r->psibar, false, h->tb, st), "flow adjoint x inverse");
