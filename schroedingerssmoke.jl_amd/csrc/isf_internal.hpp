@@ -78,6 +78,9 @@ int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const Dev
 int launch_flow_plane(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s, Batch b = Batch{});
 int launch_flow_x_inv(const GridInfo& g, const float2* S, float4* psi, bool normalize, const DevTables& t,
                       hipStream_t s, Batch b = Batch{});
+// the inverse x pass writing the raw psi and, beside it, normalize_psi of it
+int launch_flow_x_inv_raw_normalized(const GridInfo& g, const float2* S, float4* psi, float4* psin,
+                                     const DevTables& t, hipStream_t s);
 // the same x passes on the split-component layout (psi1 and psi2 as separate complex arrays)
 int launch_flow_x_fwd_split(const GridInfo& g, const float2* sa, const float2* sb, float2* S, const DevTables& t,
                             hipStream_t s);
@@ -90,10 +93,12 @@ int launch_div_r2c(const GridInfo& g, const float4* psi, float* f, float2* P, co
                    Batch b = Batch{});
 int launch_poisson_plane(const GridInfo& g, float2* P, double scale, const DevTables& t, hipStream_t s,
                          Batch b = Batch{});
-// mode 0: write q (float per cell) ; mode 1: gauge psi ; mode 2: gauge psi + restrict.
+// mode 0: write q (float per cell) ; mode 1: gauge psi ; mode 2: gauge psi + restrict ;
+// mode 3: gauge psi, and its restriction into rout (not in batch mode).
 // In batch mode norm_partials != nullptr requests the members' own partial sums.
 int launch_c2r_x(const GridInfo& g, const float2* P, int mode, float* q, float4* psi, const JetParams* jet,
-                 double* norm_partials, const DevTables& t, hipStream_t s, Batch b = Batch{});
+                 double* norm_partials, const DevTables& t, hipStream_t s, Batch b = Batch{},
+                 float4* rout = nullptr);
 size_t c2r_x_blocks(const GridInfo& g);
 
 // ---- stencil / pointwise

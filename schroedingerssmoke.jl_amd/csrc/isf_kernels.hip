@@ -120,10 +120,11 @@ k_flow_x_fwd(const float4* __restrict__ psi, const float2* __restrict__ sa, cons
 // Inverse x transform from S back to psi lines; optionally normalize_psi
 // (src/parallel.jl:162-165,184-185).  The 1/N of ifft is folded into the mask.  With
 // oa != null the components go to two separate complex arrays oa, ob instead of psi.
+// Without NORMALIZE, psin != null receives the normalized psi beside the raw one in psi.
 template <int NX, int TY, bool NORMALIZE>
 __global__ void __launch_bounds__(2 * TY * XCfg<NX>::T)
 k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, float2* __restrict__ oa, float2* __restrict__ ob,
-             int ny, int nz, const float2* __restrict__ tw, Batch B) {
+             float4* __restrict__ psin, int ny, int nz, const float2* __restrict__ tw, Batch B) {
     constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
     const StTile tl = stencil_tile();  // x: y tile, y: z, z: member
     if (B.inst) {
@@ -171,6 +172,7 @@ k_flow_x_inv(const float2* __restrict__ S, float4* __restrict__ psi, float2* __r
             float4 p = make_float4(a.x, a.y, b.x, b.y);
             if (NORMALIZE) p = normalized(p);
             dst[i] = p;
+            if (!NORMALIZE && psin) psin[line0 + i] = normalized(p);
         }
     }
 }
@@ -550,7 +552,7 @@ k_line_z_reg(float2* __restrict__ base, int nx, int ny, double scale, DevTables 
     Z2I::store(v, col, t);
 }
 
-enum { C2R_Q = 0, C2R_GAUGE = 1, C2R_GAUGE_RESTRICT = 2 };
+enum { C2R_Q = 0, C2R_GAUGE = 1, C2R_GAUGE_RESTRICT = 2, C2R_GAUGE_AND_RESTRICTED = 3 };
 
 // Complex-to-real inverse x transform of TY lines of P (src/parallel.jl:172) and, in the
 // gauge modes, psi <- psi * exp(-i q) (src/parallel.jl:125-129,159) with the jet
@@ -559,7 +561,7 @@ enum { C2R_Q = 0, C2R_GAUGE = 1, C2R_GAUGE_RESTRICT = 2 };
 template <int NX, int TY, int MODE>
 __global__ void __launch_bounds__(TY * C2rCfg<NX / 2>::T)
 k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restrict__ psi, GridInfo g, JetParams jet,
-        double* __restrict__ partials, const float2* __restrict__ tw, Batch B) {
+        double* __restrict__ partials, const float2* __restrict__ tw, Batch B, float4* __restrict__ rout) {
     constexpr int M = NX / 2;
     const StTile tl = stencil_tile();  // x: y tile, y: z, z: member
     if (B.inst) {
@@ -645,6 +647,8 @@ k_c2r_x(const float2* __restrict__ P, float* __restrict__ qout, float4* __restri
                 if (MODE == C2R_GAUGE_RESTRICT && is_jet_cell(x, y0 + l, z, g, jet)) p = jet_phase(p, x, y0 + l, z, g, jet);
                 if (partials) acc += (double)p.x * p.x + (double)p.y * p.y + (double)p.z * p.z + (double)p.w * p.w;
                 psi[row0 + i] = p;
+                if (MODE == C2R_GAUGE_AND_RESTRICTED)  // the gauge output in psi, its restriction in rout
+                    rout[row0 + i] = is_jet_cell(x, y0 + l, z, g, jet) ? jet_phase(p, x, y0 + l, z, g, jet) : p;
             }
         }
     }
@@ -1160,13 +1164,14 @@ static int flow_x_fwd_t(const GridInfo& g, const float4* psi, const float2* sa, 
     ISF_LAUNCH((k_flow_x_fwd<NX, TY>), grid, dim3(2 * TY * XCfg<NX>::T), s, psi, sa, sb, S, g.ny, g.nz, t.tw, b);
 }
 template <int NX, int TY>
-static int flow_x_inv_t(const GridInfo& g, const float2* S, float4* psi, float2* oa, float2* ob, bool nrm,
-                        const DevTables& t, hipStream_t s, Batch b) {
+static int flow_x_inv_t(const GridInfo& g, const float2* S, float4* psi, float2* oa, float2* ob, float4* psin,
+                        bool nrm, const DevTables& t, hipStream_t s, Batch b) {
     dim3 grid(g.ny / TY, g.nz, members_of(b));
     if (nrm)
-        ISF_LAUNCH((k_flow_x_inv<NX, TY, true>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, oa, ob, g.ny, g.nz, t.tw,
-                   b);
-    ISF_LAUNCH((k_flow_x_inv<NX, TY, false>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, oa, ob, g.ny, g.nz, t.tw, b);
+        ISF_LAUNCH((k_flow_x_inv<NX, TY, true>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, oa, ob, nullptr, g.ny,
+                   g.nz, t.tw, b);
+    ISF_LAUNCH((k_flow_x_inv<NX, TY, false>), grid, dim3(2 * TY * XCfg<NX>::T), s, S, psi, oa, ob, psin, g.ny, g.nz,
+               t.tw, b);
 }
 template <int NX, int TY>
 static int r2c_x_t(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s, Batch b) {
@@ -1175,14 +1180,19 @@ static int r2c_x_t(const GridInfo& g, const float* f, float2* P, const DevTables
 }
 template <int NX, int TY>
 static int c2r_x_t(const GridInfo& g, const float2* P, int mode, float* q, float4* psi, const JetParams* jet,
-                   double* parts, const DevTables& t, hipStream_t s, Batch b) {
+                   double* parts, const DevTables& t, hipStream_t s, Batch b, float4* rout) {
     dim3 grid(g.ny / TY, g.nz, members_of(b));
     dim3 block(TY * C2rCfg<NX / 2>::T);
     JetParams j{};
     if (jet) j = *jet;
-    if (mode == C2R_Q) ISF_LAUNCH((k_c2r_x<NX, TY, C2R_Q>), grid, block, s, P, q, psi, g, j, parts, t.tw, b);
-    if (mode == C2R_GAUGE) ISF_LAUNCH((k_c2r_x<NX, TY, C2R_GAUGE>), grid, block, s, P, q, psi, g, j, parts, t.tw, b);
-    ISF_LAUNCH((k_c2r_x<NX, TY, C2R_GAUGE_RESTRICT>), grid, block, s, P, q, psi, g, j, parts, t.tw, b);
+    if (mode == C2R_Q) ISF_LAUNCH((k_c2r_x<NX, TY, C2R_Q>), grid, block, s, P, q, psi, g, j, parts, t.tw, b, nullptr);
+    if (mode == C2R_GAUGE)
+        ISF_LAUNCH((k_c2r_x<NX, TY, C2R_GAUGE>), grid, block, s, P, q, psi, g, j, parts, t.tw, b, nullptr);
+    if (mode == C2R_GAUGE_AND_RESTRICTED) {
+        if (!jet || !rout || b.inst) return (int)hipErrorInvalidValue;
+        ISF_LAUNCH((k_c2r_x<NX, TY, C2R_GAUGE_AND_RESTRICTED>), grid, block, s, P, q, psi, g, j, parts, t.tw, b, rout);
+    }
+    ISF_LAUNCH((k_c2r_x<NX, TY, C2R_GAUGE_RESTRICT>), grid, block, s, P, q, psi, g, j, parts, t.tw, b, nullptr);
 }
 
 // dispatch on (nx, tile) for the x passes
@@ -1212,7 +1222,11 @@ int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const Dev
 }
 int launch_flow_x_inv(const GridInfo& g, const float2* S, float4* psi, bool normalize, const DevTables& t,
                       hipStream_t s, Batch b) {
-    ISF_X_DISPATCH(flow_x_inv_t, g, S, psi, nullptr, nullptr, normalize, t, s, b);
+    ISF_X_DISPATCH(flow_x_inv_t, g, S, psi, nullptr, nullptr, nullptr, normalize, t, s, b);
+}
+int launch_flow_x_inv_raw_normalized(const GridInfo& g, const float2* S, float4* psi, float4* psin,
+                                     const DevTables& t, hipStream_t s) {
+    ISF_X_DISPATCH(flow_x_inv_t, g, S, psi, nullptr, nullptr, psin, false, t, s, Batch{});
 }
 int launch_flow_x_fwd_split(const GridInfo& g, const float2* sa, const float2* sb, float2* S, const DevTables& t,
                             hipStream_t s) {
@@ -1220,7 +1234,7 @@ int launch_flow_x_fwd_split(const GridInfo& g, const float2* sa, const float2* s
 }
 int launch_flow_x_inv_split(const GridInfo& g, const float2* S, float2* oa, float2* ob, const DevTables& t,
                             hipStream_t s) {
-    ISF_X_DISPATCH(flow_x_inv_t, g, S, nullptr, oa, ob, false, t, s, Batch{});
+    ISF_X_DISPATCH(flow_x_inv_t, g, S, nullptr, oa, ob, nullptr, false, t, s, Batch{});
 }
 int launch_r2c_x(const GridInfo& g, const float* f, float2* P, const DevTables& t, hipStream_t s, Batch b) {
     ISF_X_DISPATCH(r2c_x_t, g, f, P, t, s, b);
@@ -1244,8 +1258,8 @@ int launch_div_r2c(const GridInfo& g, const float4* psi, float* f, float2* P, co
     ISF_X_DISPATCH(div_r2c_t, g, psi, P, t, s, b);
 }
 int launch_c2r_x(const GridInfo& g, const float2* P, int mode, float* q, float4* psi, const JetParams* jet,
-                 double* norm_partials, const DevTables& t, hipStream_t s, Batch b) {
-    ISF_X_DISPATCH(c2r_x_t, g, P, mode, q, psi, jet, norm_partials, t, s, b);
+                 double* norm_partials, const DevTables& t, hipStream_t s, Batch b, float4* rout) {
+    ISF_X_DISPATCH(c2r_x_t, g, P, mode, q, psi, jet, norm_partials, t, s, b, rout);
 }
 size_t c2r_x_blocks(const GridInfo& g) { return (size_t)(g.ny / x_tile(g)) * g.nz; }
 

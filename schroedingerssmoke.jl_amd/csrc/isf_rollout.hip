@@ -160,13 +160,20 @@ static int poisson(isf_rollout_t r, const float* fin, float* fout) {
     LAUNCH_CHECK(launch_c2r_x(r->g, P_of(r), 0, fout, nullptr, nullptr, nullptr, h->tb, h->stream), "poisson x inverse");
     return ISF_OK;
 }
-// pressure_project! in place keeping the gauge phase q (src/parallel.jl:155-160)
-static int project_q(isf_rollout_t r, float4* psi, float* q, const Work& w) {
+// pressure_project! in place keeping the gauge phase q (src/parallel.jl:155-160); with
+// restricted != null the gauge pass also writes restrict_velocity! of its output there
+// (src/particles.jl:12-27, jet parameters *j).
+static int project_q(isf_rollout_t r, float4* psi, float* q, const Work& w, float4* restricted = nullptr,
+                     const JetParams* j = nullptr) {
     isf_t h = r->h;
     float2* P = P_in(r, w.S);
     LAUNCH_CHECK(launch_div_r2c(r->g, psi, w.f, P, h->tb, w.st), "div(one_form(psi)) + poisson x forward");
     LAUNCH_CHECK(launch_poisson_plane(r->g, P, poisson_scale(r), h->tb, w.st), "poisson plane");
-    LAUNCH_CHECK(launch_c2r_x(r->g, P, 1, q, psi, nullptr, nullptr, h->tb, w.st), "poisson x inverse + gauge");
+    if (restricted)
+        LAUNCH_CHECK(launch_c2r_x(r->g, P, 3, q, psi, j, nullptr, h->tb, w.st, Batch{}, restricted),
+                     "poisson x inverse + gauge + restrict_velocity");
+    else
+        LAUNCH_CHECK(launch_c2r_x(r->g, P, 1, q, psi, nullptr, nullptr, h->tb, w.st), "poisson x inverse + gauge");
     return ISF_OK;
 }
 
@@ -214,11 +221,12 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
             HIP_CHECK(hipMemcpyAsync(tp.p0[k], w.parts, pb, hipMemcpyDeviceToDevice, st), "particles tape");
         LAUNCH_CHECK(launch_flow_x_fwd(r->g, in, w.S, h->tb, st), "flow x forward");
         LAUNCH_CHECK(launch_flow_plane(r->g, w.S, h->tb, st), "flow plane");
-        LAUNCH_CHECK(launch_flow_x_inv(r->g, w.S, tp.psi1[k], false, h->tb, st), "flow x inverse");
-        LAUNCH_CHECK(launch_normalize_to(r->g, tp.psi1[k], tp.psi3[k], st), "normalize");
-        int e = project_q(r, tp.psi3[k], tp.q1[k], w);
+        // psi1 for the tape, normalize_psi of it into psi3 (projected in place next)
+        LAUNCH_CHECK(launch_flow_x_inv_raw_normalized(r->g, w.S, tp.psi1[k], tp.psi3[k], h->tb, st),
+                     "flow x inverse + normalize");
+        const JetParams j = jet_at(r, it);
+        int e = project_q(r, tp.psi3[k], tp.q1[k], w, out, &j);  // out = restrict_velocity!(psi3)
         if (e) return e;
-        LAUNCH_CHECK(launch_restrict_to(r->g, tp.psi3[k], out, jet_at(r, it), st), "restrict_velocity");
         if ((e = project_q(r, out, tp.q2[k], w))) return e;
         LAUNCH_CHECK(launch_velocity(r->g, out, h->hbar, 1, w.vel, st), "velocity_one_form");
         if (r->P > 0) LAUNCH_CHECK(launch_advect(r->g, w.parts, r->P, nullptr, w.vel, h->dt, st), "staggered_advect");
@@ -481,10 +489,13 @@ int isf_rollout_forward(isf_rollout_t r, const float velocity[3], const float* t
     }
     // test/runtests.jl:27-30
     const Work mw = main_work(r);
-    for (int j = 0; j < r->C; j++) {
-        LAUNCH_CHECK(launch_restrict_to(r->g, r->cpsi[j], r->cpsi[j + 1], jet_at(r, 0), st), "restrict_velocity");
-        if ((e = project_q(r, r->cpsi[j + 1], r->cq[j], mw))) return e;
-    }
+    // cpsi[j + 1] = project(restrict(cpsi[j])); each projection's gauge pass also writes
+    // the next iteration's restricted input
+    const JetParams j0 = jet_at(r, 0);
+    if (r->C > 0)
+        LAUNCH_CHECK(launch_restrict_to(r->g, r->cpsi[0], r->cpsi[1], j0, st), "restrict_velocity");
+    for (int j = 0; j < r->C; j++)
+        if ((e = project_q(r, r->cpsi[j + 1], r->cq[j], mw, j + 1 < r->C ? r->cpsi[j + 2] : nullptr, &j0))) return e;
     for (int s = 0; s < r->nseg; s++)
         if ((e = seg_forward(r, s, r->tape[0], mw, true))) return e;
     if (r->P > 0)
