@@ -78,6 +78,14 @@ struct isf_rollout {
     // work
     float4* psibar;
     float *f, *fbar, *vel, *parts, *parts_final, *pbar, *phibar, *target, *resid;
+    // spawn batches of steps 1..T (n_spawn x 3 each, in slot order): they depend only on
+    // the nozzle geometry, the seed and the step, so a segment forward copies them in
+    // instead of drawing and sorting them again; built for spawn_key
+    float* spawn_tab;
+    long long spawn_n;
+    bool spawn_ready;
+    float spawn_key[4];
+    unsigned long long spawn_seed;
     double *rho, *partials, *loss_dev, *vacc;
     double* splat_part;  // per-block density grids (null: global atomics)
     int* adj_order;      // cell-sorted slot order of the advection adjoint
@@ -197,6 +205,39 @@ static void spawn_slots(isf_rollout_t r, int it, long long* first0, long long* n
     *n = r->h->n_spawn;
 }
 
+// Build the spawn table when missing or when the handle's nozzle or seed changed.
+static int ensure_spawn_table(isf_rollout_t r, hipStream_t st) {
+    isf_t h = r->h;
+    if (!r->spawn_tab || h->n_spawn != r->spawn_n) return ISF_OK;
+    const float key[4] = {h->jet.cx, h->jet.cy, h->jet.cz, h->jet.rad};
+    if (r->spawn_ready && r->spawn_seed == h->seed && std::memcmp(key, r->spawn_key, sizeof key) == 0) return ISF_OK;
+    for (int it = 1; it <= r->T; it++)
+        LAUNCH_CHECK(launch_spawn(r->spawn_tab + 3 * (size_t)r->spawn_n * (it - 1), r->spawn_n, 0, r->spawn_n, h->jet,
+                                  h->seed, (unsigned)it, st),
+                     "add_particles table");
+    std::memcpy(r->spawn_key, key, sizeof key);
+    r->spawn_seed = h->seed;
+    r->spawn_ready = true;
+    return ISF_OK;
+}
+// add_particles! of step it into the ring slots first0 .. first0 + nsp - 1 (mod P), from
+// the spawn table (the same values k_spawn writes there), else with k_spawn.
+static int spawn_step(isf_rollout_t r, int it, long long first0, long long nsp, float* parts, hipStream_t st) {
+    isf_t h = r->h;
+    if (!r->spawn_ready || nsp != r->spawn_n || nsp > r->P) {
+        LAUNCH_CHECK(launch_spawn(parts, r->P, first0, nsp, h->jet, h->seed, (unsigned)it, st), "add_particles");
+        return ISF_OK;
+    }
+    const float* src = r->spawn_tab + 3 * (size_t)nsp * (it - 1);
+    const long long n1 = std::min(nsp, r->P - first0);
+    HIP_CHECK(hipMemcpyAsync(parts + 3 * first0, src, sizeof(float) * 3 * n1, hipMemcpyDeviceToDevice, st),
+              "add_particles");
+    if (n1 < nsp)
+        HIP_CHECK(hipMemcpyAsync(parts, src + 3 * n1, sizeof(float) * 3 * (nsp - n1), hipMemcpyDeviceToDevice, st),
+                  "add_particles");
+    return ISF_OK;
+}
+
 // Steps of segment s from its checkpoint into tape tp, on w's stream and scratch.  The
 // forward (checkpoint = true) also writes the next checkpoint; a recompute writes only
 // the tape (its last step output is the next psi checkpoint again, rewritten with the
@@ -215,7 +256,8 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
         long long first0, nsp;
         spawn_slots(r, it, &first0, &nsp);
         if (nsp > 0) {
-            LAUNCH_CHECK(launch_spawn(w.parts, r->P, first0, nsp, h->jet, h->seed, (unsigned)it, st), "add_particles");
+            int e2 = spawn_step(r, it, first0, nsp, w.parts, st);
+            if (e2) return e2;
         }
         if (r->P > 0)
             HIP_CHECK(hipMemcpyAsync(tp.p0[k], w.parts, pb, hipMemcpyDeviceToDevice, st), "particles tape");
@@ -401,6 +443,8 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
     RA(&r->partials, std::max(4 * restrict_adj_blocks(r->g), splat_loss_blocks(r->M)));
     RA(&r->loss_dev, 1);
     RA(&r->vacc, 4 * (size_t)(r->T + r->C));
+    r->spawn_n = h->n_spawn;
+    if (r->P > 0 && r->spawn_n > 0 && r->spawn_n <= r->P) RA(&r->spawn_tab, 3 * (size_t)r->spawn_n * r->T);
     if (const size_t sp = splat_partial_doubles(r->sc)) RA(&r->splat_part, sp);
     if (r->P > 0) {
         r->adj_order_bytes = cell_order_scratch_bytes(r->g, r->P);
@@ -467,6 +511,7 @@ int isf_rollout_forward(isf_rollout_t r, const float velocity[3], const float* t
     if (e) return e;
     hipStream_t st = h->stream;
     r->forward_done = false;
+    if ((e = ensure_spawn_table(r, st))) return e;
     // kvec = v / hbar, omega = sum(v.^2) / (2 hbar) in Float32 (test/runtests.jl:23-24)
     for (int a = 0; a < 3; a++) r->v[a] = velocity[a];
     r->jet = h->jet;

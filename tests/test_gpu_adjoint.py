@@ -226,6 +226,30 @@ def test_rollout_forward_equals_isf_step():
     assert abs(loss - float(L)) < 1e-4 * float(L)
 
 
+def test_rollout_spawn_table_follows_the_nozzle():
+    """The rollout draws its spawn batches once per nozzle and seed and copies them in at
+    each step; moving the handle's nozzle between two forwards must redraw them, so the
+    second forward equals that of a rollout whose first forward sees the moved nozzle."""
+    from isf_amd.rollout import Rollout, gaussian_target
+    isf, r, tr, v = rollout_case(steps=3, K=2)
+    r2 = Rollout(isf, 3, checkpoint_every=2, constraint_iters=4, phase_res=(8, 8, 8), splat_res=SRES,
+                 splat_sigma=0.25, splat_cutoff=4.0, nslots=1500, n_init=1000, fill_seed=9)
+    r2.set_target(gaussian_target(BOX, SRES, 3, centre_lo=(0.2, 0.6, 0.6), centre_hi=(0.8, 1.4, 1.4)))
+    r.forward(v)
+    _, parts_old, _ = r.state()
+    hb = np.float32(0.1)
+    omega = np.float32(np.sum(v ** np.float32(2)) / (np.float32(2) * hb))
+    isf.jet_configure(v / hb, omega, np.array(CEN, np.float32) + np.array([0.0, 0.05, -0.05], np.float32), LEN, RAD,
+                      50, 7, 0)
+    l1 = r.forward(v)
+    _, p1, _ = r.state()
+    l2 = r2.forward(v)
+    _, p2, _ = r2.state()
+    assert not np.array_equal(p1, parts_old)
+    assert np.array_equal(p1, p2)
+    assert l1 == l2
+
+
 def test_rollout_gradients_match_autograd():
     isf, r, tr, v = rollout_case()
     rng = np.random.default_rng(1)
