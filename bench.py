@@ -102,7 +102,7 @@ def cpu_baseline(w, budget_s=12.0):
         run.step(1)
         n += 1
         el = time.perf_counter() - t0
-        if (el >= budget_s and n >= 2) or n >= 50:
+        if (el >= budget_s and n >= 2) or n >= 200:
             break
     return {"value": n / el, "unit": "steps/s", "cores": threads, "kind": "port",
             "sample": f"FP32 oracle (oracle/isf_oracle.c, OpenMP {threads} threads, radix-2 FFT) on the same "
