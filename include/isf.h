@@ -309,12 +309,15 @@ int isf_rollout_destroy(isf_rollout_t r);
 /* target density (device, splat_res cells, x fastest); copied */
 int isf_rollout_set_target(isf_rollout_t r, const float* target);
 /* velocity: host float[3]; theta: device (phase_res cells) or NULL (zero); psi_init:
- * device psi or NULL (the default).  Writes the loss (FP64) after synchronising. */
+ * device psi or NULL (the default).  Writes the loss (FP64) after synchronising.  The
+ * spawn batches of all steps are drawn once, at the first forward and again after the
+ * handle's nozzle or seed has changed (isf_jet_configure). */
 int isf_rollout_forward(isf_rollout_t r, const float velocity[3], const float* theta, const float* psi_init,
                         double* loss);
 /* Gradients of the last forward's loss; any output may be NULL.  grad_velocity: host
  * double[3]; grad_theta: device (phase_res cells); grad_psi_init: device psi layout;
- * grad_parts_init: device (nslots x 3).  Synchronous. */
+ * grad_parts_init: device (nslots x 3).  Synchronous.  Besides the handle's stream it uses
+ * two streams of the rollout (segment recompute, particle adjoint), ordered by events. */
 int isf_rollout_backward(isf_rollout_t r, double grad_velocity[3], float* grad_theta, float* grad_psi_init,
                          float* grad_parts_init);
 /* Device views of the last forward's final psi, particles and density (rho, FP64). */
