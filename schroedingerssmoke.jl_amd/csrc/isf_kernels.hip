@@ -1011,7 +1011,7 @@ __global__ void __launch_bounds__(256) k_reduce_sum(const double* __restrict__ p
 // which gathers share cache lines, never a result.
 __global__ void __launch_bounds__(256)
 k_advect(float* __restrict__ parts, long long nslots, const int* __restrict__ order, const float* __restrict__ vel,
-         GridInfo g, float dt, Batch B) {
+         GridInfo g, float dt, Batch B, const float* src) {
     if (B.inst) {
         const Inst& m = B.inst[blockIdx.y];
         parts = m.parts;
@@ -1020,11 +1020,12 @@ k_advect(float* __restrict__ parts, long long nslots, const int* __restrict__ or
         vel = m.vel;
         g = m.g;
         dt = m.dt;
+        src = nullptr;
     }
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nslots) return;
     const long long s = order ? (long long)order[k] : k;
-    const F3 p3 = ld3(parts + 3 * s);
+    const F3 p3 = ld3((src ? src : parts) + 3 * s);  // src: out of place, from a separate array
     float p[3] = {p3.x, p3.y, p3.z};
     const bool in = (p[0] > 0.0f) & (p[0] < g.Lx) & (p[1] > 0.0f) & (p[1] < g.Ly) & (p[2] > 0.0f) & (p[2] < g.Lz);
     if (!in) {
@@ -1463,11 +1464,12 @@ int launch_spawn(float* parts, long long nslots, long long first, long long n, c
 }
 // batch mode: max_slots = the largest nslots of the members (grid size)
 int launch_advect(const GridInfo& g, float* parts, long long nslots, const int* order, const float* vel, float dt,
-                  hipStream_t s, Batch b, long long max_slots) {
+                  hipStream_t s, Batch b, long long max_slots, const float* src) {
     const long long nn = b.inst ? max_slots : nslots;
     if (nn <= 0) return 0;
+    if (src == parts) src = nullptr;
     const dim3 grid((unsigned)((nn + 255) / 256), members_of(b));
-    ISF_LAUNCH(k_advect, grid, dim3(256), s, parts, nslots, order, vel, g, dt, b);
+    ISF_LAUNCH(k_advect, grid, dim3(256), s, parts, nslots, order, vel, g, dt, b, src);
 }
 
 __global__ void k_set_counter(int* c, int v) { c[0] = v; }

@@ -246,9 +246,11 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
     isf_t h = r->h;
     hipStream_t st = w.st;
     const size_t pb = sizeof(float) * 3 * (size_t)r->P;
-    if (r->P > 0)
-        HIP_CHECK(hipMemcpyAsync(w.parts, r->ck_parts[s], pb, hipMemcpyDeviceToDevice, st), "particles checkpoint");
     const int ks = seg_steps(r, s);
+    // particles: tape entry p0[k] holds step k's state after its spawn; the advection of
+    // step k writes p0[k + 1] (the segment's last step: w.parts), so no step copies them
+    if (r->P > 0)
+        HIP_CHECK(hipMemcpyAsync(tp.p0[0], r->ck_parts[s], pb, hipMemcpyDeviceToDevice, st), "particles checkpoint");
     for (int k = 0; k < ks; k++) {
         const int it = s * r->K + k + 1;
         const float4* in = psi0_of(r, tp, s, k);
@@ -256,11 +258,9 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
         long long first0, nsp;
         spawn_slots(r, it, &first0, &nsp);
         if (nsp > 0) {
-            int e2 = spawn_step(r, it, first0, nsp, w.parts, st);
+            int e2 = spawn_step(r, it, first0, nsp, tp.p0[k], st);
             if (e2) return e2;
         }
-        if (r->P > 0)
-            HIP_CHECK(hipMemcpyAsync(tp.p0[k], w.parts, pb, hipMemcpyDeviceToDevice, st), "particles tape");
         LAUNCH_CHECK(launch_flow_x_fwd(r->g, in, w.S, h->tb, st), "flow x forward");
         LAUNCH_CHECK(launch_flow_plane(r->g, w.S, h->tb, st), "flow plane");
         // psi1 for the tape, normalize_psi of it into psi3 (projected in place next)
@@ -271,7 +271,10 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
         if (e) return e;
         if ((e = project_q(r, out, tp.q2[k], w))) return e;
         LAUNCH_CHECK(launch_velocity(r->g, out, h->hbar, 1, w.vel, st), "velocity_one_form");
-        if (r->P > 0) LAUNCH_CHECK(launch_advect(r->g, w.parts, r->P, nullptr, w.vel, h->dt, st), "staggered_advect");
+        if (r->P > 0)
+            LAUNCH_CHECK(launch_advect(r->g, k + 1 < ks ? tp.p0[k + 1] : w.parts, r->P, nullptr, w.vel, h->dt, st,
+                                       Batch{}, 0, tp.p0[k]),
+                         "staggered_advect");
     }
     if (checkpoint && s + 1 < r->nseg && r->P > 0)
         HIP_CHECK(hipMemcpyAsync(r->ck_parts[s + 1], w.parts, pb, hipMemcpyDeviceToDevice, st), "particles checkpoint");
