@@ -96,6 +96,7 @@ struct isf_rollout {
     float omega;
     JetParams jet;  // kvec from v, omgterm set per use
     bool forward_done;
+    bool last_tape;  // tape set (nseg - 1) % 2 still holds the forward's last segment
 };
 
 static int check_rollout(isf_rollout_t r) {
@@ -545,7 +546,7 @@ int isf_rollout_forward(isf_rollout_t r, const float velocity[3], const float* t
     for (int j = 0; j < r->C; j++)
         if ((e = project_q(r, r->cpsi[j + 1], r->cq[j], mw, j + 1 < r->C ? r->cpsi[j + 2] : nullptr, &j0))) return e;
     for (int s = 0; s < r->nseg; s++)
-        if ((e = seg_forward(r, s, r->tape[0], mw, true))) return e;
+        if ((e = seg_forward(r, s, r->tape[s % 2], mw, true))) return e;
     if (r->P > 0)
         HIP_CHECK(hipMemcpyAsync(r->parts_final, r->parts, sizeof(float) * 3 * r->P, hipMemcpyDeviceToDevice, st),
                   "copy");
@@ -557,6 +558,7 @@ int isf_rollout_forward(isf_rollout_t r, const float velocity[3], const float* t
     HIP_CHECK(hipStreamSynchronize(st), "hipStreamSynchronize");
     *loss = sum / (double)r->M;  // mean over the splat cells
     r->forward_done = true;
+    r->last_tape = true;  // segment s went to tape set s % 2: the backward's first set
     return ISF_OK;
 }
 
@@ -593,7 +595,14 @@ int isf_rollout_backward(isf_rollout_t r, double grad_velocity[3], float* grad_t
         HIP_CHECK(hipEventRecord(r->rec_done[b], r->rw.st), "hipEventRecord");
         return ISF_OK;
     };
-    if ((e = recompute(r->nseg - 1))) return e;
+    // the last segment's tape is still there after a forward (not after a backward, whose
+    // recomputes reuse the set)
+    if (r->last_tape) {
+        HIP_CHECK(hipEventRecord(r->rec_done[(r->nseg - 1) % 2], st), "hipEventRecord");
+        r->last_tape = false;
+    } else if ((e = recompute(r->nseg - 1))) {
+        return e;
+    }
     for (int s = r->nseg - 1; s >= 0; s--) {
         const int b = s % 2;
         HIP_CHECK(hipStreamWaitEvent(r->pst, r->rec_done[b], 0), "hipStreamWaitEvent");
