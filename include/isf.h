@@ -152,8 +152,10 @@ int isf_particles_remove_dead(isf_t h, isf_particles_t p);
 /* StaggeredAdvect(p, isf, vx, vy, vz, dt) (src/MatlabPort.jl:201-222) over the active
  * slots; vel NULL = the handle's velocity field */
 int isf_particles_advect(isf_t h, isf_particles_t p, const float* vel);
-/* Keep(p, ind) (src/MatlabPort.jl:226-230): positions become x[ind] (device, 0-based, n
- * entries) in slots 0..n-1; length = n and all n are active */
+/* Keep(p, ind) (src/MatlabPort.jl:226-230): x, y, z become x[ind], y[ind], z[ind] (ind:
+ * device, 0-based, n entries): slots 0..n-1 hold the gathered positions, the slots beyond
+ * the end of the reference's n-long arrays hold -1, and append! then checks against n.
+ * length and the active list are unchanged, as in the reference. */
 int isf_particles_keep(isf_t h, isf_particles_t p, const int* ind, long long n);
 /* length, active count (synchronises), device views of x, y, z and the active list */
 int isf_particles_state(isf_t h, isf_particles_t p, long long* length, long long* active_count, const float** x,

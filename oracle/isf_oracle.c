@@ -12,9 +12,20 @@
  */
 #define _GNU_SOURCE
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 #include "isf_philox.h"
+
+/* Optional replacement of the 3-D FFT (isfo_fft3_*): the timed CPU baseline installs a
+ * library FFT here (MKL through torch.fft); z, n and inverse as for isfo_fft3, dbl = 1
+ * for the Float64 instance.  NULL: the radix-2 restatement below. */
+typedef void (*isfo_fft_hook_t)(void* z, const int* n, int inverse, int dbl);
+static isfo_fft_hook_t isfo_fft_hook = NULL;
+void isfo_set_fft_hook(isfo_fft_hook_t f) { isfo_fft_hook = f; }
+/* OpenMP threads of the loops (and of the radix-2 FFT) */
+void isfo_set_threads(int t) { omp_set_num_threads(t > 0 ? t : 1); }
+int isfo_get_threads(void) { return omp_get_max_threads(); }
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846

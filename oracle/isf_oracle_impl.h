@@ -67,6 +67,10 @@ static void FN(fft1d)(R* re, R* im, int n, int sign, const R* twr, const R* twi)
  * inverse != 0 applies the backward transform scaled by 1/N, the AbstractFFTs
  * convention of plan_ifft! (src/parallel.jl:79,172,184). */
 void FN(isfo_fft3)(R* z, const int* n, int inverse) {
+    if (isfo_fft_hook) {
+        isfo_fft_hook(z, n, inverse, sizeof(R) == sizeof(double));
+        return;
+    }
     const int nx = n[0], ny = n[1], nz = n[2];
     const long N = (long)nx * ny * nz;
     const int sign = inverse ? 1 : -1;

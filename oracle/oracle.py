@@ -261,3 +261,49 @@ class OracleRun:
     @property
     def velocity(self):
         return self._vel.reshape(self.n[2], self.n[1], self.n[0], 3)
+
+
+# ---------------------------------------------------------------- library FFT for the CPU baseline
+_FFT_HOOK_T = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int)
+
+
+def _torch_fft3(ptr, n, inverse, dbl):
+    """The 3-D transform of isfo_fft3 by MKL through torch.fft on the CPU: in place on the
+    interleaved array at ptr (x fastest), forward unnormalised, inverse scaled by 1/N
+    (plan_fft! / plan_ifft!, src/parallel.jl:79)."""
+    import torch
+    nx, ny, nz = n[0], n[1], n[2]
+    ct = C.c_double if dbl else C.c_float
+    a = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(nz, ny, nx, 2))
+    z = torch.view_as_complex(torch.from_numpy(a))
+    z.copy_(torch.fft.ifftn(z, dim=(0, 1, 2)) if inverse else torch.fft.fftn(z, dim=(0, 1, 2)))
+
+
+_fft_hook_ref = _FFT_HOOK_T(_torch_fft3)
+
+
+class cpu_engine:
+    """Context: the oracle's FFTs through MKL (torch.fft, fft="mkl") or the radix-2
+    restatement (fft="radix2"), with `threads` OpenMP threads for the per-statement loops
+    and the same number of torch intra-op threads."""
+
+    def __init__(self, fft="mkl", threads=None):
+        self.fft = fft
+        self.threads = int(threads) if threads else os.cpu_count() or 1
+
+    def __enter__(self):
+        import torch
+        L = lib()
+        self._prev = (L.isfo_get_threads(), torch.get_num_threads())
+        L.isfo_set_threads(C.c_int(self.threads))
+        torch.set_num_threads(self.threads)
+        L.isfo_set_fft_hook(_fft_hook_ref if self.fft == "mkl" else _FFT_HOOK_T())
+        return self
+
+    def __exit__(self, *exc):
+        import torch
+        L = lib()
+        L.isfo_set_fft_hook(_FFT_HOOK_T())
+        L.isfo_set_threads(C.c_int(self._prev[0]))
+        torch.set_num_threads(self._prev[1])
+        return False

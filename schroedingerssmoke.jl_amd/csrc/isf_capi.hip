@@ -250,11 +250,15 @@ int isf_create(isf_t* out, const int res[3], const int box[3], float hbar, float
         const size_t most = (size_t)(g.ny / (g.ny < 8 ? g.ny : 8)) * g.nz;
         h->norm_nparts = most > 1024 ? most : 1024;
     }
-    if ((e = hipMalloc(&h->norm_parts, sizeof(double) * (h->norm_nparts + 1))) != hipSuccess)
+    // [step partials | step sum | isf_norm2 partials | isf_norm2 sum]
+    const size_t norm_doubles = h->norm_nparts + 1 + kNorm2Blocks + 1;
+    if ((e = hipMalloc(&h->norm_parts, sizeof(double) * norm_doubles)) != hipSuccess)
         return cleanup(fail(ISF_ENOMEM, "norm partials: %s", hipGetErrorString(e)));
     h->norm_out = h->norm_parts + h->norm_nparts;
+    h->norm2_parts = h->norm_out + 1;
+    h->norm2_out = h->norm2_parts + kNorm2Blocks;
     if ((e = hipMemset(h->vel, 0, 12 * h->N)) != hipSuccess) return cleanup(hip_fail(e, "hipMemset"));
-    if ((e = hipMemset(h->norm_parts, 0, sizeof(double) * (h->norm_nparts + 1))) != hipSuccess)
+    if ((e = hipMemset(h->norm_parts, 0, sizeof(double) * norm_doubles)) != hipSuccess)
         return cleanup(hip_fail(e, "hipMemset"));
     // a blocking stream: it also orders against the null stream used by plain hipMemcpy
     if ((e = hipStreamCreate(&h->stream)) != hipSuccess)
@@ -601,6 +605,7 @@ static int step_impl(isf_t h, float* psi, float* parts, long long nslots, int it
     STAGE(launch_div_r2c(h->g, p, f_of(h), P_of(h), h->tb, s), "div(one_form(psi)) + poisson x forward");
     STAGE(launch_poisson_plane(h->g, P_of(h), sc, h->tb, s), "poisson plane");
     STAGE(launch_c2r_x(h->g, P_of(h), 1, nullptr, p, nullptr, h->norm_parts, h->tb, s), "poisson x inverse + gauge");
+    h->norm_count = c2r_x_blocks(h->g);
     // :59-62 velocity_one_form!(isf, psi, hbar) and the in-place StaggeredSharp, after the
     // previous step's advect is done reading vel (same side stream, or joined above)
     if (side) {
@@ -737,9 +742,9 @@ int isf_norm2(isf_t h, const float* psi, double* out) {
     CHECK_PTR(psi, "psi");
     CHECK_PTR(out, "out");
     size_t np = 0;
-    LAUNCH_CHECK(launch_norm2(h->g, reinterpret_cast<const float4*>(psi), h->norm_parts, &np, h->stream), "norm2");
-    LAUNCH_CHECK(launch_reduce_sum(h->norm_parts, np, h->norm_out, h->stream), "norm2 reduce");
-    HIP_CHECK(hipMemcpyAsync(out, h->norm_out, sizeof(double), hipMemcpyDeviceToHost, h->stream), "norm2 copy");
+    LAUNCH_CHECK(launch_norm2(h->g, reinterpret_cast<const float4*>(psi), h->norm2_parts, &np, h->stream), "norm2");
+    LAUNCH_CHECK(launch_reduce_sum(h->norm2_parts, np, h->norm2_out, h->stream), "norm2 reduce");
+    HIP_CHECK(hipMemcpyAsync(out, h->norm2_out, sizeof(double), hipMemcpyDeviceToHost, h->stream), "norm2 copy");
     HIP_CHECK(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
     return ISF_OK;
 }
@@ -747,12 +752,16 @@ int isf_norm2(isf_t h, const float* psi, double* out) {
 int isf_step_norm2(isf_t h, double* out) {
     CHECK_HJ(h);
     CHECK_PTR(out, "out");
-    LAUNCH_CHECK(launch_reduce_sum(h->norm_parts, c2r_x_blocks(h->g), h->norm_out, h->stream), "norm2 reduce");
+    if (h->norm_count == 0) return fail(ISF_EINVAL, "no step has run on this handle");
+    LAUNCH_CHECK(launch_reduce_sum(h->norm_parts, h->norm_count, h->norm_out, h->stream), "norm2 reduce");
     HIP_CHECK(hipMemcpyAsync(out, h->norm_out, sizeof(double), hipMemcpyDeviceToHost, h->stream), "norm2 copy");
     HIP_CHECK(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
     return ISF_OK;
 }
 
+// Both timer calls first join the side stream (CHECK_HJ: the handle's stream waits for
+// the event recorded after the last isf_step's velocity pass and advect), so the stop
+// event completes only after the side-stream work of the timed steps.
 int isf_timer_start(isf_t h) {
     CHECK_HJ(h);
     HIP_CHECK(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
@@ -1076,6 +1085,8 @@ int isf_batch_step(isf_batch_t b, int iter) {
         h->batch_ev = b->done;
         h->ext_pending = true;
     }
+    for (const BatchGroup& gr : b->groups)
+        for (int i : gr.idx) b->mem[i]->norm_count = c2r_x_blocks(gr.g);
     return ISF_OK;
 }
 

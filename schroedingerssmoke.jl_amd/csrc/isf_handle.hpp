@@ -58,9 +58,12 @@ struct isf_handle {
     DevTables tb;
     float2* S;          // Schroedinger spectrum, 2N complex; f and P alias it in projections
     float* vel;         // velocity field, 3N floats
-    double* norm_parts; // per-block FP64 partial sums
-    size_t norm_nparts;
+    double* norm_parts; // per-block FP64 partial sums of the last step's gauge pass
+    size_t norm_nparts; // room in norm_parts
+    size_t norm_count;  // partials the last step's gauge pass wrote (0: no step yet)
     double* norm_out;
+    double* norm2_parts;  // partials of isf_norm2 (kNorm2Blocks), kept apart from the step's
+    double* norm2_out;
     // jet (isf_jet_configure)
     bool jet_set;
     JetParams jet;

@@ -111,6 +111,8 @@ int launch_sharp(const GridInfo& g, float* vel, hipStream_t s);
 int launch_gauge(const GridInfo& g, float4* psi, const float* q, hipStream_t s);
 int launch_isjet(const GridInfo& g, const JetParams& jet, unsigned char* out, hipStream_t s);
 int launch_restrict(const GridInfo& g, float4* psi, const unsigned char* isjet, const JetParams& jet, hipStream_t s);
+// sum of |psi1|^2 + |psi2|^2 as kNorm2Blocks FP64 partials (*nparts = kNorm2Blocks)
+constexpr size_t kNorm2Blocks = 1024;
 int launch_norm2(const GridInfo& g, const float4* psi, double* partials, size_t* nparts, hipStream_t s);
 int launch_reduce_sum(const double* partials, size_t n, double* out, hipStream_t s);
 

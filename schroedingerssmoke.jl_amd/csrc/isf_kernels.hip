@@ -1439,7 +1439,7 @@ int launch_restrict(const GridInfo& g, float4* psi, const unsigned char* isjet, 
     ISF_LAUNCH(k_restrict, grid1d(ncells(g), 256), dim3(256), s, psi, isjet, g, jet);
 }
 int launch_norm2(const GridInfo& g, const float4* psi, double* partials, size_t* nparts, hipStream_t s) {
-    const size_t nb = 1024;
+    const size_t nb = kNorm2Blocks;
     *nparts = nb;
     ISF_LAUNCH(k_norm2<256>, dim3((unsigned)nb), dim3(256), s, psi, ncells(g), partials);
 }

@@ -159,24 +159,25 @@ k_advect_active(float* __restrict__ x, float* __restrict__ y, float* __restrict_
     z[a] = p[2] + dt / 6.0f * (((k1[2] + 2.0f * k2[2]) + 2.0f * k3[2]) + k4[2]);
 }
 
-// ---- Keep(particle, ind): x = x[ind] ... (src/MatlabPort.jl:226-230), then slots 0..n-1
-// are the particles and all of them are active
+// ---- Keep(particle, ind): x = x[ind], y = y[ind], z = z[ind] (src/MatlabPort.jl:226-230).
+// The position arrays become n long: slots 0..n-1 receive the gathered positions and the
+// slots beyond (which the reference's shorter arrays no longer have) the dead marker -1.
+// length and the active list are left as they are, as in the reference.
 __global__ void __launch_bounds__(NT)
 k_gather_soa(float* __restrict__ nx_, float* __restrict__ ny_, float* __restrict__ nz_, const float* __restrict__ x,
              const float* __restrict__ y, const float* __restrict__ z, const int* __restrict__ ind, int n, int cap,
-             int* __restrict__ active, int* __restrict__ bad) {
+             int* __restrict__ bad) {
     const int k = blockIdx.x * NT + threadIdx.x;
-    if (k >= n) return;
-    const int s = ind[k];
-    if (s < 0 || s >= cap) {  // the reference raises a BoundsError; reported to the caller
-        atomicOr(bad, 1);
+    if (k >= cap) return;
+    const int s = k < n ? ind[k] : -1;
+    if (k < n && (s < 0 || s >= cap)) atomicOr(bad, 1);  // the reference raises a BoundsError
+    if (s < 0 || s >= cap) {
         nx_[k] = ny_[k] = nz_[k] = -1.0f;
     } else {
         nx_[k] = x[s];
         ny_[k] = y[s];
         nz_[k] = z[s];
     }
-    active[k] = k;
 }
 __global__ void k_set_int(int* p, int v) { p[0] = v; }
 
@@ -186,6 +187,7 @@ struct isf_particles {
     uint32_t magic;
     int capacity;
     int length;          // slots ever filled (host copy, append! only)
+    int size;            // length(p.x): capacity, or n after Keep(p, ind) with n indices
     float *x, *y, *z;    // capacity each
     float *x2, *y2, *z2; // gather targets of Keep
     int *active, *active2;
@@ -208,6 +210,7 @@ int isf_particles_create(isf_particles_t* out, long long capacity) {
     isf_particles* p = new isf_particles();
     p->magic = kPartMagic;
     p->capacity = (int)capacity;
+    p->size = (int)capacity;
     const size_t fb = sizeof(float) * (size_t)capacity, ib = sizeof(int) * (size_t)capacity;
     hipError_t e = hipSuccess;
     float** fbufs[6] = {&p->x, &p->y, &p->z, &p->x2, &p->y2, &p->z2};
@@ -253,8 +256,8 @@ static int stream_of(isf_t h, isf_particles_t p, hipStream_t* s) {
 // append!: the reference asserts length(x) + p.length < length(p.x) (src/MatlabPort.jl:187)
 static int room(isf_particles_t p, long long n) {
     if (n < 0) return fail(ISF_EINVAL, "negative count");
-    if ((long long)p->length + n >= p->capacity)
-        return fail(ISF_EINVAL, "append!: %lld + %d particles need more than %d slots", n, p->length, p->capacity);
+    if ((long long)p->length + n >= p->size)
+        return fail(ISF_EINVAL, "append!: %lld + %d particles need more than %d slots", n, p->length, p->size);
     return ISF_OK;
 }
 static int append_active(isf_particles_t p, int first, int n, hipStream_t s) {
@@ -344,10 +347,8 @@ int isf_particles_keep(isf_t h, isf_particles_t p, const int* ind, long long n) 
     if (n < 0 || n > p->capacity) return fail(ISF_EINVAL, "Keep: bad count");
     if (n > 0 && !ind) return fail(ISF_EINVAL, "ind must not be NULL");
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, p->count + 1, 0);
-    if (n > 0)
-        hipLaunchKernelGGL(k_gather_soa, dim3(nblocks((int)n)), dim3(NT), 0, s, p->x2, p->y2, p->z2, p->x, p->y, p->z,
-                           ind, (int)n, p->capacity, p->active2, p->count + 1);
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, p->count, (int)n);
+    hipLaunchKernelGGL(k_gather_soa, dim3(nblocks(p->capacity)), dim3(NT), 0, s, p->x2, p->y2, p->z2, p->x, p->y, p->z,
+                       ind, (int)n, p->capacity, p->count + 1);
     HIP_CHECK(hipGetLastError(), "Keep");
     int bad = 0;
     HIP_CHECK(hipMemcpyAsync(&bad, p->count + 1, sizeof(int), hipMemcpyDeviceToHost, s), "Keep");
@@ -355,8 +356,7 @@ int isf_particles_keep(isf_t h, isf_particles_t p, const int* ind, long long n) 
     std::swap(p->x, p->x2);
     std::swap(p->y, p->y2);
     std::swap(p->z, p->z2);
-    std::swap(p->active, p->active2);
-    p->length = (int)n;
+    p->size = (int)n;
     if (bad) return fail(ISF_EINVAL, "Keep: an index is outside 0..%d (those slots now hold -1)", p->capacity - 1);
     return ISF_OK;
 }

@@ -9,6 +9,7 @@ fills its particle buffer inside the nozzle cylinder.
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 
@@ -16,6 +17,12 @@ from .api import Batch, DeviceArray
 
 # grid shapes of the uneven ensemble, each with 16 cells per unit length
 ENSEMBLE_GRIDS = ((32, 32, 32), (64, 32, 32), (64, 64, 64), (128, 64, 64))
+
+# Schedule cache (isf_batch_tune format: nx ny nz members x_tile plane_elements) written by
+# the autotuner on an MI355X for the default 48-jet draw (draw_jets()); Batch.tune(path)
+# reads it instead of timing the candidates again.
+ENSEMBLE48_SCHEDULES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "schedules",
+                                    "mi355x_ensemble48.txt")
 
 
 def draw_jets(count=48, seed=20261203, grids=ENSEMBLE_GRIDS, n_slots=10_000, n_spawn=100, dt=1.0 / 48.0,

@@ -193,3 +193,78 @@ def test_autotuned_tiles_give_identical_results(ens, tmp_path):
     assert_close(results[8], results[16], "tile 8 vs 16")
     assert_close(results[8], results[32], "tile 8 vs 32")
     assert os.path.getsize(cache) > 0
+
+
+def test_step_norm2_after_tile_change_and_norm2(ens, tmp_path):
+    """isf_step_norm2 reduces exactly the partials the last gauge pass wrote: a step with
+    the default x tile (16 lines), then the tuner switches the members to 8 or 32 lines
+    (twice or half the blocks), then an isf_norm2 call with its own partition; the step's
+    sum still equals the norm of the stepped psi."""
+    ms = members(ens, 2, seed=31, grids=((64, 32, 32),))
+    for forced in (8, 32):
+        path = str(tmp_path / f"t{forced}.txt")
+        with open(path, "w") as f:
+            f.write(f"64 32 32 2 {forced} 8\n")
+        b = ens.build_batch(ms, n_init=300)
+        b.step_sequential_(1)
+        b.tune(path)
+        assert b.tile((64, 32, 32)) == forced
+        for i in range(2):
+            m = b.members[i]
+            s1 = m.step_norm2()
+            direct = m.norm2(b._bound[i][0])
+            s2 = m.step_norm2()
+            assert s1 == s2, (forced, i, s1, s2)
+            assert abs(s1 - direct) <= 1e-9 * direct, (forced, i, s1, direct)
+        b.close()
+
+
+@pytest.mark.timeout(600)
+def test_c3_ensemble_of_48_jets(ens, tmp_path):
+    """The batched uneven ensemble at its full size: 48 jets drawn by draw_jets (grids
+    32^3, 64x32x32, 64^3, 128x64x64; hbar in [0.05, 0.2]; |v| in [0.5, 1.5] within 20
+    degrees of +x; 10k slots filled in the nozzle, 100 spawned per step) with the
+    autotuner's cached MI355X schedules.  Batched (one graph replay per step) equals the
+    members stepped one after another with isf_step bit for bit, and the first member of
+    each grid shape is within 1e-4 of the FP64 oracle after 6 steps
+    (the reference's per-instance loop, test/runtests.jl:45-64)."""
+    import shutil
+    ms = ens.draw_jets()
+    sched = str(tmp_path / "sched.txt")
+    shutil.copy(ens.ENSEMBLE48_SCHEDULES, sched)
+    a = ens.build_batch(ms)
+    s = ens.build_batch(ms)
+    a.tune(sched)
+    s.tune(sched)
+    for res, ty in (((32, 32, 32), 8), ((64, 32, 32), 16), ((64, 64, 64), 32), ((128, 64, 64), 16)):
+        assert a.tile(res) == ty and s.tile(res) == ty
+    steps = 6
+    for it in range(1, steps + 1):
+        a.step_(it)
+        s.step_sequential_(it)
+    sa, ss = state(a), state(s)
+    assert_same(sa, ss, "48-jet ensemble, 6 steps")
+    for i in range(len(ms)):
+        n = int(np.prod(ms[i]["res"]))
+        assert abs(a.members[i].step_norm2() - n) < 1e-5 * n
+    a.close()
+    s.close()
+    seen = set()
+    for i, m in enumerate(ms):
+        if m["res"] in seen:
+            continue
+        seen.add(m["res"])
+        cfg = JetConfig(vol_size=m["box"], dims=m["res"], hbar=float(m["hbar"]), dt=float(m["dt"]),
+                        jet_velocity=tuple(float(x) for x in m["velocity"]),
+                        nozzle_cen=tuple(float(x) for x in m["cen"]), nozzle_len=float(m["len"]),
+                        nozzle_rad=float(m["rad"]), n_spawn=m["n_spawn"], n_slots=m["n_slots"],
+                        n_init=m["n_slots"], constraint_iters=10, seed=m["seed"], phase_mode=m["phase_mode"])
+        o = oracle_run(cfg, np.float64)
+        o.step(steps)
+        psi, parts = sa[i]
+        err = rel_l2(psi, o.psi)
+        rep = particle_report(parts, o.parts, cfg.vol_size)
+        print(f"  member {i} {m['res']} hbar {m['hbar']:.3f}: psi rel L2 vs FP64 {err:.3e}; particles {rep}",
+              flush=True)
+        assert err < 1e-4
+        assert rep["status_mismatch"] <= 10 and rep["p99"] < 1e-4

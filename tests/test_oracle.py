@@ -385,3 +385,30 @@ def test_step_matches_matlabport_sequence():
     px, py, pz = mp.staggered_advect(spawned[:, 0], spawned[:, 1], spawned[:, 2], obj, vx, vy, vz, run.dt)
     assert np.max(np.abs(run.parts[:20] - np.stack([px, py, pz], -1))) < 1e-12
     assert np.all(run.parts[20:] == -1.0)
+
+
+def test_library_fft_engine_matches_the_radix2_restatement():
+    """The CPU baseline's FFT engine (MKL through torch.fft, installed as the oracle's 3-D
+    transform) against the oracle's own radix-2 transform and numpy, both precisions, and
+    one full step of the jet in FP64 with each engine."""
+    from oracle.oracle import cpu_engine
+    rng = np.random.default_rng(21)
+    z = rng.standard_normal((8, 16, 32, 2))
+    for dt, tol in ((np.float64, 1e-13), (np.float32, 1e-6)):
+        o = Oracle(dt)
+        zz = z.astype(dt)
+        for inv in (False, True):
+            a = o.fft3(zz.copy(), inverse=inv)
+            with cpu_engine("mkl", 2):
+                b = o.fft3(zz.copy(), inverse=inv)
+            c = zz[..., 0] + 1j * zz[..., 1]
+            ref = np.fft.ifftn(c) if inv else np.fft.fftn(c)
+            assert rel(a, b) < tol, (dt, inv)
+            assert rel(b[..., 0] + 1j * b[..., 1], ref) < tol, (dt, inv)
+    cfg = JetConfig(dims=(32, 16, 16), vol_size=(2, 1, 1), nozzle_cen=(0.3, 0.466, 0.566), n_spawn=20,
+                    n_slots=200, constraint_iters=2)
+    r1 = OracleRun(cfg, np.float64).step(2)
+    with cpu_engine("mkl", 2):
+        r2 = OracleRun(cfg, np.float64).step(2)
+    assert rel(r1.psi, r2.psi) < 1e-12
+    assert np.max(np.abs(r1.parts - r2.parts)) < 1e-12
