@@ -131,6 +131,14 @@ int isf_set_sort_interval(isf_t h, int steps);
 int isf_norm2(isf_t h, const float* psi, double* out);
 /* The same sum accumulated by the last isf_step (read after isf_synchronize). */
 int isf_step_norm2(isf_t h, double* out);
+/* The sum of isf_norm2 without allocating: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops).
+ * psi: ncells cells (1 .. ISF_NOALLOC_MAX_CELLS) of the interleaved layout; out: one
+ * double in device memory; stream: a hipStream_t (NULL: the null stream).  Asynchronous,
+ * needs no handle.  Its partial sums use one fixed 8 KiB device array per GPU, shared by
+ * every call on that GPU: calls that overlap on one GPU race. */
+#define ISF_NOALLOC_MAX_CELLS (1LL << 30)
+int isf_norm2_noalloc(const float* psi, long long ncells, double* out, void* stream);
 
 /* ---- MatlabPort particle store (src/MatlabPort.jl:178-230) ----------------------------- */
 /* Particles{x, y, z, length, active}: Float32 structure-of-arrays positions with

@@ -69,6 +69,7 @@ SIGNATURES = {
     "isf_set_sort_interval": ([C.c_void_p, C.c_int], C.c_int),
     "isf_norm2": ([C.c_void_p, C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "isf_step_norm2": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "isf_norm2_noalloc": ([C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p], C.c_int),
     "isf_step_stage_count": ([], C.c_int),
     "isf_step_stage_name": ([C.c_int], C.c_char_p),
     "isf_step_profile": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.POINTER(C.c_float)], C.c_int),
