@@ -165,6 +165,11 @@ int isf_particles_advect(isf_t h, isf_particles_t p, const float* vel);
  * the end of the reference's n-long arrays hold -1, and append! then checks against n.
  * length and the active list are unchanged, as in the reference. */
 int isf_particles_keep(isf_t h, isf_particles_t p, const int* ind, long long n);
+/* filter!(p.active) of the driver's removethedead! (examples/matlabport/simple.jl:63-70),
+ * or any other host-side edit of the active list: the list becomes the n slot indices in
+ * `active` (host memory, 0-based, each below the length of the position arrays);
+ * synchronous. */
+int isf_particles_set_active(isf_t h, isf_particles_t p, const int* active, long long n);
 /* length, active count (synchronises), device views of x, y, z and the active list */
 int isf_particles_state(isf_t h, isf_particles_t p, long long* length, long long* active_count, const float** x,
                         const float** y, const float** z, const int** active);

@@ -361,6 +361,22 @@ int isf_particles_keep(isf_t h, isf_particles_t p, const int* ind, long long n) 
     return ISF_OK;
 }
 
+int isf_particles_set_active(isf_t h, isf_particles_t p, const int* active, long long n) {
+    hipStream_t s;
+    int r = stream_of(h, p, &s);
+    if (r) return r;
+    if (n < 0 || n > p->capacity) return fail(ISF_EINVAL, "set_active: %lld entries for %d slots", n, p->capacity);
+    if (n > 0 && !active) return fail(ISF_EINVAL, "active must not be NULL");
+    for (long long k = 0; k < n; k++)
+        if (active[k] < 0 || active[k] >= p->size)
+            return fail(ISF_EINVAL, "set_active: entry %lld = %d outside 0..%d", k, active[k], p->size - 1);
+    if (n > 0) HIP_CHECK(hipMemcpyAsync(p->active, active, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s), "set_active");
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, p->count, (int)n);
+    HIP_CHECK(hipGetLastError(), "set_active");
+    HIP_CHECK(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return ISF_OK;
+}
+
 int isf_particles_state(isf_t h, isf_particles_t p, long long* length, long long* active_count, const float** x,
                         const float** y, const float** z, const int** active) {
     hipStream_t s;

@@ -80,6 +80,7 @@ SIGNATURES = {
     "isf_particles_remove_dead": ([C.c_void_p, C.c_void_p], C.c_int),
     "isf_particles_advect": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "isf_particles_keep": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong], C.c_int),
+    "isf_particles_set_active": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong], C.c_int),
     "isf_particles_state": ([C.c_void_p, C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                              C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                              C.POINTER(C.c_void_p)], C.c_int),

@@ -43,6 +43,12 @@ class Particles:
         d = DeviceArray.from_host(np.asarray(ind, np.int32))
         check(self.lib.isf_particles_keep(self.isf.h, self.h, d.ptr, d.shape[0]), "isf_particles_keep")
 
+    def set_active_(self, active):
+        """Replace the active list (0-based slot indices), e.g. after a host-side filter!."""
+        a = np.ascontiguousarray(np.asarray(active, np.int32))
+        check(self.lib.isf_particles_set_active(self.isf.h, self.h, a.ctypes.data if a.size else None, a.size),
+              "isf_particles_set_active")
+
     def state(self):
         """(length, active indices, x, y, z) as host arrays (x, y, z over all capacity slots)."""
         ln, cnt = C.c_longlong(), C.c_longlong()

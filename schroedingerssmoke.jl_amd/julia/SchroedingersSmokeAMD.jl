@@ -2,9 +2,10 @@
 # implementation of the SchroedingersSmoke.jl ISF step.
 #
 # It keeps the names and call forms of the reference's exports (src/parallel.jl:264-266,
-# src/particles.jl:97) so the driver loop of test/runtests.jl:40-66 runs unchanged apart
-# from `ArrayType = ISFArray`.  Every call goes through `ccall` into libisf; device arrays
-# are `ISFArray`s (device pointer + shape, freed by a finalizer).
+# src/particles.jl:97) so the driver of test/runtests.jl (and examples/benchmark.jl) runs
+# unchanged apart from `ArrayType = ISFArray`, loaded under the reference's package name
+# through SchroedingersSmoke.jl beside this file.  Every call goes through `ccall` into
+# libisf; device arrays are `ISFArray`s (device pointer + shape, freed by a finalizer).
 #
 # Julia is not installed in the build container, so this file is not exercised by the
 # test suite; the same C ABI is exercised through the Python mirror (isf_amd).
@@ -12,9 +13,14 @@ module SchroedingersSmokeAMD
 
 export ISF, ISFArray, normalize_psi!, pressure_project!, velocity_one_form!, schroedinger_flow!
 export staggered_advect!, add_particles!, restrict_velocity!, isjet, synchronize
-export ISFBatch, ISFRollout, rollout_loss, rollout_gradient, normalize_psi
+export ISFBatch, ISFRollout, rollout_loss, rollout_gradient, normalize_psi, set_spawn_seed!
+export Complex64, GPUArrays
 
 const libisf = joinpath(@__DIR__, "..", "lib", "libisf.so")
+
+# The Julia 0.6 name the reference drivers use for the element type of psi
+# (test/runtests.jl:19, examples/benchmark.jl:20).
+const Complex64 = ComplexF32
 
 function check(rc::Cint, what::AbstractString)
     if rc != 0
@@ -56,6 +62,39 @@ function Base.Array(a::ISFArray{T,N}) where {T,N}
 end
 devptr(a::ISFArray) = a.ptr
 devptr(::Nothing) = C_NULL
+
+# Broadcasts over ISFArrays that libisf has no kernel for (everything but the reference's
+# normalize_psi, isjet and StaggeredSharp forms below) run on the host: the device
+# operands are copied there, the broadcast is evaluated by Base, and the result comes
+# back as an ISFArray.  Logical-mask indexing works the same way, so host-style lines
+# such as `psi1[isJet] = amp1[isJet] .* exp.(1.0im*phase[isJet])`
+# (examples/matlabport/simple.jl:107-110) also accept device arrays.
+struct ISFStyle <: Base.Broadcast.AbstractArrayStyle{Any} end
+ISFStyle(::Val) = ISFStyle()
+Base.BroadcastStyle(::Type{<:ISFArray}) = ISFStyle()
+Base.BroadcastStyle(::ISFStyle, ::Base.Broadcast.DefaultArrayStyle) = ISFStyle()
+Base.BroadcastStyle(::ISFStyle, ::Base.Broadcast.Style{Tuple}) = ISFStyle()
+hostify(x::ISFArray) = Array(x)
+hostify(bc::Base.Broadcast.Broadcasted) = Base.Broadcast.broadcasted(bc.f, map(hostify, bc.args)...)
+hostify(x) = x
+Base.Broadcast.materialize(bc::Base.Broadcast.Broadcasted{ISFStyle}) =
+    ISFArray(Base.Broadcast.materialize(hostify(bc)))
+function Base.Broadcast.materialize!(dest::ISFArray, bc::Base.Broadcast.Broadcasted)
+    h = Array(dest)
+    Base.Broadcast.materialize!(h, hostify(bc))
+    upload!(dest, h)
+end
+function upload!(dest::ISFArray{T}, h::Array{T}) where {T}
+    check(ccall((:isf_memcpy_h2d, libisf), Cint, (Ptr{Cvoid}, Ptr{T}, Csize_t), dest.ptr, h, sizeof(h)),
+          "isf_memcpy_h2d")
+    dest
+end
+Base.getindex(a::ISFArray, mask::AbstractArray{Bool}) = Array(a)[hostify(mask)]
+function Base.setindex!(a::ISFArray, v, mask::AbstractArray{Bool})
+    h = Array(a)
+    h[hostify(mask)] = hostify(v)
+    upload!(a, h)
+end
 
 # ---------------------------------------------------------------- ISF
 """
@@ -143,20 +182,34 @@ function isjet(isf::ISF, nozzle_cen, nozzle_len, nozzle_rad)
     out
 end
 
-"`restrict_velocity!(isf, psi, kvec, isjet, omgterm)` (src/particles.jl:25-27); `phase_mode` 1 reproduces
-the reference's `tuple_dot` sum, 0 (default) the k.x phase of the MatlabPort driver."
-restrict_velocity!(isf::ISF, psi, kvec, isjetarr, omgterm = 0f0; phase_mode::Integer = 0) =
+"`restrict_velocity!(isf, psi, kvec, isjet, omgterm)` (src/particles.jl:25-27); `phase_mode` 1 (default,
+the reference's own call) reproduces its `tuple_dot` sum, 0 the k.x phase of the MatlabPort driver."
+restrict_velocity!(isf::ISF, psi, kvec, isjetarr, omgterm = 0f0; phase_mode::Integer = 1) =
     (check(ccall((:isf_restrict_velocity, libisf), Cint,
                  (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cfloat}, Cfloat, Cint),
                  isf.h, devptr(psi), devptr(isjetarr), Float32.(collect(kvec)), Float32(omgterm), Cint(phase_mode)),
            "restrict_velocity!"); psi)
 
-"`add_particles!(particles, range, cen, rad)` (src/particles.jl:34-94) with a seeded draw."
+"`add_particles!(isf, particles, range, cen, rad; seed, step)` (src/particles.jl:34-94): the
+draws of `range` from Philox4x32-10 with counter (j, step) and key `seed`."
 add_particles!(isf::ISF, particles::ISFArray, range::UnitRange, nozzle_cen, nozzle_rad; seed = 0, step = 0) =
     check(ccall((:isf_add_particles, libisf), Cint,
                 (Ptr{Cvoid}, Ptr{Cvoid}, Clonglong, Clonglong, Clonglong, Ptr{Cfloat}, Cfloat, Culonglong, Cuint),
                 isf.h, particles.ptr, length(particles), first(range) - 1, length(range),
                 Float32.(collect(nozzle_cen)), Float32(nozzle_rad), UInt64(seed), UInt32(step)), "add_particles!")
+
+# The reference's own call form `add_particles!(particles, start:stop, nozzle_cen, nozzle_rad)`
+# (src/particles.jl:60, test/runtests.jl:54) on the ISF created last; its unseeded rand
+# becomes Philox draws with the seed of set_spawn_seed! and the call number as the step.
+const SPAWN = Ref((seed = UInt64(20261203), calls = UInt32(0)))
+"Seed of the reference-form `add_particles!` (restarts its call counter)."
+set_spawn_seed!(seed::Integer) = (SPAWN[] = (seed = UInt64(seed), calls = UInt32(0)); nothing)
+function add_particles!(particles::ISFArray, range::UnitRange, nozzle_cen, nozzle_rad)
+    s = SPAWN[]
+    SPAWN[] = (seed = s.seed, calls = s.calls + UInt32(1))
+    add_particles!(CURRENT[], particles, range, nozzle_cen, nozzle_rad; seed = s.seed, step = s.calls + UInt32(1))
+    particles
+end
 
 "`staggered_advect!(particles, isf)` (src/parallel.jl:248-261), using isf.velocity."
 staggered_advect!(particles::ISFArray, isf::ISF) =
@@ -204,7 +257,14 @@ struct IsjetBC
     isf::Any
     args::Tuple
 end
-Base.Broadcast.broadcasted(::typeof(isjet), p::ISFPositions, cen, len, rad) = IsjetBC(p.isf, (cen, len, rad))
+# The reference passes the scalars wrapped as 1-tuples so they broadcast as scalars,
+# `isjet.(isf.positions, (nozzle_cen,), (nozzle_len,), (nozzle_rad,))` (test/runtests.jl:25);
+# Ref(x) is the modern spelling of the same thing.
+unwrap1(x) = x
+unwrap1(x::Tuple{Any}) = x[1]
+unwrap1(x::Base.RefValue) = x[]
+Base.Broadcast.broadcasted(::typeof(isjet), p::ISFPositions, cen, len, rad) =
+    IsjetBC(p.isf, (unwrap1(cen), unwrap1(len), unwrap1(rad)))
 Base.Broadcast.materialize(bc::IsjetBC) = isjet(bc.isf, bc.args...)
 
 # the driver's in-place StaggeredSharp: velocity .* inv.(isf.d) per component
@@ -219,6 +279,11 @@ end
 
 "GPUArrays.synchronize(array) of the drivers (test/runtests.jl:65)."
 synchronize(::ISFArray) = synchronize(CURRENT[])
+
+"The `GPUArrays.synchronize(particle)` call of the drivers (test/runtests.jl:65) resolves here."
+module GPUArrays
+import ..SchroedingersSmokeAMD: synchronize
+end
 
 # ---------------------------------------------------------------- batched uneven ensembles
 "Mirror of `isf_member_desc` (include/isf.h)."
@@ -386,7 +451,7 @@ module MatlabPort
 
 import ..SchroedingersSmokeAMD: libisf, check, ISFArray, devptr
 export ISF, land, SchroedingerFlow, Normalize, VelocityOneForm, StaggeredSharp, Div, PoissonSolve, GaugeTransform
-export PressureProject, StaggeredVelocity, StaggeredAdvect, Particles, Keep, removethedead!
+export PressureProject, StaggeredVelocity, StaggeredAdvect, Particles, Keep
 
 "land(a, b), src/MatlabPort.jl:5-8."
 land(a, b) = a .& b
@@ -536,12 +601,13 @@ mutable struct Particles
     h::Ptr{Cvoid}
     isf::ISF
     capacity::Int
+    snap::Any  # host copies of x, y, z while filter! runs over the active list
     function Particles(x::AbstractVector, y::AbstractVector, z::AbstractVector, len::Integer, active::AbstractVector)
         obj = CURRENT[]
         h = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:isf_particles_create, libisf), Cint, (Ref{Ptr{Cvoid}}, Clonglong), h, length(x)),
               "isf_particles_create")
-        p = new(h[], obj, length(x))
+        p = new(h[], obj, length(x), nothing)
         finalizer(q -> ccall((:isf_particles_destroy, libisf), Cint, (Ptr{Cvoid},), q.h), p)
         if len > 0
             active == 1:len || error("Particles: active must be 1:length")
@@ -567,14 +633,49 @@ function download(ptr::Ptr{Cvoid}, ::Type{T}, n::Integer) where {T}
     h
 end
 Base.length(p::Particles) = Int(state(p)[1])
+"""
+    ActiveList
+
+`p.active`: the 1-based active slot list read from the device.  `filter!(f, p.active)` (the
+driver's own removethedead!, examples/matlabport/simple.jl:63-70) evaluates `f` on the host,
+with `p.x`, `p.y`, `p.z` served from one snapshot of the device arrays, and writes the
+surviving list back to the device store (isf_particles_set_active).
+"""
+struct ActiveList <: AbstractVector{Int}
+    p::Particles
+    idx::Vector{Int}
+end
+Base.size(a::ActiveList) = size(a.idx)
+Base.getindex(a::ActiveList, i::Int) = a.idx[i]
+function Base.filter!(f, a::ActiveList)
+    p = a.p
+    st = state(p)
+    setfield!(p, :snap, (download(st[3], Float32, p.capacity), download(st[4], Float32, p.capacity),
+                         download(st[5], Float32, p.capacity)))
+    kept = try
+        filter(f, a.idx)
+    finally
+        setfield!(p, :snap, nothing)
+    end
+    zb = Cint.(kept .- 1)
+    check(ccall((:isf_particles_set_active, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cint}, Clonglong),
+                p.isf.h, p.h, zb, length(zb)), "filter!(p.active)")
+    empty!(a.idx)
+    append!(a.idx, kept)
+    a
+end
 function Base.getproperty(p::Particles, s::Symbol)
-    s in (:h, :isf, :capacity) && return getfield(p, s)
+    s in (:h, :isf, :capacity, :snap) && return getfield(p, s)
+    snap = getfield(p, :snap)
+    snap !== nothing && s === :x && return snap[1]
+    snap !== nothing && s === :y && return snap[2]
+    snap !== nothing && s === :z && return snap[3]
     st = state(p)
     s === :length && return Int(st[1])
     s === :x && return download(st[3], Float32, p.capacity)
     s === :y && return download(st[4], Float32, p.capacity)
     s === :z && return download(st[5], Float32, p.capacity)
-    s === :active && return Int.(download(st[6], Cint, st[2])) .+ 1
+    s === :active && return ActiveList(p, Int.(download(st[6], Cint, st[2])) .+ 1)
     error("Particles has no field $s")
 end
 
@@ -588,7 +689,8 @@ function Base.append!(p::Particles, x, y, z)
     nothing
 end
 
-"removethedead!(p, vol_size) of examples/matlabport/simple.jl:63-70 (strictly inside the box)."
+"removethedead!(p, vol_size) of examples/matlabport/simple.jl:63-70 (strictly inside the box) in one
+device pass; not exported, since that driver defines its own over `filter!(p.active)`."
 removethedead!(p::Particles, vol_size = nothing) =
     check(ccall((:isf_particles_remove_dead, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), p.isf.h, p.h), "removethedead!")
 
@@ -603,7 +705,8 @@ function StaggeredAdvect(p::Particles, torus::ISF, vx, vy, vz, dt)
     nothing
 end
 
-"Keep(particle, ind), src/MatlabPort.jl:226-230 (1-based ind)."
+"Keep(particle, ind), src/MatlabPort.jl:226-230 (1-based ind): x, y, z become x[ind], y[ind], z[ind];
+length and the active list are left as they are."
 function Keep(p::Particles, ind)
     d = ISFArray(Cint.(collect(ind)) .- Cint(1))
     check(ccall((:isf_particles_keep, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Clonglong), p.isf.h, p.h,

@@ -116,6 +116,39 @@ def test_remove_dead_on_the_device():
     assert 3 in active and 0 not in active and 1 not in active and 2 not in active
 
 
+def test_host_filter_of_the_active_list():
+    """The MatlabPort driver's own removethedead! (examples/matlabport/simple.jl:63-70)
+    filters p.active on the host with isinside; written back with isf_particles_set_active
+    it leaves the same active list as the device removethedead!, and the next advection
+    moves the same slots."""
+    from isf_amd import ISF, Particles
+    isf = ISF((4, 2, 2), (64, 32, 32), 0.1, 1 / 48)
+    rng = np.random.default_rng(5)
+    n = 4000
+    pts = rng.uniform(-0.3, 4.3, (n, 3)) * np.array([1.0, 0.5, 0.5])
+    A, B = Particles(isf, 2 * n), Particles(isf, 2 * n)
+    for P in (A, B):
+        P.append_(pts[:, 0], pts[:, 1], pts[:, 2])
+    A.remove_dead_()
+    ln, act, x, y, z = B.state()
+    keep = [i for i in act if 0 < x[i] < 4 and 0 < y[i] < 2 and 0 < z[i] < 2]  # isinside, strict
+    B.set_active_(keep)
+    la, acta, _, _, _ = A.state()
+    lb, actb, _, _, _ = B.state()
+    assert la == lb == n and np.array_equal(acta, actb) and 0 < len(actb) < n
+    vel = isf.velocity
+    vel.copy_from(np.full(isf.shape3 + (3,), 0.25, np.float32))
+    A.advect_()
+    B.advect_()
+    xa, xb = A.state()[2], B.state()[2]
+    assert np.array_equal(xa, xb)
+    with pytest.raises(Exception):
+        B.set_active_([2 * n])  # beyond the position arrays
+    A.close()
+    B.close()
+    isf.close()
+
+
 def test_keep_matches_the_reference():
     """Keep(p, ind) (src/MatlabPort.jl:226-230) replaces x, y, z by x[ind], y[ind], z[ind]
     and leaves length and the active list alone; the arrays are then len(ind) long, so the
