@@ -121,6 +121,12 @@ int isf_constrain(isf_t h, float* psi, int iterations);
 /* One simulation step number iter (1-based, t = iter*dt):
  * flow, normalize, project, spawn, restrict, project, one-form(hbar) * 1/d, RK4 advect. */
 int isf_step(isf_t h, float* psi, float* parts, long long nslots, int iter);
+/* simloop(N, isf, psi, ...) (test/runtests.jl:40-66, examples/benchmark.jl:41-67): steps
+ * iter0 .. iter0 + count - 1, the same results as count calls of isf_step (bit for bit),
+ * pipelined across steps (each step's velocity pass is fused into the next step's forward
+ * flow pass).  Asynchronous like isf_step; it may block the host until the GPU is at most
+ * about two steps behind. */
+int isf_simloop(isf_t h, float* psi, float* parts, long long nslots, int iter0, int count);
 /* isf_step measures how scattered the slot order of the particle buffer is (when the
  * buffer changes and every 256 steps); if it is scattered, slots are advected in a
  * spatial order (slots grouped by cell) recomputed every `steps` steps (default 32;

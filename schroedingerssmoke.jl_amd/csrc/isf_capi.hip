@@ -298,6 +298,10 @@ int isf_destroy(isf_t h) {
     if (h->order_scratch) (void)hipFree(h->order_scratch);
     if (h->jump_partial) (void)hipFree(h->jump_partial);
     if (h->vjp_mem) (void)hipFree(h->vjp_mem);
+    if (h->vel_alt) (void)hipFree(h->vel_alt);
+    if (h->ev_v) (void)hipEventDestroy(h->ev_v);
+    for (hipEvent_t e : h->ev_buf)
+        if (e) (void)hipEventDestroy(e);
     if (h->mp_psi) (void)hipFree(h->mp_psi);
     if (h->vjp_order) (void)hipFree(h->vjp_order);
     if (h->vjp_order_scratch) (void)hipFree(h->vjp_order_scratch);
@@ -707,6 +711,112 @@ int isf_step(isf_t h, float* psi, float* parts, long long nslots, int iter) {
     int r = step_args_ok(h, psi, parts, nslots, iter);
     if (r) return r;
     return step_impl(h, psi, parts, nslots, iter, nullptr);
+}
+
+// The driver's simloop (test/runtests.jl:40-66): steps iter0 .. iter0 + count - 1.  The
+// same kernels as isf_step, pipelined across steps: the forward flow x pass of step k + 1
+// also forms the velocity field of step k from the psi it reads (the one-form needs
+// psi(c + e_y) and psi(c + e_z), which no pass of step k has at hand), into one of two
+// alternating velocity buffers, and step k's advection runs on the side stream after
+// that pass (one event), overlapping step k + 1's remaining transforms.  psi is not read
+// on the side stream except by the last step's velocity pass, so the inverse flow pass
+// waits for nothing.  Before a fused pass rewrites a velocity buffer, the host waits for
+// the event after that buffer's last reader (the advection two steps back, long done by
+// then), so no stream waits on it.  Results are bit-identical to count calls of isf_step.
+static int simloop_impl(isf_t h, float* psi, float* parts, long long nslots, int iter0, int count) {
+    if (!h->vel_alt) {
+        hipError_t e = hipMalloc(&h->vel_alt, 12 * h->N);
+        if (e != hipSuccess) return fail(ISF_ENOMEM, "simloop velocity buffer: %s", hipGetErrorString(e));
+        if ((e = hipEventCreateWithFlags(&h->ev_v, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+        for (int b = 0; b < 2; b++)
+            if ((e = hipEventCreateWithFlags(&h->ev_buf[b], hipEventDisableTiming)) != hipSuccess)
+                return hip_fail(e, "hipEventCreate");
+    }
+    {
+        int r = join(h);  // earlier side work (readers of h->vel and psi) completes first
+        if (r) return r;
+    }
+    hipStream_t s = h->stream, ps = h->pstream;
+    float4* p = reinterpret_cast<float4*>(psi);
+    float* bufs[2] = {h->vel, h->vel_alt};
+    h->buf_rec[0] = h->buf_rec[1] = false;
+    const double sc = poisson_scale(h);
+    const VelOut vo0{nullptr, h->hbar, 1.0f / h->g.dx, 1.0f / h->g.dy, 1.0f / h->g.dz, 1};
+    for (int q = 0; q < count; q++) {
+        const int iter = iter0 + q;
+        const float t = (float)iter * h->dt;  // test/runtests.jl:46
+        JetParams j = h->jet;
+        j.omgterm = h->omega * t;             // test/runtests.jl:57
+        long long first0 = 0, nsp = 0;
+        if (nslots > 0 && h->n_spawn > 0) {
+            const long long start1 = ((long long)(iter - 1) * h->n_spawn + 1) % nslots;  // :52-54
+            first0 = start1 == 0 ? nslots - 1 : start1 - 1;
+            nsp = h->n_spawn;
+        }
+        if (q == 0 || nslots == 0) {
+            // first step of the call (the previous step's advection is already done), or no
+            // particles (the velocity field is only needed after the last step)
+            if (nslots > 0)
+                LAUNCH_CHECK(launch_spawn(parts, nslots, first0, nsp, h->jet, h->seed, (unsigned)iter, ps), "add_particles");
+            LAUNCH_CHECK(launch_flow_x_fwd(h->g, p, h->S, h->tb, s), "flow x forward");
+        } else {
+            // :59-62 of step iter - 1 inside this step's forward x pass, into bufs[q & 1]
+            const int b = q & 1;
+            if (h->buf_rec[b]) {  // its last reader, the advection of step iter - 3
+                if (hipEventQuery(h->ev_buf[b]) != hipSuccess)
+                    HIP_CHECK(hipEventSynchronize(h->ev_buf[b]), "hipEventSynchronize");
+            }
+            VelOut vo = vo0;
+            vo.vel = bufs[b];
+            LAUNCH_CHECK(launch_flow_x_fwd(h->g, p, h->S, h->tb, s, Batch{}, &vo), "flow x forward + velocity_one_form");
+            HIP_CHECK(hipEventRecord(h->ev_v, s), "hipEventRecord");
+            HIP_CHECK(hipStreamWaitEvent(ps, h->ev_v, 0), "hipStreamWaitEvent");
+            // :63 staggered_advect! of step iter - 1, then :52-54 add_particles! of this step
+            const int* order = nullptr;
+            int r = ensure_order(h, parts, nslots, iter - 1, &order, ps);
+            if (r) return r;
+            LAUNCH_CHECK(launch_advect(h->g, parts, nslots, order, bufs[b], h->dt, ps), "staggered_advect");
+            HIP_CHECK(hipEventRecord(h->ev_buf[b], ps), "hipEventRecord");
+            h->buf_rec[b] = true;
+            LAUNCH_CHECK(launch_spawn(parts, nslots, first0, nsp, h->jet, h->seed, (unsigned)iter, ps), "add_particles");
+        }
+        // :48-49 schroedinger_flow! + normalize, :50 + :57 projection with the restrict fused
+        // into its gauge pass, :58 projection with the FP64 norm partials
+        LAUNCH_CHECK(launch_flow_plane(h->g, h->S, h->tb, s), "flow plane");
+        LAUNCH_CHECK(launch_flow_x_inv(h->g, h->S, p, true, h->tb, s), "flow x inverse + normalize");
+        LAUNCH_CHECK(launch_div_r2c(h->g, p, f_of(h), P_of(h), h->tb, s), "div(one_form(psi)) + poisson x forward");
+        LAUNCH_CHECK(launch_poisson_plane(h->g, P_of(h), sc, h->tb, s), "poisson plane");
+        LAUNCH_CHECK(launch_c2r_x(h->g, P_of(h), 2, nullptr, p, &j, nullptr, h->tb, s), "poisson x inverse + gauge + restrict");
+        LAUNCH_CHECK(launch_div_r2c(h->g, p, f_of(h), P_of(h), h->tb, s), "div(one_form(psi)) + poisson x forward");
+        LAUNCH_CHECK(launch_poisson_plane(h->g, P_of(h), sc, h->tb, s), "poisson plane");
+        LAUNCH_CHECK(launch_c2r_x(h->g, P_of(h), 1, nullptr, p, nullptr, h->norm_parts, h->tb, s), "poisson x inverse + gauge");
+        h->norm_count = c2r_x_blocks(h->g);
+    }
+    // the last step's velocity and advection on the side stream, as isf_step leaves them
+    HIP_CHECK(hipEventRecord(h->ev_vel, s), "hipEventRecord");
+    HIP_CHECK(hipStreamWaitEvent(ps, h->ev_vel, 0), "hipStreamWaitEvent");
+    LAUNCH_CHECK(launch_velocity(h->g, p, h->hbar, 1, h->vel, ps), "velocity_one_form");
+    HIP_CHECK(hipEventRecord(h->ev_vread, ps), "hipEventRecord");
+    h->vread_pending = true;
+    if (nslots > 0) {
+        const int* order = nullptr;
+        int r = ensure_order(h, parts, nslots, iter0 + count - 1, &order, ps);
+        if (r) return r;
+        LAUNCH_CHECK(launch_advect(h->g, parts, nslots, order, h->vel, h->dt, ps), "staggered_advect");
+    }
+    HIP_CHECK(hipEventRecord(h->ev_adv, ps), "hipEventRecord");
+    h->side_pending = true;
+    return ISF_OK;
+}
+
+int isf_simloop(isf_t h, float* psi, float* parts, long long nslots, int iter0, int count) {
+    int r = step_args_ok(h, psi, parts, nslots, iter0);
+    if (r) return r;
+    if (count < 1) return fail(ISF_EINVAL, "count must be >= 1");
+    if (count > INT32_MAX - iter0) return fail(ISF_EINVAL, "iter0 + count overflows");
+    r = wait_ext(h);
+    if (r) return r;
+    return simloop_impl(h, psi, parts, nslots, iter0, count);
 }
 
 int isf_step_stage_count(void) { return kNumStages; }

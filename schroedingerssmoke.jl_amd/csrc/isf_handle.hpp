@@ -82,6 +82,12 @@ struct isf_handle {
     int order_check_iter;
     bool order_sorted;               // slots are advected through `order` (else slot order)
     unsigned* jump_partial;          // per-block counts of the order measure
+    // isf_simloop: second velocity buffer (allocated on first use), the event after each
+    // fused forward x pass, and per velocity buffer the event after its last reader
+    float* vel_alt;
+    hipEvent_t ev_v;
+    hipEvent_t ev_buf[2];
+    bool buf_rec[2];
     void* vjp_mem;                   // scratch of the operator VJPs (allocated on first use)
     float4* mp_psi;                  // interleaved staging of the MatlabPort PressureProject
     int* vjp_order;                  // cell-sorted slot order of the advection VJP

@@ -73,8 +73,16 @@ int x_tile_lines(int nx, int ny);
 // ---- transform passes (return hipError_t as int)
 // Every launcher takes an optional Batch (default: single instance).  In batch mode the
 // pointer arguments are ignored and g gives the shape shared by all members.
+// Velocity output of the fused forward x pass: vel = velocity_one_form(psi, hbar), times
+// (s0, s1, s2) = 1/d when sharp (the driver's StaggeredSharp).
+struct VelOut {
+    float* vel;
+    float hbar, s0, s1, s2;
+    int sharp;
+};
+// vo != null: the pass also writes the velocity field of psi (not in batch mode)
 int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s,
-                      Batch b = Batch{});
+                      Batch b = Batch{}, const VelOut* vo = nullptr);
 int launch_flow_plane(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s, Batch b = Batch{});
 int launch_flow_x_inv(const GridInfo& g, const float2* S, float4* psi, bool normalize, const DevTables& t,
                       hipStream_t s, Batch b = Batch{});

@@ -66,10 +66,16 @@ __device__ __forceinline__ StTile stencil_tile() {
 // (c, kx) planes of S with y fastest (src/parallel.jl:179-181, x part of fft!).  With
 // sa != null the components come from two separate complex arrays sa, sb (the MatlabPort
 // layout, src/MatlabPort.jl:87-93) instead of psi.
-template <int NX, int TY>
+// VEL (the step loop of isf_simloop): the pass also writes velocity_one_form!(isf, psi,
+// hbar) of the psi it reads, with the driver's StaggeredSharp (src/parallel.jl:115-122,
+// test/runtests.jl:59-62) -- the field the previous step's advection needs.  Each edge
+// angle has the operands and arithmetic of k_velocity (bit-identical field): psi(c+e_x)
+// and psi(c+e_y) come from the block's lines in LDS, the row above the tile and the lines
+// of the next z plane from global memory, loaded with the block's own lines.
+template <int NX, int TY, bool VEL>
 __global__ void __launch_bounds__(2 * TY * XCfg<NX>::T)
 k_flow_x_fwd(const float4* __restrict__ psi, const float2* __restrict__ sa, const float2* __restrict__ sb,
-             float2* __restrict__ S, int ny, int nz, const float2* __restrict__ tw, Batch B) {
+             float2* __restrict__ S, int ny, int nz, const float2* __restrict__ tw, Batch B, VelOut vo) {
     constexpr int T = XCfg<NX>::T, E = XCfg<NX>::E, LS = XCfg<NX>::LS;
     const StTile tl = stencil_tile();  // x: y tile, y: z, z: member
     if (B.inst) {
@@ -86,7 +92,9 @@ k_flow_x_fwd(const float4* __restrict__ psi, const float2* __restrict__ sa, cons
     const size_t line0 = ((size_t)z * ny + y0) * NX;
     // the block's lines: all loads issued before the twiddle-table loads and the LDS stores
     constexpr int K = TY * NX / NT;
+    static_assert(!VEL || NT >= NX, "one cell per line and thread");
     float2 va[K], vb[K];
+    float4 pz[VEL ? K : 1], ph = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (sa) {
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -100,6 +108,17 @@ k_flow_x_fwd(const float4* __restrict__ psi, const float2* __restrict__ sa, cons
             va[k] = make_float2(a.x, a.y);
             vb[k] = make_float2(a.z, a.w);
         }
+        if constexpr (VEL) {
+            // psi(c + e_z) of the block's cells, psi(c + e_y) of its last line
+            const size_t znext0 = ((size_t)(z + 1 < nz ? z + 1 : 0) * ny + y0) * NX;
+#pragma unroll
+            for (int k = 0; k < K; k++) pz[k] = psi[znext0 + tid + k * NT];
+            const int i_last = tid + (K - 1) * NT;  // the thread's cell on the last line, if any
+            if (i_last / NX == TY - 1) {
+                const int yn = y0 + TY < ny ? y0 + TY : 0;
+                ph = psi[((size_t)z * ny + yn) * NX + i_last % NX];
+            }
+        }
     }
     FftTw<NX, E>::template fill<NT>(twl, tw, tid);
 #pragma unroll
@@ -109,6 +128,34 @@ k_flow_x_fwd(const float4* __restrict__ psi, const float2* __restrict__ sa, cons
         lds[(TY + l) * LS + pad16(x)] = vb[k];
     }
     __syncthreads();
+    if constexpr (VEL) {
+        if (!sa) {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int i = tid + k * NT, l = i / NX, x = i % NX, xp = x + 1 < NX ? x + 1 : 0;
+                const float4 pc = make_float4(va[k].x, va[k].y, vb[k].x, vb[k].y);
+                const float2 ax = lds[l * LS + pad16(xp)], bx = lds[(TY + l) * LS + pad16(xp)];
+                const float4 px = make_float4(ax.x, ax.y, bx.x, bx.y);
+                float4 py = ph;
+                if (l + 1 < TY) {
+                    const float2 ay = lds[(l + 1) * LS + pad16(x)], by = lds[(TY + l + 1) * LS + pad16(x)];
+                    py = make_float4(ay.x, ay.y, by.x, by.y);
+                }
+                float v0 = edge_angle(pc, px) * vo.hbar, v1 = edge_angle(pc, py) * vo.hbar,
+                      v2 = edge_angle(pc, pz[k]) * vo.hbar;
+                if (vo.sharp) {
+                    v0 = v0 * vo.s0;
+                    v1 = v1 * vo.s1;
+                    v2 = v2 * vo.s2;
+                }
+                float* v = vo.vel + 3 * (line0 + i);
+                v[0] = v0;
+                v[1] = v1;
+                v[2] = v2;
+            }
+        }
+        __syncthreads();  // the lines' neighbours are read before the transform rewrites them
+    }
     const int line = tid / T;
     fft_line<NX, E, -1>(LayContig{lds + line * LS}, tid % T, true, twl);
     for (int i = tid; i < 2 * NX * TY; i += NT) {
@@ -1160,9 +1207,15 @@ static inline unsigned members_of(const Batch& b) { return b.inst ? (unsigned)b.
 
 template <int NX, int TY>
 static int flow_x_fwd_t(const GridInfo& g, const float4* psi, const float2* sa, const float2* sb, float2* S,
-                        const DevTables& t, hipStream_t s, Batch b) {
+                        const DevTables& t, hipStream_t s, Batch b, const VelOut* vo) {
     dim3 grid(g.ny / TY, g.nz, members_of(b));
-    ISF_LAUNCH((k_flow_x_fwd<NX, TY>), grid, dim3(2 * TY * XCfg<NX>::T), s, psi, sa, sb, S, g.ny, g.nz, t.tw, b);
+    if (vo) {
+        if (sa || b.inst) return (int)hipErrorInvalidValue;
+        ISF_LAUNCH((k_flow_x_fwd<NX, TY, true>), grid, dim3(2 * TY * XCfg<NX>::T), s, psi, sa, sb, S, g.ny, g.nz,
+                   t.tw, b, *vo);
+    }
+    ISF_LAUNCH((k_flow_x_fwd<NX, TY, false>), grid, dim3(2 * TY * XCfg<NX>::T), s, psi, sa, sb, S, g.ny, g.nz,
+               t.tw, b, VelOut{});
 }
 template <int NX, int TY>
 static int flow_x_inv_t(const GridInfo& g, const float2* S, float4* psi, float2* oa, float2* ob, float4* psin,
@@ -1218,8 +1271,9 @@ static int c2r_x_t(const GridInfo& g, const float2* P, int mode, float* q, float
         return (int)hipErrorInvalidValue;                                                       \
     } while (0)
 
-int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s, Batch b) {
-    ISF_X_DISPATCH(flow_x_fwd_t, g, psi, nullptr, nullptr, S, t, s, b);
+int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s, Batch b,
+                      const VelOut* vo) {
+    ISF_X_DISPATCH(flow_x_fwd_t, g, psi, nullptr, nullptr, S, t, s, b, vo);
 }
 int launch_flow_x_inv(const GridInfo& g, const float2* S, float4* psi, bool normalize, const DevTables& t,
                       hipStream_t s, Batch b) {
@@ -1231,7 +1285,7 @@ int launch_flow_x_inv_raw_normalized(const GridInfo& g, const float2* S, float4*
 }
 int launch_flow_x_fwd_split(const GridInfo& g, const float2* sa, const float2* sb, float2* S, const DevTables& t,
                             hipStream_t s) {
-    ISF_X_DISPATCH(flow_x_fwd_t, g, nullptr, sa, sb, S, t, s, Batch{});
+    ISF_X_DISPATCH(flow_x_fwd_t, g, nullptr, sa, sb, S, t, s, Batch{}, nullptr);
 }
 int launch_flow_x_inv_split(const GridInfo& g, const float2* S, float2* oa, float2* ob, const DevTables& t,
                             hipStream_t s) {

@@ -66,6 +66,7 @@ SIGNATURES = {
     "isf_fill_nozzle": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong, C.c_ulonglong], C.c_int),
     "isf_constrain": ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),
     "isf_step": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_int], C.c_int),
+    "isf_simloop": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.c_int], C.c_int),
     "isf_set_sort_interval": ([C.c_void_p, C.c_int], C.c_int),
     "isf_norm2": ([C.c_void_p, C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "isf_step_norm2": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),

@@ -230,6 +230,13 @@ class ISF:
         check(self.lib.isf_step(self.h, _ptr(psi), _ptr(particles), nslots, int(iteration)), "isf_step")
         return psi
 
+    def simloop_(self, psi, particles, first_iteration, count):
+        """Steps first_iteration .. first_iteration + count - 1 (the driver's simloop)."""
+        nslots = particles.shape[0] if particles is not None else 0
+        check(self.lib.isf_simloop(self.h, _ptr(psi), _ptr(particles), nslots, int(first_iteration), int(count)),
+              "isf_simloop")
+        return psi
+
     def step_profile(self, psi, particles, iteration):
         """One step with a HIP event between kernel stages: {stage name: ms}."""
         n = self.lib.isf_step_stage_count()

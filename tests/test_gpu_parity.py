@@ -430,3 +430,30 @@ def test_full_steps_large_plane(isf_mod):
     print(f"  32x256x128 psi rel L2 vs FP64 = {err:.3e}; particles {rep}")
     assert err < 1e-4
     assert rep["status_mismatch"] <= 2 and rep["p99"] < 1e-4
+
+
+@pytest.mark.parametrize("dims,box,n_slots,n_spawn", [((64, 32, 32), (4, 2, 2), 2400, 50),
+                                                      ((256, 128, 128), (8, 4, 4), 1_000_000, 1000),
+                                                      ((64, 32, 32), (4, 2, 2), 0, 0)])
+def test_simloop_equals_isf_step(isf_mod, dims, box, n_slots, n_spawn):
+    """isf_simloop (the driver's simloop, test/runtests.jl:40-66: the velocity pass of each
+    step fused into the next step's forward x pass, two alternating velocity buffers)
+    gives bit for bit the psi, particles, velocity field and FP64 norm of the same steps
+    taken one isf_step at a time, over calls of 1, 2, 3 and 5 steps in a row."""
+    cen = (0.3, box[1] / 2 - 0.034, box[2] / 2 + 0.066)
+    cfg = JetConfig(vol_size=box, dims=dims, nozzle_cen=cen, n_spawn=n_spawn, n_slots=n_slots,
+                    n_init=n_slots // 2, constraint_iters=4)
+    a, b = GpuRun(cfg), GpuRun(cfg)
+    it = 1
+    for count in (1, 2, 3, 5):
+        for k in range(count):
+            a.isf.step_(a.psi, a.parts if n_slots else None, it + k)
+        b.isf.simloop_(b.psi, b.parts if n_slots else None, it, count)
+        it += count
+        pa, pb = a.host_psi(), b.host_psi()
+        print(f"  {dims} after {it - 1} steps (call of {count}): psi rel {rel_l2(pb, pa):.2e}", flush=True)
+        assert np.array_equal(pa, pb)
+        if n_slots:
+            assert np.array_equal(a.host_parts(), b.host_parts())
+        assert np.array_equal(a.isf.velocity.to_host(), b.isf.velocity.to_host())
+        assert a.isf.step_norm2() == b.isf.step_norm2()
