@@ -14,8 +14,10 @@ namespace isf {
 // sign of a zero imaginary part is kept, as atan2 does.
 __device__ __forceinline__ float angle_of(float re, float im) {
     const float ax = fabsf(re), ay = fabsf(im);
-    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
-    const float a = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+    // a = mn / mx with the divisor clamped to the smallest normal float: the same value for
+    // every normal mx, and 0 (not 0 * inf) when both parts are zero
+    const float mx = fmaxf(fmaxf(ax, ay), 1.17549435e-38f), mn = fminf(ax, ay);
+    const float a = mn * __builtin_amdgcn_rcpf(mx);
     const float s = a * a;
     float p = 2.903499175e-03f;
     p = fmaf(p, s, -1.628280245e-02f);
