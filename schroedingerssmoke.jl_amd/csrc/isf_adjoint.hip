@@ -52,6 +52,13 @@ __global__ void __launch_bounds__(256) k_fill_f32(float* __restrict__ p, float v
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+// the same fill with 16-byte stores (p 16-byte aligned): float4 i of the n / 4, and the
+// last n % 4 floats by the first threads of block 0
+__global__ void __launch_bounds__(256) k_fill_f32x4(float* __restrict__ p, float v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n / 4) reinterpret_cast<float4*>(p)[i] = make_float4(v, v, v, v);
+    if (blockIdx.x == 0 && threadIdx.x < n % 4) p[n / 4 * 4 + threadIdx.x] = v;
+}
 
 __global__ void __launch_bounds__(256) k_fill_f64(double* __restrict__ p, double v, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -113,32 +120,33 @@ k_apply_phase_adj(float4* __restrict__ psibar, const float4* __restrict__ psi_ou
 }
 
 // thetabar = U^T phibar: each coarse node gathers the cells of its support (weight
-// (1 - |s - node|) per axis), a fixed summation order.
-__global__ void __launch_bounds__(64)
+// (1 - |s - node|) per axis), one workgroup per node: the threads take the support's cells
+// x fastest (coalesced rows), sum in FP64 and reduce in a fixed tree order.
+__global__ void __launch_bounds__(256)
 k_phase_gather(const float* __restrict__ phibar, float* __restrict__ thbar, GridInfo g, PhaseGrid P) {
-    const int node = blockIdx.x * blockDim.x + threadIdx.x;
-    const int M = P.pr[0] * P.pr[1] * P.pr[2];
-    if (node >= M) return;
+    const int node = blockIdx.x;
     const int m3[3] = {node % P.pr[0], (node / P.pr[0]) % P.pr[1], node / (P.pr[0] * P.pr[1])};
     const int n3[3] = {g.nx, g.ny, g.nz};
-    int r[3];
+    int r[3], w[3];
 #pragma unroll
-    for (int a = 0; a < 3; a++) r[a] = n3[a] / P.pr[a];
-    double acc = 0.0;
-    for (int dz = -r[2] + 1; dz < r[2]; dz++) {
-        const int z = (m3[2] * r[2] + dz + n3[2]) % n3[2];
-        const float wz = 1.0f - (float)abs(dz) / (float)r[2];
-        for (int dy = -r[1] + 1; dy < r[1]; dy++) {
-            const int y = (m3[1] * r[1] + dy + n3[1]) % n3[1];
-            const float wy = 1.0f - (float)abs(dy) / (float)r[1];
-            for (int dx = -r[0] + 1; dx < r[0]; dx++) {
-                const int x = (m3[0] * r[0] + dx + n3[0]) % n3[0];
-                const float wx = 1.0f - (float)abs(dx) / (float)r[0];
-                acc += (double)(wx * wy * wz) * (double)phibar[cidx(x, y, z, g)];
-            }
-        }
+    for (int a = 0; a < 3; a++) {
+        r[a] = n3[a] / P.pr[a];
+        w[a] = 2 * r[a] - 1;
     }
-    thbar[node] = (float)acc;
+    const int cnt = w[0] * w[1] * w[2];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < cnt; i += 256) {
+        const int dx = i % w[0] - (r[0] - 1), dy = (i / w[0]) % w[1] - (r[1] - 1), dz = i / (w[0] * w[1]) - (r[2] - 1);
+        const int x = (m3[0] * r[0] + dx + n3[0]) % n3[0];
+        const int y = (m3[1] * r[1] + dy + n3[1]) % n3[1];
+        const int z = (m3[2] * r[2] + dz + n3[2]) % n3[2];
+        const float wx = 1.0f - (float)abs(dx) / (float)r[0];
+        const float wy = 1.0f - (float)abs(dy) / (float)r[1];
+        const float wz = 1.0f - (float)abs(dz) / (float)r[2];
+        acc += (double)(wx * wy * wz) * (double)phibar[cidx(x, y, z, g)];
+    }
+    const double sum = block_sum<256>(acc);
+    if (threadIdx.x == 0) thbar[node] = (float)sum;
 }
 
 // ======================================================================== out-of-place forms
@@ -330,6 +338,22 @@ k_restrict_adj(float4* __restrict__ psibar, const float4* __restrict__ psi_in, G
 }
 
 // out[k] += sum over blocks of partials[4 b + k], k < 4 (one block, fixed order)
+// First level of the four sums of the restrict adjoint: block b adds the partial rows
+// [b chunk, (b + 1) chunk) (fixed order) into out[4 b .. 4 b + 3].
+__global__ void __launch_bounds__(256)
+k_reduce4_part(const double* __restrict__ partials, size_t nrows, size_t chunk, double* __restrict__ out) {
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    const size_t lo = (size_t)blockIdx.x * chunk, hi = lo + chunk < nrows ? lo + chunk : nrows;
+    for (size_t b = lo + threadIdx.x; b < hi; b += 256)
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[k] += partials[4 * b + k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double s = block_sum<256>(a[k]);
+        if (threadIdx.x == 0) out[4 * blockIdx.x + k] = s;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_reduce4(const double* __restrict__ partials, size_t nblocks, double* __restrict__ out) {
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     for (size_t b = threadIdx.x; b < nblocks; b += 256)
@@ -644,9 +668,12 @@ k_splat(const float* __restrict__ parts, long long nslots, SplatCfg sc, double* 
     }
 }
 
-// Same density, per block in LDS (M <= SPLAT_LDS cells): block b accumulates the slots
-// b, b + grid, ... into its own grid and writes it to partial[b * M ...]; k_splat_reduce
-// sums the block grids in block order.
+// Same density, per block in LDS (M <= SPLAT_LDS cells): each block accumulates its share
+// of the slots into its own grid and writes it to partial[b * M ...]; k_splat_reduce sums
+// the block grids in block order.  Neighbouring slots hold neighbouring particles (the
+// nozzle fill and the spawn batches are stored in spatial order), so the 64 lanes of a
+// wavefront take slots nslots / 64 apart: lanes that shared a cell would serialise on
+// the same LDS address in every atomic of the support loop.
 constexpr int SPLAT_LDS = 8192;
 __global__ void __launch_bounds__(256)
 k_splat_lds(const float* __restrict__ parts, long long nslots, SplatCfg sc, double* __restrict__ partial) {
@@ -655,7 +682,13 @@ k_splat_lds(const float* __restrict__ parts, long long nslots, SplatCfg sc, doub
     for (int j = threadIdx.x; j < M; j += 256) acc[j] = 0.0;
     __syncthreads();
     const float R2 = sc.R * sc.R, k = -0.5f / (sc.sigma * sc.sigma);
-    for (long long s = (long long)blockIdx.x * 256 + threadIdx.x; s < nslots; s += (long long)gridDim.x * 256) {
+    // slot lane * W * J + j * W + wave: a bijection of (lane, j, wave) onto [0, 64 W J),
+    // which covers [0, nslots) (W waves in the grid, J rounds)
+    const long long W = (long long)gridDim.x * 4, J = (nslots + 64 * W - 1) / (64 * W);
+    const long long wave = ((long long)blockIdx.x * 256 + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    for (long long j = 0; j < J; j++) {
+        const long long s = lane * W * J + j * W + wave;
+        if (s >= nslots) continue;
         const float p[3] = {parts[3 * s], parts[3 * s + 1], parts[3 * s + 2]};
         if (!splat_live(p, sc)) continue;
         int lo[3], hi[3];
@@ -748,6 +781,8 @@ int launch_fill_psi(float4* psi, float4 v, size_t n, hipStream_t s) {
 }
 int launch_fill_f32(float* p, float v, size_t n, hipStream_t s) {
     if (n == 0) return 0;
+    if ((reinterpret_cast<uintptr_t>(p) & 15u) == 0)
+        ISF_LAUNCH(k_fill_f32x4, grid1d(n / 4 > 0 ? n / 4 : 1, 256), dim3(256), s, p, v, n);
     ISF_LAUNCH(k_fill_f32, grid1d(n, 256), dim3(256), s, p, v, n);
 }
 int launch_fill_f64(double* p, double v, size_t n, hipStream_t s) {
@@ -765,7 +800,7 @@ int launch_apply_phase_adj(const GridInfo& g, float4* psibar, const float4* psi_
 }
 int launch_phase_gather(const GridInfo& g, const float* phibar, float* thbar, const int pr[3], hipStream_t s) {
     const PhaseGrid P{{pr[0], pr[1], pr[2]}};
-    ISF_LAUNCH(k_phase_gather, grid1d((size_t)pr[0] * pr[1] * pr[2], 64), dim3(64), s, phibar, thbar, g, P);
+    ISF_LAUNCH(k_phase_gather, dim3((unsigned)(pr[0] * pr[1] * pr[2])), dim3(256), s, phibar, thbar, g, P);
 }
 int launch_normalize_to(const GridInfo& g, const float4* in, float4* out, hipStream_t s) {
     ISF_LAUNCH(k_normalize_to, grid1d(ncells(g), 256), dim3(256), s, in, out, ncells(g));
@@ -796,13 +831,22 @@ int launch_oneform_adj_vel(const GridInfo& g, float4* psibar, const float4* psi,
                g, hbar, sharp, JetParams{}, qbar_out);
 }
 size_t restrict_adj_blocks(const GridInfo& g) { return (ncells(g) + 255) / 256; }
+// the first reduction level's blocks (their sums follow the per-block rows in partials)
+constexpr size_t kReduce4Blocks = 128;
+size_t restrict_adj_partials(const GridInfo& g) { return 4 * restrict_adj_blocks(g) + 4 * kReduce4Blocks; }
 int launch_restrict_adj(const GridInfo& g, float4* psibar, const float4* psi_in, const JetParams& j, double* partials,
                         double* out4, hipStream_t s, float* qbar_out) {
     hipLaunchKernelGGL(k_restrict_adj, grid1d(ncells(g), 256), dim3(256), 0, s, psibar, psi_in, g, j, partials,
                        qbar_out);
     int e = (int)hipGetLastError();
     if (e) return e;
-    ISF_LAUNCH(k_reduce4, dim3(1), dim3(256), s, partials, restrict_adj_blocks(g), out4);
+    const size_t nb = restrict_adj_blocks(g);
+    const size_t G = nb < kReduce4Blocks ? nb : kReduce4Blocks, chunk = (nb + G - 1) / G;
+    double* part2 = partials + 4 * nb;
+    hipLaunchKernelGGL(k_reduce4_part, dim3((unsigned)G), dim3(256), 0, s, partials, nb, chunk, part2);
+    e = (int)hipGetLastError();
+    if (e) return e;
+    ISF_LAUNCH(k_reduce4, dim3(1), dim3(256), s, part2, G, out4);
 }
 int launch_normalize_adj(const GridInfo& g, float4* psibar, const float4* psi_in, hipStream_t s) {
     ISF_LAUNCH(k_normalize_adj, grid1d(ncells(g), 256), dim3(256), s, psibar, psi_in, ncells(g));

@@ -181,9 +181,11 @@ int launch_oneform_adj_div(const GridInfo& g, float4* psibar, const float4* psi,
 // qbar_out (optional): the gauge cotangent of the new psibar against psi
 int launch_oneform_adj_vel(const GridInfo& g, float4* psibar, const float4* psi, const float* velbar, float hbar,
                            int sharp, hipStream_t s, float* qbar_out = nullptr);
-// partials: restrict_adj_blocks(g) x 4 doubles; out4[k] += the block sums (sum phibar,
+// partials: restrict_adj_partials(g) doubles (restrict_adj_blocks(g) rows of 4 per-block
+// sums, then the first reduction level's rows); out4[k] += the sums (sum phibar,
 // sum phibar x, sum phibar y, sum phibar z)
 size_t restrict_adj_blocks(const GridInfo& g);
+size_t restrict_adj_partials(const GridInfo& g);
 // qbar_out (optional): the gauge cotangent of the new psibar against psi_in
 int launch_restrict_adj(const GridInfo& g, float4* psibar, const float4* psi_in, const JetParams& j, double* partials,
                         double* out4, hipStream_t s, float* qbar_out = nullptr);

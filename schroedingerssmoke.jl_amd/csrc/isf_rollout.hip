@@ -69,8 +69,11 @@ struct isf_rollout {
     // velocity, particles); rec_done[i] / adj_done[i]: tape set i recomputed / reversed
     Work rw;
     hipEvent_t rec_done[2], adj_done[2], start;
-    // particle stream of the backward: vbar[i][k] = velocity cotangent of step k of the
-    // segment in tape set i (ready at pev[i][k]); pdone[i]: that segment's particle part done
+    // particle stream of the backward (the handle's side stream, idle during a rollout: a
+    // stream of its own would be the handle's fourth and share a hardware queue with the
+    // recompute stream under the runtime's default of four queues per process, which
+    // serialises the two); vbar[i][k] = velocity cotangent of step k of the segment in tape
+    // set i (ready at pev[i][k]); pdone[i]: that segment's particle part done
     hipStream_t pst;
     std::vector<float*> vbar[2];
     std::vector<hipEvent_t> pev[2];
@@ -123,7 +126,6 @@ static void rollout_free(isf_rollout_t r) {
             if (e) (void)hipEventDestroy(e);
         if (r->pdone[i]) (void)hipEventDestroy(r->pdone[i]);
     }
-    if (r->pst) (void)hipStreamDestroy(r->pst);
     if (r->rw.st) (void)hipStreamDestroy(r->rw.st);
     for (void* p : r->allocs) (void)hipFree(p);
     r->magic = 0;
@@ -444,7 +446,7 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
     RA(&r->target, (size_t)r->M);
     RA(&r->resid, (size_t)r->M);
     RA(&r->rho, (size_t)r->M);
-    RA(&r->partials, std::max(4 * restrict_adj_blocks(r->g), splat_loss_blocks(r->M)));
+    RA(&r->partials, std::max(restrict_adj_partials(r->g), splat_loss_blocks(r->M)));
     RA(&r->loss_dev, 1);
     RA(&r->vacc, 4 * (size_t)(r->T + r->C));
     r->spawn_n = h->n_spawn;
@@ -463,7 +465,7 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
             if (he == hipSuccess) he = hipEventCreateWithFlags(&r->adj_done[i], hipEventDisableTiming);
         }
         if (he == hipSuccess) he = hipEventCreateWithFlags(&r->start, hipEventDisableTiming);
-        if (he == hipSuccess) he = hipStreamCreate(&r->pst);
+        r->pst = h->pstream;
         for (int i = 0; i < 2 && he == hipSuccess; i++) {
             r->pev[i].assign(r->K, nullptr);
             for (int k = 0; k < r->K && he == hipSuccess; k++)
@@ -681,7 +683,7 @@ int isf_rollout_state(isf_rollout_t r, const float** psi, const float** parts, c
 // Scratch of the operator VJPs (psi copy, q, f, fbar), allocated on first use.
 static int vjp_scratch(isf_t h) {
     if (h->vjp_mem) return ISF_OK;
-    const hipError_t e = hipMalloc(&h->vjp_mem, 16 * h->N + 12 * h->N + 4 * 4 * restrict_adj_blocks(h->g) * 8);
+    const hipError_t e = hipMalloc(&h->vjp_mem, 16 * h->N + 12 * h->N + (4 + restrict_adj_partials(h->g)) * 8);
     if (e != hipSuccess) return fail(ISF_ENOMEM, "vjp scratch: %s", hipGetErrorString(e));
     return ISF_OK;
 }

@@ -52,18 +52,38 @@ k_bucket_keys(const float* __restrict__ parts, long long nslots, GridInfo g, int
 }
 
 // counting sort by cell: rank of each slot inside its cell, then the scatter
+// zero n counters (p 16-byte aligned): uint4 i of the n / 4, the last n % 4 by block 0
 __global__ void __launch_bounds__(256) k_fill_u32(unsigned* __restrict__ p, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
+    if (i < n / 4) reinterpret_cast<uint4*>(p)[i] = make_uint4(0u, 0u, 0u, 0u);
+    if (blockIdx.x == 0 && threadIdx.x < n % 4) p[n / 4 * 4 + threadIdx.x] = 0u;
 }
+// Rank of each slot inside its cell.  Neighbouring slots mostly share a cell, so the lanes
+// of a wavefront take their ranks with one atomic per distinct cell: the lowest pending
+// lane's cell is added once with the count of lanes in it, and each of those lanes takes
+// base + the number of lower lanes in the same cell.
 __global__ void __launch_bounds__(256)
 k_cell_rank(const float* __restrict__ parts, long long nslots, GridInfo g, int nb, unsigned* __restrict__ count,
             unsigned* __restrict__ key, unsigned* __restrict__ rank) {
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    const unsigned k = particle_bucket(parts[3 * s], parts[3 * s + 1], parts[3 * s + 2], g, nb, 0);
-    key[s] = k;
-    rank[s] = atomicAdd(count + k, 1u);
+    const bool act = s < nslots;
+    const unsigned k = act ? particle_bucket(parts[3 * s], parts[3 * s + 1], parts[3 * s + 2], g, nb, 0) : 0u;
+    if (act) key[s] = k;
+    const int lane = (int)(threadIdx.x & 63u);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long pending = __ballot(act);
+    unsigned r = 0u;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const unsigned kl = __shfl(k, leader, 64);
+        const unsigned long long same = __ballot(act && k == kl) & pending;
+        unsigned base = 0u;
+        if (lane == leader) base = atomicAdd(count + kl, (unsigned)__popcll(same));
+        base = __shfl(base, leader, 64);
+        if ((same >> lane) & 1ull) r = base + (unsigned)__popcll(same & below);
+        pending &= ~same;
+    }
+    if (act) rank[s] = r;
 }
 __global__ void __launch_bounds__(256)
 k_cell_scatter(long long nslots, const unsigned* __restrict__ key, const unsigned* __restrict__ rank,
@@ -115,7 +135,7 @@ int launch_cell_order(const GridInfo& g, const float* parts, long long nslots, i
     void* tmp = p + 2 * ab + 2 * an;
     size_t tmp_bytes = scratch_bytes - (2 * ab + 2 * an);
     const dim3 gs((unsigned)((nslots + 255) / 256));
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, count, nb);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nb / 4 + 255) / 256 + 1)), dim3(256), 0, s, count, nb);
     hipLaunchKernelGGL(k_cell_rank, gs, dim3(256), 0, s, parts, nslots, g, nbk, count, key, rank);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
