@@ -1053,9 +1053,13 @@ __global__ void __launch_bounds__(256) k_reduce_sum(const double* __restrict__ p
 // ======================================================================== particles
 
 // staggered_advect! (src/parallel.jl:190-216,248-261): RK4 for slots strictly inside
-// the box, the dead marker (-1,-1,-1) otherwise.  Thread k processes slot order[k]
-// (order == nullptr: slot k); every slot is independent, so the order only changes
-// which gathers share cache lines, never a result.
+// the box, the dead marker (-1,-1,-1) otherwise.  Each thread advances two slots, k and
+// k + blockDim.x of its block's range of 2 blockDim.x (order == nullptr: slot k; else
+// slot order[k]), one RK stage of both at a time: the two slots' gathers are independent,
+// so each stage waits for memory once for both.  Every slot is independent, so neither
+// the order nor the pairing changes a result.  A slot outside the box is stepped at a
+// stand-in position and only its marker is written.
+constexpr int ADVECT_SLOTS_PER_THREAD = 2;
 __global__ void __launch_bounds__(256)
 k_advect(float* __restrict__ parts, long long nslots, const int* __restrict__ order, const float* __restrict__ vel,
          GridInfo g, float dt, Batch B, const float* src) {
@@ -1069,31 +1073,63 @@ k_advect(float* __restrict__ parts, long long nslots, const int* __restrict__ or
         dt = m.dt;
         src = nullptr;
     }
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nslots) return;
-    const long long s = order ? (long long)order[k] : k;
-    const F3 p3 = ld3((src ? src : parts) + 3 * s);  // src: out of place, from a separate array
-    float p[3] = {p3.x, p3.y, p3.z};
-    const bool in = (p[0] > 0.0f) & (p[0] < g.Lx) & (p[1] > 0.0f) & (p[1] < g.Ly) & (p[2] > 0.0f) & (p[2] < g.Lz);
-    if (!in) {
-        parts[3 * s] = -1.0f;
-        parts[3 * s + 1] = -1.0f;
-        parts[3 * s + 2] = -1.0f;
-        return;
+    constexpr int Q = ADVECT_SLOTS_PER_THREAD;
+    const long long k0 = (long long)blockIdx.x * (Q * blockDim.x) + threadIdx.x;
+    long long s[Q];
+    bool act[Q], in[Q];
+    float p[Q][3];
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const long long k = k0 + (long long)j * blockDim.x;
+        act[j] = k < nslots;
+        s[j] = act[j] ? (order ? (long long)order[k] : k) : 0;
     }
-    float k1[3], k2[3], k3[3], k4[3], t[3];
-    stag_vel(vel, p, g, k1);
 #pragma unroll
-    for (int a = 0; a < 3; a++) t[a] = p[a] + k1[a] * dt * 0.5f;
-    stag_vel(vel, t, g, k2);
+    for (int j = 0; j < Q; j++) {
+        p[j][0] = p[j][1] = p[j][2] = -1.0f;
+        if (act[j]) {
+            const F3 p3 = ld3((src ? src : parts) + 3 * s[j]);  // src: out of place, from a separate array
+            p[j][0] = p3.x;
+            p[j][1] = p3.y;
+            p[j][2] = p3.z;
+        }
+        in[j] = act[j] & (p[j][0] > 0.0f) & (p[j][0] < g.Lx) & (p[j][1] > 0.0f) & (p[j][1] < g.Ly) &
+                (p[j][2] > 0.0f) & (p[j][2] < g.Lz);
+        if (!in[j]) {
+            p[j][0] = 0.5f * g.Lx;
+            p[j][1] = 0.5f * g.Ly;
+            p[j][2] = 0.5f * g.Lz;
+        }
+    }
+    float k1[Q][3], k2[Q][3], k3[Q][3], k4[Q][3], t[Q][3];
 #pragma unroll
-    for (int a = 0; a < 3; a++) t[a] = p[a] + k2[a] * dt * 0.5f;
-    stag_vel(vel, t, g, k3);
+    for (int j = 0; j < Q; j++) stag_vel(vel, p[j], g, k1[j]);
 #pragma unroll
-    for (int a = 0; a < 3; a++) t[a] = p[a] + k3[a] * dt;
-    stag_vel(vel, t, g, k4);
+    for (int j = 0; j < Q; j++)
 #pragma unroll
-    for (int a = 0; a < 3; a++) parts[3 * s + a] = p[a] + dt / 6.0f * (((k1[a] + 2.0f * k2[a]) + 2.0f * k3[a]) + k4[a]);
+        for (int a = 0; a < 3; a++) t[j][a] = p[j][a] + k1[j][a] * dt * 0.5f;
+#pragma unroll
+    for (int j = 0; j < Q; j++) stag_vel(vel, t[j], g, k2[j]);
+#pragma unroll
+    for (int j = 0; j < Q; j++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) t[j][a] = p[j][a] + k2[j][a] * dt * 0.5f;
+#pragma unroll
+    for (int j = 0; j < Q; j++) stag_vel(vel, t[j], g, k3[j]);
+#pragma unroll
+    for (int j = 0; j < Q; j++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) t[j][a] = p[j][a] + k3[j][a] * dt;
+#pragma unroll
+    for (int j = 0; j < Q; j++) stag_vel(vel, t[j], g, k4[j]);
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        if (!act[j]) continue;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            parts[3 * s[j] + a] =
+                in[j] ? p[j][a] + dt / 6.0f * (((k1[j][a] + 2.0f * k2[j][a]) + 2.0f * k3[j][a]) + k4[j][a]) : -1.0f;
+    }
 }
 
 // add_particles! (src/particles.jl:34-54,75-94): n particles at
@@ -1522,7 +1558,8 @@ int launch_advect(const GridInfo& g, float* parts, long long nslots, const int* 
     const long long nn = b.inst ? max_slots : nslots;
     if (nn <= 0) return 0;
     if (src == parts) src = nullptr;
-    const dim3 grid((unsigned)((nn + 255) / 256), members_of(b));
+    constexpr long long per_block = 256LL * ADVECT_SLOTS_PER_THREAD;
+    const dim3 grid((unsigned)((nn + per_block - 1) / per_block), members_of(b));
     ISF_LAUNCH(k_advect, grid, dim3(256), s, parts, nslots, order, vel, g, dt, b, src);
 }
 
