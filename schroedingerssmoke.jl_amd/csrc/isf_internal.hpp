@@ -141,8 +141,9 @@ int launch_advect(const GridInfo& g, float* parts, long long nslots, const int* 
 // Batch step counter: counter[0] = value / counter[0] += 1.
 int launch_set_counter(int* counter, int value, hipStream_t s);
 int launch_bump_counter(int* counter, hipStream_t s);
-// Slot processing order for the advection: slots sorted by bucket of 8 x-cells (stable
-// radix sort of (bucket, slot) pairs).  order_scratch_bytes(g, n) bytes of scratch.
+// Slot processing order for the advection: slots grouped by bucket of 8 x-cells
+// (counting sort; order inside a bucket arbitrary).  order_scratch_bytes(g, n) bytes of
+// scratch.
 size_t order_scratch_bytes(const GridInfo& g, long long nslots);
 int launch_particle_order(const GridInfo& g, const float* parts, long long nslots, int* order, void* scratch,
                           size_t scratch_bytes, hipStream_t s);
