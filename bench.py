@@ -604,6 +604,11 @@ def main():
         "stages": stages,
         "export": exports or None,
     }
+    # release the step leg's handle (its streams hold hardware queues, which the runtime
+    # shares out among the live streams of the process) before the other legs
+    psi.free()
+    parts.free()
+    isf.close()
     if not args.no_batch:
         print("[bench] batched ensemble leg", file=sys.stderr, flush=True)
         out["batched"] = bench_ensemble(args.batch_steps, 5)
