@@ -133,10 +133,11 @@ def bench_ensemble(steps, warmup, count=48, seed=20261203):
     """Batched uneven ensemble: `count` jets on grids drawn from {32^3, 64x32x32, 64^3,
     128x64x64} (box = grid/16), hbar in [0.05, 0.2], |v| in [0.5, 1.5] within 20 degrees of
     +x, 10k particle slots each filled in the nozzle, 100 spawned per step.  The same
-    ensemble is stepped (a) batched: one hipGraph replay per ensemble step, and (b)
-    sequentially: one isf_step per member, both with the autotuner's schedules cached for
-    this draw on an MI355X (isf_amd/schedules); wall-clock time over `steps` ensemble
-    steps after `warmup`, synchronised on both sides."""
+    ensemble is stepped (a) batched through the driver's simloop, `steps` steps in one
+    isf_batch_simloop call (the headline), (b) batched with one isf_batch_step (one graph
+    replay) per step, and (c) sequentially: one isf_step per member, all with the
+    autotuner's schedules cached for this draw on an MI355X (isf_amd/schedules); wall-clock
+    time over `steps` ensemble steps after `warmup`, synchronised on both sides."""
     from isf_amd.ensemble import ENSEMBLE48_SCHEDULES, build_batch, draw_jets
     ms = draw_jets(count=count, seed=seed)
     cells = sum(int(np.prod(m["res"])) for m in ms)
@@ -167,12 +168,22 @@ def bench_ensemble(steps, warmup, count=48, seed=20261203):
         return (time.perf_counter() - t) / steps, it
 
     t_seq, it = run(b.step_sequential_, 0)
-    t_bat, it = run(b.step_, it)
+    t_call, it = run(b.step_, it)
+    # the driver's simloop over the ensemble: the same steps in one isf_batch_simloop call
+    b.simloop_(it + 1, warmup)
+    it += warmup
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.simloop_(it + 1, steps)
+    b.synchronize()
+    t_bat = (time.perf_counter() - t0) / steps
+    it += steps
     norms = [b.members[i].step_norm2() / np.prod(ms[i]["res"]) for i in range(count)]
     b.close()
     balg = 248 * cells + 24 * slots
-    print(f"[bench] ensemble of {count}: batched {t_bat * 1e3:.3f} ms, sequential {t_seq * 1e3:.3f} ms per "
-          f"ensemble step (setup {setup_s:.1f} s)", file=sys.stderr, flush=True)
+    print(f"[bench] ensemble of {count}: batched simloop {t_bat * 1e3:.3f} ms, one isf_batch_step per step "
+          f"{t_call * 1e3:.3f} ms, sequential {t_seq * 1e3:.3f} ms per ensemble step (setup {setup_s:.1f} s)",
+          file=sys.stderr, flush=True)
     return {
         "workload": f"{count} independent jets, grids {counts}, 10k particle slots each, 100 spawned per step",
         "ensemble_steps_s": round(1.0 / t_bat, 2),
@@ -181,6 +192,8 @@ def bench_ensemble(steps, warmup, count=48, seed=20261203):
         "member_steps_s": round(count / t_bat, 1),
         "ms_per_step": round(t_bat * 1e3, 4),
         "ms_per_step_sequential": round(t_seq * 1e3, 4),
+        "isf_batch_step_per_call": {"ensemble_steps_s": round(1.0 / t_call, 2), "ms_per_step": round(t_call * 1e3, 4),
+                                    "speedup_vs_sequential": round(t_seq / t_call, 2)},
         "cells": cells,
         "B_alg_per_step": balg,
         "achieved_GB_s": round(balg / t_bat / 1e9, 1),

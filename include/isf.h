@@ -260,6 +260,13 @@ int isf_batch_bind(isf_batch_t b, int i, float* psi, float* parts, long long nsl
  * branches of one hipGraph captured on first use and replayed each step (set_graph 0:
  * launched directly).  Spawned particles are written in slot order. */
 int isf_batch_step(isf_batch_t b, int iter);
+/* Steps iter0 .. iter0 + count - 1 of every member, the driver's simloop over the
+ * ensemble (test/runtests.jl:40-66, one member per instance of its loop): equal to count
+ * isf_batch_step calls, bit for bit.  Each later step's forward flow pass also writes the
+ * previous step's velocity (two buffers per member, chosen by the step number), so the
+ * velocity pass leaves the step; one graph for the first step and one for every later
+ * step, captured on first use. */
+int isf_batch_simloop(isf_batch_t b, int iter0, int count);
 int isf_batch_set_graph(isf_batch_t b, int use_graph);
 /* The same step as n separate isf_step calls, one member after another (baseline). */
 int isf_batch_step_sequential(isf_batch_t b, int iter);

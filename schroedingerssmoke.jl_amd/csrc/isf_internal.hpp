@@ -49,6 +49,7 @@ struct Inst {
     float4* psi;
     float2* S;               // spectrum; f = (float*)S and P = S + 4N bytes alias it
     float* vel;
+    float* vel_alt;          // second velocity buffer (batched step loop)
     float* parts;
     long long nslots;
     long long n_spawn;
@@ -65,6 +66,9 @@ struct Batch {
     const Inst* inst;
     const int* iter;
     int members;
+    // batched step loop: the velocity of step *iter - 1 lives in vel_alt when that step is
+    // odd, in vel when even (the fused forward pass writes it, the advection reads it)
+    int vel_by_step;
 };
 
 // Launch-time choices of the x-direction passes (g.ty, when non-zero, overrides).
@@ -80,7 +84,8 @@ struct VelOut {
     float hbar, s0, s1, s2;
     int sharp;
 };
-// vo != null: the pass also writes the velocity field of psi (not in batch mode)
+// vo != null: the pass also writes the velocity field of psi (in batch mode each member's
+// hbar and 1/d, into the buffer of step *iter - 1; vo's fields are then not read)
 int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s,
                       Batch b = Batch{}, const VelOut* vo = nullptr);
 int launch_flow_plane(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s, Batch b = Batch{});

@@ -83,6 +83,14 @@ k_flow_x_fwd(const float4* __restrict__ psi, const float2* __restrict__ sa, cons
         psi = m.psi;
         S = m.S;
         tw = m.tb.tw;
+        if constexpr (VEL) {
+            vo.vel = ((*B.iter - 1) & 1) ? m.vel_alt : m.vel;
+            vo.hbar = m.hbar;
+            vo.s0 = 1.0f / m.g.dx;
+            vo.s1 = 1.0f / m.g.dy;
+            vo.s2 = 1.0f / m.g.dz;
+            vo.sharp = 1;
+        }
     }
     constexpr int NT = 2 * TY * T;
     __shared__ float2 lds[2 * TY * LS];
@@ -1068,7 +1076,7 @@ k_advect(float* __restrict__ parts, long long nslots, const int* __restrict__ or
         parts = m.parts;
         nslots = m.nslots;
         order = nullptr;
-        vel = m.vel;
+        vel = (B.vel_by_step && ((*B.iter - 1) & 1)) ? m.vel_alt : m.vel;
         g = m.g;
         dt = m.dt;
         src = nullptr;
@@ -1246,7 +1254,7 @@ static int flow_x_fwd_t(const GridInfo& g, const float4* psi, const float2* sa, 
                         const DevTables& t, hipStream_t s, Batch b, const VelOut* vo) {
     dim3 grid(g.ny / TY, g.nz, members_of(b));
     if (vo) {
-        if (sa || b.inst) return (int)hipErrorInvalidValue;
+        if (sa) return (int)hipErrorInvalidValue;
         ISF_LAUNCH((k_flow_x_fwd<NX, TY, true>), grid, dim3(2 * TY * XCfg<NX>::T), s, psi, sa, sb, S, g.ny, g.nz,
                    t.tw, b, *vo);
     }

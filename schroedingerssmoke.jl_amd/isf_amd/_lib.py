@@ -109,6 +109,7 @@ SIGNATURES = {
     "isf_batch_member": ([C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
     "isf_batch_bind": ([C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_longlong], C.c_int),
     "isf_batch_step": ([C.c_void_p, C.c_int], C.c_int),
+    "isf_batch_simloop": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "isf_batch_set_graph": ([C.c_void_p, C.c_int], C.c_int),
     "isf_batch_step_sequential": ([C.c_void_p, C.c_int], C.c_int),
     "isf_batch_synchronize": ([C.c_void_p], C.c_int),

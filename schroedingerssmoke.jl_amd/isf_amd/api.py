@@ -341,6 +341,11 @@ class Batch:
     def step_(self, iteration):
         check(self.lib.isf_batch_step(self.h, int(iteration)), "isf_batch_step")
 
+    def simloop_(self, first_iteration, count):
+        """Steps first_iteration .. first_iteration + count - 1 of every member (the
+        driver's simloop over the ensemble); equal to count step_ calls."""
+        check(self.lib.isf_batch_simloop(self.h, int(first_iteration), int(count)), "isf_batch_simloop")
+
     def step_sequential_(self, iteration):
         check(self.lib.isf_batch_step_sequential(self.h, int(iteration)), "isf_batch_step_sequential")
 

@@ -13,7 +13,7 @@ module SchroedingersSmokeAMD
 
 export ISF, ISFArray, normalize_psi!, pressure_project!, velocity_one_form!, schroedinger_flow!
 export staggered_advect!, add_particles!, restrict_velocity!, isjet, synchronize
-export ISFBatch, ISFRollout, rollout_loss, rollout_gradient, normalize_psi, set_spawn_seed!
+export ISFBatch, ISFRollout, rollout_loss, rollout_gradient, normalize_psi, set_spawn_seed!, simloop!
 export Complex64, GPUArrays
 
 const libisf = joinpath(@__DIR__, "..", "lib", "libisf.so")
@@ -228,6 +228,15 @@ jet_configure!(isf::ISF, kvec, omega, nozzle_cen, nozzle_len, nozzle_rad, n_spaw
 step!(isf::ISF, psi, particles::ISFArray, iter::Integer) =
     check(ccall((:isf_step, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Clonglong, Cint),
                 isf.h, devptr(psi), particles.ptr, length(particles), Cint(iter)), "isf_step")
+"""
+    simloop!(isf, psi, particles, first_iter, count)
+
+The driver's `simloop(N, …)` (test/runtests.jl:40-66): `count` iterations from `first_iter`
+in one call (`isf_simloop`), equal to `count` `step!` calls.
+"""
+simloop!(isf::ISF, psi, particles::ISFArray, first_iter::Integer, count::Integer) =
+    check(ccall((:isf_simloop, libisf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Clonglong, Cint, Cint),
+                isf.h, devptr(psi), particles.ptr, length(particles), Cint(first_iter), Cint(count)), "isf_simloop")
 
 # ---------------------------------------------------------------- the reference's broadcast call forms
 # The reference drivers write (test/runtests.jl:21,25,49,61-62)
@@ -336,6 +345,10 @@ function bind!(b::ISFBatch, i::Integer, psi::ISFArray, particles::ISFArray)
 end
 step!(b::ISFBatch, iter::Integer) =
     check(ccall((:isf_batch_step, libisf), Cint, (Ptr{Cvoid}, Cint), b.h, iter), "isf_batch_step")
+"`count` ensemble steps from `first_iter` in one call (`isf_batch_simloop`), equal to `count` `step!` calls."
+simloop!(b::ISFBatch, first_iter::Integer, count::Integer) =
+    check(ccall((:isf_batch_simloop, libisf), Cint, (Ptr{Cvoid}, Cint, Cint), b.h, first_iter, count),
+          "isf_batch_simloop")
 tune!(b::ISFBatch, cache::AbstractString = "") =
     check(ccall((:isf_batch_tune, libisf), Cint, (Ptr{Cvoid}, Cstring), b.h, isempty(cache) ? C_NULL : cache),
           "isf_batch_tune")

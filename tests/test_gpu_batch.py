@@ -77,6 +77,28 @@ def test_batched_equals_sequential(ens):
     s.close()
 
 
+def test_batch_simloop_equals_batch_steps(ens):
+    """isf_batch_simloop (the driver's simloop over the ensemble) over steps 1..7, in calls
+    of 3 and 4 steps, equals seven isf_batch_step calls bit for bit: psi, particles, the
+    last step's velocity field and the FP64 norm, with and without graph replay."""
+    ms = members(ens, 6, seed=41)
+    for use_graph in (True, False):
+        a = ens.build_batch(ms, n_init=1500)
+        s = ens.build_batch(ms, n_init=1500)
+        a.set_graph(use_graph)
+        s.set_graph(use_graph)
+        for it in range(1, 8):
+            s.step_(it)
+        a.simloop_(1, 3)
+        a.simloop_(4, 4)
+        assert_same(state(a), state(s), f"simloop vs steps (graph {use_graph})")
+        for i in range(len(ms)):
+            assert np.array_equal(a.members[i].velocity.to_host(), s.members[i].velocity.to_host()), i
+            assert a.members[i].step_norm2() == s.members[i].step_norm2(), i
+        a.close()
+        s.close()
+
+
 def test_graph_replays_match_direct_launches(ens):
     ms = members(ens, 5, seed=12)
     g = ens.build_batch(ms, n_init=1000)
@@ -224,8 +246,8 @@ def test_c3_ensemble_of_48_jets(ens, tmp_path):
     """The batched uneven ensemble at its full size: 48 jets drawn by draw_jets (grids
     32^3, 64x32x32, 64^3, 128x64x64; hbar in [0.05, 0.2]; |v| in [0.5, 1.5] within 20
     degrees of +x; 10k slots filled in the nozzle, 100 spawned per step) with the
-    autotuner's cached MI355X schedules.  Batched (one graph replay per step) equals the
-    members stepped one after another with isf_step bit for bit, and the first member of
+    autotuner's cached MI355X schedules.  Batched (isf_batch_simloop over the 6 steps)
+    equals the members stepped one after another with isf_step bit for bit, and the first member of
     each grid shape is within 1e-4 of the FP64 oracle after 6 steps
     (the reference's per-instance loop, test/runtests.jl:45-64)."""
     import shutil
@@ -239,8 +261,8 @@ def test_c3_ensemble_of_48_jets(ens, tmp_path):
     for res, ty in (((32, 32, 32), 8), ((64, 32, 32), 16), ((64, 64, 64), 32), ((128, 64, 64), 16)):
         assert a.tile(res) == ty and s.tile(res) == ty
     steps = 6
+    a.simloop_(1, steps)  # the driver's simloop over the ensemble, as the bench runs it
     for it in range(1, steps + 1):
-        a.step_(it)
         s.step_sequential_(it)
     sa, ss = state(a), state(s)
     assert_same(sa, ss, "48-jet ensemble, 6 steps")
