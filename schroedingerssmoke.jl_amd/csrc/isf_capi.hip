@@ -628,7 +628,8 @@ static int step_impl(isf_t h, float* psi, float* parts, long long nslots, int it
         int r = ensure_order(h, parts, nslots, iter, &order, ps);
         if (r) return r;
         if ((r = mark())) return r;
-        STAGE(launch_advect(h->g, parts, nslots, order, h->vel, h->dt, ps), "staggered_advect");
+        STAGE(launch_advect(h->g, parts, nslots, order, h->vel, h->dt, ps, Batch{}, 0, nullptr, side),
+              "staggered_advect");
     }
     if (side) {
         HIP_CHECK(hipEventRecord(h->ev_adv, ps), "hipEventRecord");
@@ -763,7 +764,8 @@ static int simloop_impl(isf_t h, float* psi, float* parts, long long nslots, int
             const int* order = nullptr;
             int r = ensure_order(h, parts, nslots, iter - 1, &order, ps);
             if (r) return r;
-            LAUNCH_CHECK(launch_advect(h->g, parts, nslots, order, bufs[side_b], h->dt, ps), "staggered_advect");
+            LAUNCH_CHECK(launch_advect(h->g, parts, nslots, order, bufs[side_b], h->dt, ps, Batch{}, 0, nullptr, true),
+                         "staggered_advect");
             HIP_CHECK(hipEventRecord(h->ev_buf[side_b], ps), "hipEventRecord");
             h->buf_rec[side_b] = true;
             LAUNCH_CHECK(launch_spawn(parts, nslots, first0, nsp, h->jet, h->seed, (unsigned)iter, ps), "add_particles");
@@ -816,7 +818,8 @@ static int simloop_impl(isf_t h, float* psi, float* parts, long long nslots, int
         const int* order = nullptr;
         int r = ensure_order(h, parts, nslots, iter0 + count - 1, &order, ps);
         if (r) return r;
-        LAUNCH_CHECK(launch_advect(h->g, parts, nslots, order, h->vel, h->dt, ps), "staggered_advect");
+        LAUNCH_CHECK(launch_advect(h->g, parts, nslots, order, h->vel, h->dt, ps, Batch{}, 0, nullptr, true),
+                     "staggered_advect");
     }
     HIP_CHECK(hipEventRecord(h->ev_adv, ps), "hipEventRecord");
     h->side_pending = true;

@@ -141,8 +141,11 @@ size_t order_measure_ints(long long nslots);
 int measure_order_jumps(const GridInfo& g, const float* parts, long long nslots, unsigned* dev_partial,
                         unsigned long long* host_count, hipStream_t s);
 // src (optional, not in batch mode): advect the positions in src into parts (out of place)
+// beside: the advection runs beside other kernels (the step's side stream): one block per
+// CU, so it slows the concurrent plane passes evenly
 int launch_advect(const GridInfo& g, float* parts, long long nslots, const int* order, const float* vel, float dt,
-                  hipStream_t s, Batch b = Batch{}, long long max_slots = 0, const float* src = nullptr);
+                  hipStream_t s, Batch b = Batch{}, long long max_slots = 0, const float* src = nullptr,
+                  bool beside = false);
 // Batch step counter: counter[0] = value / counter[0] += 1.
 int launch_set_counter(int* counter, int value, hipStream_t s);
 int launch_bump_counter(int* counter, hipStream_t s);
