@@ -790,8 +790,8 @@ static int simloop_impl(isf_t h, float* psi, float* parts, long long nslots, int
             side_b = b;
         }
         // the advection right after the fused pass overlaps the flow plane pass: measured
-        // 0.282 ms per step, against 0.300 (after the inverse flow pass) and 0.287 (after the
-        // first divergence pass, beside the Poisson plane pass)
+        // 0.272 ms per step, against 0.284 (after the flow plane pass), 0.290 (after the
+        // inverse flow pass) and 0.329 (after the first divergence pass)
         if (side_b >= 0) {
             int r = side();
             if (r) return r;
