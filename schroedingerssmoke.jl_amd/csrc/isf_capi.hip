@@ -725,9 +725,14 @@ int isf_step(isf_t h, float* psi, float* parts, long long nslots, int iter) {
 // the event after that buffer's last reader (the advection two steps back, long done by
 // then), so no stream waits on it.  Results are bit-identical to count calls of isf_step.
 static int simloop_impl(isf_t h, float* psi, float* parts, long long nslots, int iter0, int count) {
+    // the second velocity buffer (a batched step loop may have allocated it already) and
+    // the loop's events
     if (!h->vel_alt) {
-        hipError_t e = hipMalloc(&h->vel_alt, 12 * h->N);
+        const hipError_t e = hipMalloc(&h->vel_alt, 12 * h->N);
         if (e != hipSuccess) return fail(ISF_ENOMEM, "simloop velocity buffer: %s", hipGetErrorString(e));
+    }
+    if (!h->ev_v) {
+        hipError_t e;
         if ((e = hipEventCreateWithFlags(&h->ev_v, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
         for (int b = 0; b < 2; b++)
             if ((e = hipEventCreateWithFlags(&h->ev_buf[b], hipEventDisableTiming)) != hipSuccess)
@@ -940,6 +945,7 @@ struct isf_batch {
     bool dirty;              // descriptors must be rebuilt (bind, tune)
     hipGraphExec_t exec;     // isf_batch_step
     hipGraphExec_t exec_first, exec_steady;  // isf_batch_simloop: first step, later steps
+    bool alt_uploaded;       // every uploaded descriptor carries its member's second velocity buffer
 };
 
 static int check_batch(isf_batch_t b) {
@@ -1027,6 +1033,8 @@ static int batch_prepare(isf_batch_t b) {
         HIP_CHECK(hipMemcpy(gr.inst, host.data(), host.size() * sizeof(Inst), hipMemcpyHostToDevice),
                   "batch descriptors upload");
     }
+    b->alt_uploaded = true;
+    for (isf_t h : b->mem) b->alt_uploaded = b->alt_uploaded && h->vel_alt != nullptr;
     b->dirty = false;
     return ISF_OK;
 }
@@ -1272,13 +1280,14 @@ int isf_batch_simloop(isf_batch_t b, int iter0, int count) {
     if (iter0 < 1) return fail(ISF_EINVAL, "iter0 must be >= 1");
     if (count < 1) return fail(ISF_EINVAL, "count must be >= 1");
     if (count == 1) return isf_batch_step(b, iter0);
-    // the second velocity buffer of every member
+    // the second velocity buffer of every member (a member's own isf_simloop may have
+    // allocated it after the descriptors were uploaded: then they are uploaded again)
     for (isf_t h : b->mem)
         if (!h->vel_alt) {
             const hipError_t e = hipMalloc(&h->vel_alt, 12 * h->N);
             if (e != hipSuccess) return fail(ISF_ENOMEM, "second velocity buffer: %s", hipGetErrorString(e));
-            b->dirty = true;
         }
+    if (!b->alt_uploaded) b->dirty = true;
     if (b->dirty) {
         int r = batch_prepare(b);
         if (r) return r;

@@ -99,6 +99,28 @@ def test_batch_simloop_equals_batch_steps(ens):
         s.close()
 
 
+def test_member_and_batch_step_loops_interleave(ens):
+    """A member's own isf_simloop and isf_batch_simloop share the member's second velocity
+    buffer: step 1 batched, steps 2-3 in each member's isf_simloop, 4-6 in one
+    isf_batch_simloop, 7-8 in each member's isf_simloop again equal isf_batch_step 1..8."""
+    ms = members(ens, 4, seed=43)
+    a = ens.build_batch(ms, n_init=1200)
+    s = ens.build_batch(ms, n_init=1200)
+    for it in range(1, 9):
+        s.step_(it)
+    a.step_(1)
+    for i in range(len(ms)):
+        psi, parts = a._bound[i]
+        a.members[i].simloop_(psi, parts, 2, 2)
+    a.simloop_(4, 3)
+    for i in range(len(ms)):
+        psi, parts = a._bound[i]
+        a.members[i].simloop_(psi, parts, 7, 2)
+    assert_same(state(a), state(s), "interleaved member and batch loops")
+    a.close()
+    s.close()
+
+
 def test_graph_replays_match_direct_launches(ens):
     ms = members(ens, 5, seed=12)
     g = ens.build_batch(ms, n_init=1000)
