@@ -23,6 +23,8 @@
 //                                         trilinear weights scattered into velbar (atomics)
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "isf_device.hpp"
 #include "isf_internal.hpp"
 
@@ -668,24 +670,45 @@ k_splat(const float* __restrict__ parts, long long nslots, SplatCfg sc, double* 
     }
 }
 
-// Same density, per block in LDS (M <= SPLAT_LDS cells): each block accumulates its share
-// of the slots into its own grid and writes it to partial[b * M ...]; k_splat_reduce sums
-// the block grids in block order.  Neighbouring slots hold neighbouring particles (the
-// nozzle fill and the spawn batches are stored in spatial order), so the 64 lanes of a
-// wavefront take slots nslots / 64 apart: lanes that shared a cell would serialise on
-// the same LDS address in every atomic of the support loop.
-constexpr int SPLAT_LDS = 8192;
-__global__ void __launch_bounds__(256)
+// Same density, per block in LDS: each block accumulates its share of the slots into its
+// own grid and writes it to partial[b * M ...]; k_splat_reduce sums the block grids.
+//
+// The Gaussian factors over the axes, exp(k |y - p|^2) = exp(k dx^2) exp(k (dy^2 + dz^2)):
+// a slot takes one exp per x offset of its support (at most XW, held in registers) and
+// one per (y, z) row, instead of one per cell; the cut-off |y - p| <= R is still decided
+// per cell on dx^2 + (dy^2 + dz^2).  A slot wider than XW in x (possible only through
+// rounding at the support's ends) takes the per-cell path.
+//
+// LDS traffic: a cloud of slots a few cells across puts many lanes of a wavefront on the
+// same cells.  Lanes take slots nslots / 64 apart (neighbouring slots hold neighbouring
+// particles: the nozzle fill and the spawn batches are stored in spatial order), each lane
+// starts its walk over the support rows at its own row (13 lane mod rows), and the grid
+// rows in LDS are padded (x stride 1, y stride res0 + 1, z stride (res0 + 1) res1 + 1) so
+// that equal x offsets of different rows fall in different banks.
+//
+// Exactness (cut-off R <= 4 sigma, fewer than 2^25 slots): every term is a float between
+// exp(-8) inv_n > 2^-12 inv_n and inv_n, hence a multiple of 2^-35 of inv_n's binade, and a
+// block adds at most 2^17 terms per cell, so its FP64 sums are exact and do not depend on
+// the order of the atomics (each halving of the smallest term costs one bit of margin).
+constexpr int SPLAT_THREADS = 1024;
+constexpr int SPLAT_BLOCKS = 256;
+constexpr int SPLAT_LDS_DOUBLES = 16384;  // 128 KiB of the CU's 160
+__host__ __device__ inline int splat_sy(const SplatCfg& sc) { return sc.res[0] + 1; }
+__host__ __device__ inline int splat_sz(const SplatCfg& sc) { return (sc.res[0] + 1) * sc.res[1] + 1; }
+template <int XW>
+__global__ void __launch_bounds__(SPLAT_THREADS)
 k_splat_lds(const float* __restrict__ parts, long long nslots, SplatCfg sc, double* __restrict__ partial) {
-    __shared__ double acc[SPLAT_LDS];
+    extern __shared__ double acc[];
     const int M = sc.res[0] * sc.res[1] * sc.res[2];
-    for (int j = threadIdx.x; j < M; j += 256) acc[j] = 0.0;
+    const int sy = splat_sy(sc), sz = splat_sz(sc), MP = sz * sc.res[2];
+    for (int j = threadIdx.x; j < MP; j += SPLAT_THREADS) acc[j] = 0.0;
     __syncthreads();
     const float R2 = sc.R * sc.R, k = -0.5f / (sc.sigma * sc.sigma);
     // slot lane * W * J + j * W + wave: a bijection of (lane, j, wave) onto [0, 64 W J),
     // which covers [0, nslots) (W waves in the grid, J rounds)
-    const long long W = (long long)gridDim.x * 4, J = (nslots + 64 * W - 1) / (64 * W);
-    const long long wave = ((long long)blockIdx.x * 256 + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    const long long W = (long long)gridDim.x * (SPLAT_THREADS / 64), J = (nslots + 64 * W - 1) / (64 * W);
+    const long long wave = ((long long)blockIdx.x * SPLAT_THREADS + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
     for (long long j = 0; j < J; j++) {
         const long long s = lane * W * J + j * W + wave;
         if (s >= nslots) continue;
@@ -694,32 +717,81 @@ k_splat_lds(const float* __restrict__ parts, long long nslots, SplatCfg sc, doub
         int lo[3], hi[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) splat_range(p[a], sc.h[a], sc.R, sc.res[a], lo[a], hi[a]);
-        for (int jz = lo[2]; jz <= hi[2]; jz++) {
-            const float dz = ((float)jz + 0.5f) * sc.h[2] - p[2];
-            for (int jy = lo[1]; jy <= hi[1]; jy++) {
+        const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1, nr = ny * nz;
+        if (nx <= 0 || nr <= 0) continue;
+        const int r0 = (lane * 13) % nr;
+        int iy = r0 % ny, iz = r0 / ny;
+        if (nx <= XW) {
+            float dx2[XW], ex[XW];
+#pragma unroll
+            for (int i = 0; i < XW; i++) {
+                const float dx = ((float)(lo[0] + i) + 0.5f) * sc.h[0] - p[0];
+                dx2[i] = dx * dx;
+                ex[i] = __expf(dx2[i] * k) * sc.inv_n;
+            }
+            for (int r = 0; r < nr; r++) {
+                const int jy = lo[1] + iy, jz = lo[2] + iz;
+                const float dz = ((float)jz + 0.5f) * sc.h[2] - p[2];
                 const float dy = ((float)jy + 0.5f) * sc.h[1] - p[1];
                 const float r2yz = dy * dy + dz * dz;
-                if (r2yz > R2) continue;
-                for (int jx = lo[0]; jx <= hi[0]; jx++) {
-                    const float dx = ((float)jx + 0.5f) * sc.h[0] - p[0];
-                    const float r2 = dx * dx + r2yz;
-                    if (r2 > R2) continue;
-                    atomicAdd(&acc[jx + sc.res[0] * (jy + sc.res[1] * jz)], (double)(__expf(r2 * k) * sc.inv_n));
+                if (r2yz <= R2) {
+                    const float eyz = __expf(r2yz * k);
+                    double* row = acc + lo[0] + sy * jy + sz * jz;
+#pragma unroll
+                    for (int i = 0; i < XW; i++)
+                        if (i < nx && dx2[i] + r2yz <= R2) atomicAdd(row + i, (double)(eyz * ex[i]));
+                }
+                if (++iy == ny) {
+                    iy = 0;
+                    if (++iz == nz) iz = 0;
+                }
+            }
+        } else {
+            for (int r = 0; r < nr; r++) {
+                const int jy = lo[1] + iy, jz = lo[2] + iz;
+                const float dz = ((float)jz + 0.5f) * sc.h[2] - p[2];
+                const float dy = ((float)jy + 0.5f) * sc.h[1] - p[1];
+                const float r2yz = dy * dy + dz * dz;
+                if (r2yz <= R2) {
+                    const float eyz = __expf(r2yz * k);
+                    double* row = acc + sy * jy + sz * jz;
+                    for (int jx = lo[0]; jx <= hi[0]; jx++) {
+                        const float dx = ((float)jx + 0.5f) * sc.h[0] - p[0];
+                        const float dx2 = dx * dx;
+                        if (dx2 + r2yz <= R2) atomicAdd(row + jx, (double)(eyz * (__expf(dx2 * k) * sc.inv_n)));
+                    }
+                }
+                if (++iy == ny) {
+                    iy = 0;
+                    if (++iz == nz) iz = 0;
                 }
             }
         }
     }
     __syncthreads();
     double* out = partial + (size_t)blockIdx.x * M;
-    for (int j = threadIdx.x; j < M; j += 256) out[j] = acc[j];
+    for (int j = threadIdx.x; j < M; j += SPLAT_THREADS) {
+        const int x = j % sc.res[0], y = (j / sc.res[0]) % sc.res[1], z = j / (sc.res[0] * sc.res[1]);
+        out[j] = acc[x + sy * y + sz * z];
+    }
 }
+// rho[j] = sum over the block grids: 32 cells per block, 8 groups of threads each summing
+// every 8th block grid, the 8 group sums added in group order (a fixed order)
 __global__ void __launch_bounds__(256)
 k_splat_reduce(const double* __restrict__ partial, int nblocks, int M, double* __restrict__ rho) {
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= M) return;
+    __shared__ double part[8][32];
+    const int c = threadIdx.x & 31, q = threadIdx.x >> 5, j = blockIdx.x * 32 + c;
     double s = 0.0;
-    for (int b = 0; b < nblocks; b++) s += partial[(size_t)b * M + j];
-    rho[j] = s;
+    if (j < M)
+        for (int b = q; b < nblocks; b += 8) s += partial[(size_t)b * M + j];
+    part[q][c] = s;
+    __syncthreads();
+    if (q == 0 && j < M) {
+        double t = part[0][c];
+#pragma unroll
+        for (int g = 1; g < 8; g++) t += part[g][c];
+        rho[j] = t;
+    }
 }
 
 // resid = 2/M (rho - target) (the loss cotangent of rho); per-block FP64 sums of (rho - target)^2
@@ -737,7 +809,10 @@ k_splat_loss(const double* __restrict__ rho, const float* __restrict__ target, i
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-// pbar = d loss / d p = sum_y resid(y) inv_n G(y - p) (y - p) / sigma^2 over the splat support
+// pbar = d loss / d p = sum_y resid(y) inv_n G(y - p) (y - p) / sigma^2 over the splat support,
+// with the Gaussian factored over the axes as in k_splat_lds: per (y, z) row,
+// sum_x resid G_x dx and sum_x resid G_x, then times G_yz (and dy, dz).
+template <int XW>
 __global__ void __launch_bounds__(256)
 k_splat_adj(const float* __restrict__ parts, long long nslots, SplatCfg sc, const float* __restrict__ resid,
             float* __restrict__ pbar) {
@@ -750,22 +825,43 @@ k_splat_adj(const float* __restrict__ parts, long long nslots, SplatCfg sc, cons
 #pragma unroll
         for (int a = 0; a < 3; a++) splat_range(p[a], sc.h[a], sc.R, sc.res[a], lo[a], hi[a]);
         const float R2 = sc.R * sc.R, k = -0.5f / (sc.sigma * sc.sigma), is2 = 1.0f / (sc.sigma * sc.sigma);
+        const int nx = hi[0] - lo[0] + 1;
+        float dxv[XW], ex[XW];
+#pragma unroll
+        for (int i = 0; i < XW; i++) {
+            dxv[i] = ((float)(lo[0] + i) + 0.5f) * sc.h[0] - p[0];
+            ex[i] = __expf(dxv[i] * dxv[i] * k) * (sc.inv_n * is2);
+        }
         for (int jz = lo[2]; jz <= hi[2]; jz++) {
             const float dz = ((float)jz + 0.5f) * sc.h[2] - p[2];
             for (int jy = lo[1]; jy <= hi[1]; jy++) {
                 const float dy = ((float)jy + 0.5f) * sc.h[1] - p[1];
                 const float r2yz = dy * dy + dz * dz;
                 if (r2yz > R2) continue;
-                for (int jx = lo[0]; jx <= hi[0]; jx++) {
-                    const float dx = ((float)jx + 0.5f) * sc.h[0] - p[0];
-                    const float r2 = dx * dx + r2yz;
-                    if (r2 > R2) continue;
-                    const size_t j = (size_t)jx + (size_t)sc.res[0] * ((size_t)jy + (size_t)sc.res[1] * jz);
-                    const float f = resid[j] * sc.inv_n * __expf(r2 * k) * is2;
-                    g[0] += f * dx;
-                    g[1] += f * dy;
-                    g[2] += f * dz;
+                const float* row = resid + (size_t)sc.res[0] * ((size_t)jy + (size_t)sc.res[1] * jz);
+                float sx = 0.0f, s1 = 0.0f;
+                if (nx <= XW) {
+#pragma unroll
+                    for (int i = 0; i < XW; i++)
+                        if (i < nx && dxv[i] * dxv[i] + r2yz <= R2) {
+                            const float w = row[lo[0] + i] * ex[i];
+                            sx += w * dxv[i];
+                            s1 += w;
+                        }
+                } else {
+                    for (int jx = lo[0]; jx <= hi[0]; jx++) {
+                        const float dx = ((float)jx + 0.5f) * sc.h[0] - p[0];
+                        if (dx * dx + r2yz <= R2) {
+                            const float w = row[jx] * (__expf(dx * dx * k) * (sc.inv_n * is2));
+                            sx += w * dx;
+                            s1 += w;
+                        }
+                    }
                 }
+                const float eyz = __expf(r2yz * k);
+                g[0] += eyz * sx;
+                g[1] += eyz * s1 * dy;
+                g[2] += eyz * s1 * dz;
             }
         }
     }
@@ -860,20 +956,39 @@ int launch_zero_slots(float* pbar, long long nslots, long long first, long long 
     if (n <= 0 || nslots <= 0) return 0;
     ISF_LAUNCH(k_zero_slots, grid1d((size_t)n, 256), dim3(256), s, pbar, nslots, first, n);
 }
-constexpr int SPLAT_BLOCKS = 256;
+// register width of the x support: a slot's support holds floor(2R / h) + 1 cell centres
+// in x (one more only through rounding at its ends, which the kernels send to their
+// per-cell path)
+static int splat_x_width(const SplatCfg& sc) {
+    const double w = std::floor(2.0 * (double)sc.R / (double)sc.h[0]) + 1.0;
+    return w <= 9.0 ? 9 : 17;
+}
+// 0 when the padded grid does not fit in LDS or the support is wider than 17 (global atomics)
+static int splat_xw(const SplatCfg& sc) {
+    const long long mp = (long long)splat_sz(sc) * sc.res[2];
+    if (mp > SPLAT_LDS_DOUBLES) return 0;
+    return std::floor(2.0 * (double)sc.R / (double)sc.h[0]) + 1.0 <= 17.0 ? splat_x_width(sc) : 0;
+}
 size_t splat_partial_doubles(const SplatCfg& sc) {
     const size_t M = (size_t)sc.res[0] * sc.res[1] * sc.res[2];
-    return M <= (size_t)SPLAT_LDS ? M * SPLAT_BLOCKS : 0;
+    return splat_xw(sc) ? M * SPLAT_BLOCKS : 0;
 }
 // rho must hold zeros when partial is null (global atomics); otherwise it is overwritten
 int launch_splat(const float* parts, long long nslots, const SplatCfg& sc, double* rho, double* partial,
                  hipStream_t s) {
     const int M = sc.res[0] * sc.res[1] * sc.res[2];
-    if (partial && M <= SPLAT_LDS) {
-        hipLaunchKernelGGL(k_splat_lds, dim3(SPLAT_BLOCKS), dim3(256), 0, s, parts, nslots, sc, partial);
+    const int xw = splat_xw(sc);
+    if (partial && xw) {
+        const size_t lds = sizeof(double) * (size_t)splat_sz(sc) * sc.res[2];
+        if (xw == 9)
+            hipLaunchKernelGGL(k_splat_lds<9>, dim3(SPLAT_BLOCKS), dim3(SPLAT_THREADS), lds, s, parts, nslots, sc,
+                               partial);
+        else
+            hipLaunchKernelGGL(k_splat_lds<17>, dim3(SPLAT_BLOCKS), dim3(SPLAT_THREADS), lds, s, parts, nslots, sc,
+                               partial);
         int e = (int)hipGetLastError();
         if (e) return e;
-        ISF_LAUNCH(k_splat_reduce, grid1d((size_t)M, 256), dim3(256), s, partial, SPLAT_BLOCKS, M, rho);
+        ISF_LAUNCH(k_splat_reduce, dim3((M + 31) / 32), dim3(256), s, partial, SPLAT_BLOCKS, M, rho);
     }
     if (nslots <= 0) return 0;
     ISF_LAUNCH(k_splat, grid1d((size_t)nslots, 256), dim3(256), s, parts, nslots, sc, rho);
@@ -889,7 +1004,9 @@ int launch_splat_loss(const double* rho, const float* target, int M, float* resi
 int launch_splat_adj(const float* parts, long long nslots, const SplatCfg& sc, const float* resid, float* pbar,
                      hipStream_t s) {
     if (nslots <= 0) return 0;
-    ISF_LAUNCH(k_splat_adj, grid1d((size_t)nslots, 256), dim3(256), s, parts, nslots, sc, resid, pbar);
+    if (splat_x_width(sc) == 9)
+        ISF_LAUNCH(k_splat_adj<9>, grid1d((size_t)nslots, 256), dim3(256), s, parts, nslots, sc, resid, pbar);
+    ISF_LAUNCH(k_splat_adj<17>, grid1d((size_t)nslots, 256), dim3(256), s, parts, nslots, sc, resid, pbar);
 }
 
 }  // namespace isf

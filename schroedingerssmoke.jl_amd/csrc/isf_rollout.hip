@@ -6,10 +6,12 @@
 //
 // Forward: the step is run operator by operator (the same kernels and arithmetic as
 // isf_step, with the fused normalize and restrict done as separate passes so that their
-// inputs can be kept) and psi / particles are checkpointed every K steps.  Backward:
-// segments are processed last to first; each is recomputed from its checkpoint keeping
-// the per-step tape (step input psi0, flow output psi1, first gauge output psi3, the two
-// gauge phases q1 q2, particles after the spawn) and then reversed step by step.  The
+// inputs can be kept) and psi / particles are checkpointed every K steps; each step's
+// particle part (spawn, velocity, advection) runs on a second stream beside the next
+// step's psi passes.  Backward: segments are processed last to first; each is recomputed
+// from its checkpoint keeping the per-step tape (step input psi0, flow output psi1, first
+// gauge output psi3, the two gauge phases q1 q2, particles after the spawn) and then
+// reversed step by step.  The
 // recompute of segment s-1 runs on a second stream, with its own scratch and the other of
 // two tape sets, while segment s is reversed on the handle's stream.  The particle part
 // of the reverse (velocity, advection adjoint, spawn adjoint) only feeds each step's
@@ -69,15 +71,19 @@ struct isf_rollout {
     // velocity, particles); rec_done[i] / adj_done[i]: tape set i recomputed / reversed
     Work rw;
     hipEvent_t rec_done[2], adj_done[2], start;
-    // particle stream of the backward (the handle's side stream, idle during a rollout: a
-    // stream of its own would be the handle's fourth and share a hardware queue with the
-    // recompute stream under the runtime's default of four queues per process, which
-    // serialises the two); vbar[i][k] = velocity cotangent of step k of the segment in tape
+    // particle stream of the forward and the backward (the handle's side stream, idle
+    // otherwise during a rollout: a stream of its own would be the handle's fourth and
+    // share a hardware queue with the recompute stream under the runtime's default of four
+    // queues per process, which serialises the two); vbar[i][k] = velocity cotangent of step k of the segment in tape
     // set i (ready at pev[i][k]); pdone[i]: that segment's particle part done
     hipStream_t pst;
     std::vector<float*> vbar[2];
     std::vector<hipEvent_t> pev[2];
     hipEvent_t pdone[2];
+    // forward: the particle part of each step runs on pst after that step's psi (fstep);
+    // fseg[i]: the particle part of the last forward segment in tape set i done (its
+    // velocities have read the set's step outputs)
+    hipEvent_t fstep, fseg[2];
     // work
     float4* psibar;
     float *f, *fbar, *vel, *parts, *parts_final, *pbar, *phibar, *target, *resid;
@@ -125,7 +131,9 @@ static void rollout_free(isf_rollout_t r) {
         for (hipEvent_t e : r->pev[i])
             if (e) (void)hipEventDestroy(e);
         if (r->pdone[i]) (void)hipEventDestroy(r->pdone[i]);
+        if (r->fseg[i]) (void)hipEventDestroy(r->fseg[i]);
     }
+    if (r->fstep) (void)hipEventDestroy(r->fstep);
     if (r->rw.st) (void)hipStreamDestroy(r->rw.st);
     for (void* p : r->allocs) (void)hipFree(p);
     r->magic = 0;
@@ -244,26 +252,26 @@ static int spawn_step(isf_rollout_t r, int it, long long first0, long long nsp, 
 // Steps of segment s from its checkpoint into tape tp, on w's stream and scratch.  The
 // forward (checkpoint = true) also writes the next checkpoint; a recompute writes only
 // the tape (its last step output is the next psi checkpoint again, rewritten with the
-// same values).
-static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bool checkpoint) {
+// same values).  With side, the particle part (spawn, velocity, advection, particle
+// checkpoint) runs on that stream after each step's psi (event fstep), beside the next
+// step's psi passes; the caller orders side after the set-up and waits for fseg at the end.
+static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bool checkpoint,
+                       hipStream_t side = nullptr) {
     isf_t h = r->h;
     hipStream_t st = w.st;
+    hipStream_t pt = side ? side : st;
     const size_t pb = sizeof(float) * 3 * (size_t)r->P;
     const int ks = seg_steps(r, s);
+    // the step outputs of tape set s % 2 are read by the velocities of segment s - 2
+    if (side && s >= 2) HIP_CHECK(hipStreamWaitEvent(st, r->fseg[s % 2], 0), "hipStreamWaitEvent");
     // particles: tape entry p0[k] holds step k's state after its spawn; the advection of
     // step k writes p0[k + 1] (the segment's last step: w.parts), so no step copies them
     if (r->P > 0)
-        HIP_CHECK(hipMemcpyAsync(tp.p0[0], r->ck_parts[s], pb, hipMemcpyDeviceToDevice, st), "particles checkpoint");
+        HIP_CHECK(hipMemcpyAsync(tp.p0[0], r->ck_parts[s], pb, hipMemcpyDeviceToDevice, pt), "particles checkpoint");
     for (int k = 0; k < ks; k++) {
         const int it = s * r->K + k + 1;
         const float4* in = psi0_of(r, tp, s, k);
         float4* out = psi0_of(r, tp, s, k + 1);
-        long long first0, nsp;
-        spawn_slots(r, it, &first0, &nsp);
-        if (nsp > 0) {
-            int e2 = spawn_step(r, it, first0, nsp, tp.p0[k], st);
-            if (e2) return e2;
-        }
         LAUNCH_CHECK(launch_flow_x_fwd(r->g, in, w.S, h->tb, st), "flow x forward");
         LAUNCH_CHECK(launch_flow_plane(r->g, w.S, h->tb, st), "flow plane");
         // psi1 for the tape, normalize_psi of it into psi3 (projected in place next)
@@ -273,14 +281,25 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
         int e = project_q(r, tp.psi3[k], tp.q1[k], w, out, &j);  // out = restrict_velocity!(psi3)
         if (e) return e;
         if ((e = project_q(r, out, tp.q2[k], w))) return e;
-        LAUNCH_CHECK(launch_velocity(r->g, out, h->hbar, 1, w.vel, st), "velocity_one_form");
+        if (side) {
+            HIP_CHECK(hipEventRecord(r->fstep, st), "hipEventRecord");
+            HIP_CHECK(hipStreamWaitEvent(side, r->fstep, 0), "hipStreamWaitEvent");
+        }
+        long long first0, nsp;
+        spawn_slots(r, it, &first0, &nsp);
+        if (nsp > 0) {
+            int e2 = spawn_step(r, it, first0, nsp, tp.p0[k], pt);
+            if (e2) return e2;
+        }
+        LAUNCH_CHECK(launch_velocity(r->g, out, h->hbar, 1, w.vel, pt), "velocity_one_form");
         if (r->P > 0)
-            LAUNCH_CHECK(launch_advect(r->g, k + 1 < ks ? tp.p0[k + 1] : w.parts, r->P, nullptr, w.vel, h->dt, st,
-                                       Batch{}, 0, tp.p0[k]),
+            LAUNCH_CHECK(launch_advect(r->g, k + 1 < ks ? tp.p0[k + 1] : w.parts, r->P, nullptr, w.vel, h->dt, pt,
+                                       Batch{}, 0, tp.p0[k], side != nullptr),
                          "staggered_advect");
     }
     if (checkpoint && s + 1 < r->nseg && r->P > 0)
-        HIP_CHECK(hipMemcpyAsync(r->ck_parts[s + 1], w.parts, pb, hipMemcpyDeviceToDevice, st), "particles checkpoint");
+        HIP_CHECK(hipMemcpyAsync(r->ck_parts[s + 1], w.parts, pb, hipMemcpyDeviceToDevice, pt), "particles checkpoint");
+    if (side) HIP_CHECK(hipEventRecord(r->fseg[s % 2], side), "hipEventRecord");
     return ISF_OK;
 }
 
@@ -471,7 +490,9 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
             for (int k = 0; k < r->K && he == hipSuccess; k++)
                 he = hipEventCreateWithFlags(&r->pev[i][k], hipEventDisableTiming);
             if (he == hipSuccess) he = hipEventCreateWithFlags(&r->pdone[i], hipEventDisableTiming);
+            if (he == hipSuccess) he = hipEventCreateWithFlags(&r->fseg[i], hipEventDisableTiming);
         }
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&r->fstep, hipEventDisableTiming);
         if (he != hipSuccess) return bail(hip_fail(he, "rollout streams"));
     }
     hipStream_t st = h->stream;
@@ -547,8 +568,12 @@ int isf_rollout_forward(isf_rollout_t r, const float velocity[3], const float* t
         LAUNCH_CHECK(launch_restrict_to(r->g, r->cpsi[0], r->cpsi[1], j0, st), "restrict_velocity");
     for (int j = 0; j < r->C; j++)
         if ((e = project_q(r, r->cpsi[j + 1], r->cq[j], mw, j + 1 < r->C ? r->cpsi[j + 2] : nullptr, &j0))) return e;
+    // the particle part on the particle stream, after the set-up above
+    HIP_CHECK(hipEventRecord(r->start, st), "hipEventRecord");
+    HIP_CHECK(hipStreamWaitEvent(r->pst, r->start, 0), "hipStreamWaitEvent");
     for (int s = 0; s < r->nseg; s++)
-        if ((e = seg_forward(r, s, r->tape[s % 2], mw, true))) return e;
+        if ((e = seg_forward(r, s, r->tape[s % 2], mw, true, r->pst))) return e;
+    HIP_CHECK(hipStreamWaitEvent(st, r->fseg[(r->nseg - 1) % 2], 0), "hipStreamWaitEvent");
     if (r->P > 0)
         HIP_CHECK(hipMemcpyAsync(r->parts_final, r->parts, sizeof(float) * 3 * r->P, hipMemcpyDeviceToDevice, st),
                   "copy");

@@ -288,6 +288,37 @@ def test_rollout_gradients_match_autograd():
     assert np.linalg.norm(g["theta"]) < 1e-3 * scale
 
 
+@pytest.mark.parametrize("sres,nslots", [((32, 16, 16), 1500), ((48, 24, 24), 600)])
+def test_splat_paths_match_autograd(sres, nslots):
+    """The splat loss and its particle gradient on finer splat grids than the other cases:
+    (32, 16, 16) (h = 0.125: a support 17 cells wide, the kernels' wide register path) and
+    (48, 24, 24) (a grid too large for the per-block LDS grids, so global atomics, and a
+    support 25 cells wide, the adjoint's per-cell path), against the FP64 restatement."""
+    from isf_amd import ISF
+    from isf_amd.rollout import Rollout, gaussian_target
+    hb, steps, n_init = 0.1, 2, (2 * nslots) // 3
+    v = np.array([1.0, 0.05, -0.03], np.float32)
+    isf = ISF(BOX, DIMS, hb, 1 / 48)
+    isf.jet_configure(v / np.float32(hb), np.float32(np.sum(v ** np.float32(2)) / (np.float32(2) * np.float32(hb))),
+                      np.array(CEN, np.float32), LEN, RAD, 50, 7, 0)
+    tgt = gaussian_target(BOX, sres, 3, centre_lo=(0.2, 0.6, 0.6), centre_hi=(0.8, 1.4, 1.4))
+    r = Rollout(isf, steps, checkpoint_every=2, constraint_iters=4, phase_res=(8, 8, 8), splat_res=sres,
+                splat_sigma=0.25, splat_cutoff=4.0, nslots=nslots, n_init=n_init, fill_seed=9)
+    r.set_target(tgt)
+    tr = TorchRollout(DIMS, BOX, hb, 1 / 48, CEN, LEN, RAD, 50, 7, 0, nslots, n_init, 9, steps, 4, (8, 8, 8),
+                      sres, 0.25, 4.0, tgt)
+    pi = smooth_psi(DIMS, 2).astype(np.float32)
+    loss = r.forward(v, psi_init=pi)
+    g = r.backward()
+    L, gt = tr.grad(v.astype(np.float64), None, psi_init=pi.astype(np.float64), parts_init=tr.parts0.numpy())
+    e_parts = rel(g["parts_init"], gt["parts_init"])
+    print(f"  splat {sres}: loss gpu {loss:.8e} FP64 {L:.8e}; grad parts_init rel {e_parts:.2e}")
+    assert abs(loss - L) < 1e-4 * L
+    assert e_parts < 2e-3
+    r.close()
+    isf.close()
+
+
 def test_checkpointed_equals_full_tape():
     isf, r4, tr, v = rollout_case(steps=6, K=6)
     _, r2, _, _ = rollout_case(steps=6, K=2)
