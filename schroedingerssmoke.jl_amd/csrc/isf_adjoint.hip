@@ -432,35 +432,96 @@ struct AdjAcc {
     }
 };
 
+// Cross-lane moves inside 16-lane rows and across rows by DPP (no LDS round trip):
+// row_shr:n (ctrl 0x110 + n) reads lane - n of the same row, 0 where that is outside the
+// row; row_bcast:15 (0x142, rows 1 and 3) reads the last lane of the previous row,
+// row_bcast:31 (0x143, rows 2 and 3) reads lane 31.  Rows a broadcast does not write keep
+// the given 0.
+template <int CTRL>
+__device__ __forceinline__ float dpp_shr(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_bcast(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWS, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_shr_i(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_bcast_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false);
+}
+
 // The 12 (cell, component) targets of one staggered evaluation are fixed by its base cell
-// e.i (the others are its +1 neighbours, periodic).  Lanes of a wavefront that share the
-// base cell in a contiguous run of lanes sum their 12 contributions by a segmented
-// inclusive scan (offsets 1..32); the last lane of each run adds them to the workgroup's
-// accumulator.  Lanes in any order give the right sum; cell-sorted slots give long runs.
-// Every lane of the wavefront calls this (act = false: nothing to add).
+// e.i (the others are its +1 neighbours, periodic).  The wavefront's lanes are regrouped by
+// base cell: a uniform loop over the distinct cells of the wavefront (a few for
+// cell-sorted slots) gives each lane its rank in the cell-major order, and ds_permute
+// moves the key and the 12 values there.  Lanes in any order are grouped, which matters
+// for the later RK stages, whose points have crossed cell faces independently of the
+// slot order.  A segmented inclusive scan by DPP (row shifts 1, 2, 4, 8, then the row
+// broadcasts) sums each group into its last lane, which adds the 12 sums to the
+// workgroup's accumulator.  Every lane of the wavefront calls this (act = false: nothing
+// to add); all 64 lanes must be active.
 __device__ __forceinline__ void wave_scatter12(const AdjAcc& acc, const StagEval& e, const GridInfo& g, float val[12],
                                                bool act) {
     const int lane = (int)(threadIdx.x & 63u);
-    const unsigned key = act ? (unsigned)cidx(e.i[0], e.i[1], e.i[2], g) : 0xFFFFFFFFu;
+    constexpr unsigned kNone = 0xFFFFFFFFu;  // no cell (N < 2^32 - 1)
+    unsigned key = act ? (unsigned)cidx(e.i[0], e.i[1], e.i[2], g) : kNone;
+    unsigned long long rem = __ballot(true);
+    int dst = 0, base = 0;
+    while (rem) {
+        const unsigned kl = (unsigned)__builtin_amdgcn_readlane((int)key, __builtin_ctzll(rem));
+        const bool mine = key == kl;
+        const unsigned long long m = __ballot(mine);
+        const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (mine) dst = base + below;
+        base += __popcll(m);
+        rem &= ~m;
+    }
+    const int addr = dst * 4;
+    key = (unsigned)__builtin_amdgcn_ds_permute(addr, (int)key);
+#pragma unroll
+    for (int j = 0; j < 12; j++)
+        val[j] = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(addr, __builtin_bit_cast(int, val[j])));
+    // first lane of each group, propagated by a max-scan
     const unsigned kprev = __shfl_up(key, 1u, 64);
     int head = (lane == 0 || kprev != key) ? lane : 0;
+    head = max(head, dpp_shr_i<0x111>(head));
+    head = max(head, dpp_shr_i<0x112>(head));
+    head = max(head, dpp_shr_i<0x114>(head));
+    head = max(head, dpp_shr_i<0x118>(head));
+    head = max(head, dpp_bcast_i<0x142, 0xA>(head));
+    head = max(head, dpp_bcast_i<0x143, 0xC>(head));
+    // 1 where the lane read by each step lies in the lane's group
+    const float f1 = lane - 1 >= head ? 1.0f : 0.0f, f2 = lane - 2 >= head ? 1.0f : 0.0f;
+    const float f4 = lane - 4 >= head ? 1.0f : 0.0f, f8 = lane - 8 >= head ? 1.0f : 0.0f;
+    const float fb15 = (((lane >> 4) & 1) && head < (lane & ~15)) ? 1.0f : 0.0f;
+    const float fb31 = (lane >= 32 && head < 32) ? 1.0f : 0.0f;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int o = __shfl_up(head, (unsigned)d, 64);
-        if (lane >= d) head = max(head, o);
-    }
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const bool take = lane - d >= head;
-#pragma unroll
-        for (int j = 0; j < 12; j++) {
-            const float o = __shfl_up(val[j], (unsigned)d, 64);
-            if (take) val[j] += o;
-        }
+    for (int j = 0; j < 12; j++) {
+        float v = val[j];
+        v = fmaf(dpp_shr<0x111>(v), f1, v);
+        v = fmaf(dpp_shr<0x112>(v), f2, v);
+        v = fmaf(dpp_shr<0x114>(v), f4, v);
+        v = fmaf(dpp_shr<0x118>(v), f8, v);
+        v = fmaf(dpp_bcast<0x142, 0xA>(v), fb15, v);
+        v = fmaf(dpp_bcast<0x143, 0xC>(v), fb31, v);
+        val[j] = v;
     }
     const unsigned knext = __shfl_down(key, 1u, 64);
-    if (!act || (lane != 63 && knext == key)) return;
-    const int *i = e.i, *ip = e.ip;
+    if (key == kNone || (lane != 63 && knext == key)) return;
+    // the group's base cell from its key
+    int i[3], ip[3];
+    const unsigned sxy = (unsigned)g.nx * (unsigned)g.ny;
+    i[2] = (int)(key / sxy);
+    const unsigned r = key - (unsigned)i[2] * sxy;
+    i[1] = (int)(r / (unsigned)g.nx);
+    i[0] = (int)(r - (unsigned)i[1] * (unsigned)g.nx);
+    ip[0] = i[0] + 1 == g.nx ? 0 : i[0] + 1;
+    ip[1] = i[1] + 1 == g.ny ? 0 : i[1] + 1;
+    ip[2] = i[2] + 1 == g.nz ? 0 : i[2] + 1;
     // (pp, pp2, vn) cells of each component after the base cell, as (x, y, z)
     const int cv[3][3][3] = {{{i[0], ip[1], i[2]}, {i[0], i[1], ip[2]}, {i[0], ip[1], ip[2]}},
                              {{ip[0], i[1], i[2]}, {i[0], i[1], ip[2]}, {ip[0], i[1], ip[2]}},
