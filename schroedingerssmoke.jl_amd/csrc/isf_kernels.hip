@@ -878,20 +878,24 @@ template <int NX, int TY> struct DivR2cCfg {
 
 // pressure_project!'s f = div(velocity_one_form(psi, 1)) (src/parallel.jl:100-113,131-143,
 // 156-157) fused with the real-to-complex x transform of k_r2c_x, so f never goes to
-// memory.  A block computes f for TY x lines at one z: psi(c + e_x) from the neighbouring
-// lane, psi(c + e_y), psi(c +- e_z) and the row y0 - 1 through the caches.  The edge angles
-// and their differences are those of k_div_psi (same operands).  Tiles are mapped
-// XCD-aware like the stencils, so the z +- 1 rows a block reads are its neighbours' own
-// rows, in the same L2.
+// memory.  A block computes f for TY x lines: TY/2 rows y0 .. y0+TY/2-1 at z0 and the same
+// rows at z0 + 1 (z0 even).  A thread takes both cells of a (y, x) column, so the z edge
+// between them, v_z at z0, is evaluated once for both (3.5 instead of 4 edge angles per
+// cell), and the block reads one z halo row per column on each side.  psi(c + e_x) comes
+// from the neighbouring lane, psi(c + e_y), the z halo rows and the row below each z plane
+// of the tile through the caches.  The edge angles and their differences are those of
+// k_div_psi (same operands).  Tiles are mapped XCD-aware like the stencils, so the halo
+// rows a block reads are its neighbours' own rows, in the same L2.
 template <int NX, int TY>
 __global__ void __launch_bounds__((DivR2cCfg<NX, TY>::NT))
 k_div_r2c(const float4* __restrict__ psi, float2* __restrict__ P, GridInfo g, const float2* __restrict__ tw,
           Batch B) {
     using C = DivR2cCfg<NX, TY>;
     constexpr int M = NX / 2, NT = C::NT, CELLS = TY * NX;
+    constexpr int HT = TY / 2, COLS = HT * NX;  // rows per z plane, (row, x) columns
     constexpr int T = XCfg<M>::T, E = XCfg<M>::E, LS = XCfg<M>::LS;
-    static_assert(CELLS % NT == 0 && NT % 64 == 0, "div r2c tiling");
-    const StTile tl = stencil_tile();  // x: y tile, y: z, z: member
+    static_assert(CELLS % NT == 0 && COLS % NT == 0 && NT % 64 == 0, "div r2c tiling");
+    const StTile tl = stencil_tile();  // x: y tile, y: z pair, z: member
     if (B.inst) {
         const Inst& m = B.inst[tl.bz];
         psi = m.psi;
@@ -900,78 +904,102 @@ k_div_r2c(const float4* __restrict__ psi, float2* __restrict__ P, GridInfo g, co
         tw = m.tb.tw;
     }
     __shared__ float vx_s[TY][NX];
-    __shared__ float vy_s[TY + 1][NX];  // row l + 1: v_y at y0 + l; row 0: at y0 - 1
+    // z plane h, row ly: v_y at y0 + ly in row h (HT + 1) + ly + 1; at y0 - 1 in row h (HT + 1)
+    __shared__ float vy_s[TY + 2][NX];
     __shared__ float2 lds[TY * LS];     // v_z differences, then f as packed complex lines
     __shared__ float2 twm[FftTw<M, E>::SIZE];
     __shared__ float2 twn[NX];
     float* dz_s = reinterpret_cast<float*>(lds);
     const int tid = threadIdx.x, lane = tid & 63;
     const int ny = g.ny, nz = g.nz;
-    const int y0 = tl.bx * TY, z = tl.by;
+    const int y0 = tl.bx * HT, z0 = 2 * tl.by;
     const size_t sy = NX, sz = (size_t)NX * ny;
-    const size_t zo = sz * z, zpo = sz * (z + 1 < nz ? z + 1 : 0), zmo = sz * (z > 0 ? z - 1 : nz - 1);
+    const size_t zo0 = sz * z0, zo1 = sz * (z0 + 1);
+    const size_t zom = sz * (z0 > 0 ? z0 - 1 : nz - 1), zop = sz * (z0 + 2 < nz ? z0 + 2 : 0);
     FftTw<M, E>::template fill<NT>(twm, tw, tid);
     load_twiddles<NX, NT>(twn, tw);
-    // v_y of the row below the tile
+    // v_y of the row below the tile, in both z planes
     {
         const int ym = y0 > 0 ? y0 - 1 : ny - 1;
-        for (int x = tid; x < NX; x += NT) vy_s[0][x] = edge_angle(psi[sy * ym + zo + x], psi[sy * y0 + zo + x]);
+        for (int i = tid; i < 2 * NX; i += NT) {
+            const int h = i / NX, x = i % NX;
+            const size_t zo = h ? zo1 : zo0;
+            vy_s[h * (HT + 1)][x] = edge_angle(psi[sy * ym + zo + x], psi[sy * y0 + zo + x]);
+        }
     }
-    // edge angles of the tile's cells, G cells per thread at a time with their loads issued
-    // together; a wavefront holds consecutive x of one row
-    constexpr int K = CELLS / NT, G = K < 4 ? K : 4;
+    // edge angles of the tile's columns, G columns per thread at a time with their loads
+    // issued together; a wavefront holds consecutive x of one row
+    constexpr int K = COLS / NT, G = K < 2 ? K : 2;
 #pragma unroll 1
     for (int k0 = 0; k0 < K; k0 += G) {
-        float4 pc[G], py[G], pz[G], pzm[G], pxh[G];
+        float4 p0[G], p1[G], pm[G], pp[G], py0[G], py1[G], ph0[G], ph1[G];
 #pragma unroll
         for (int q = 0; q < G; q++) {
-            const int i = tid + (k0 + q) * NT, l = i / NX, x = i % NX, y = y0 + l;
-            const size_t row = sy * y;
-            pc[q] = psi[row + zo + x];
-            py[q] = psi[sy * (y + 1 < ny ? y + 1 : 0) + zo + x];
-            pz[q] = psi[row + zpo + x];
-            pzm[q] = psi[row + zmo + x];
-            pxh[q] = pc[q];
-            if (lane == 63 || x + 1 == NX) pxh[q] = psi[row + zo + (x + 1 < NX ? x + 1 : 0)];
+            const int j = tid + (k0 + q) * NT, ly = j / NX, x = j % NX, y = y0 + ly;
+            const size_t row = sy * y, rowy = sy * (y + 1 < ny ? y + 1 : 0);
+            p0[q] = psi[row + zo0 + x];
+            p1[q] = psi[row + zo1 + x];
+            pm[q] = psi[row + zom + x];
+            pp[q] = psi[row + zop + x];
+            py0[q] = psi[rowy + zo0 + x];
+            py1[q] = psi[rowy + zo1 + x];
+            ph0[q] = p0[q];
+            ph1[q] = p1[q];
+            if (lane == 63 || x + 1 == NX) {
+                const int xn = x + 1 < NX ? x + 1 : 0;
+                ph0[q] = psi[row + zo0 + xn];
+                ph1[q] = psi[row + zo1 + xn];
+            }
         }
 #pragma unroll
         for (int q = 0; q < G; q++) {
-            const int i = tid + (k0 + q) * NT, l = i / NX, x = i % NX;
-            float4 px = shfl4(pc[q], lane < 63 ? lane + 1 : lane);
-            if (lane == 63 || x + 1 == NX) px = pxh[q];
-            vx_s[l][x] = edge_angle(pc[q], px);
-            vy_s[l + 1][x] = edge_angle(pc[q], py[q]);
-            dz_s[i] = edge_angle(pc[q], pz[q]) - edge_angle(pzm[q], pc[q]);
+            const int j = tid + (k0 + q) * NT, ly = j / NX, x = j % NX;
+            const int src = lane < 63 ? lane + 1 : lane;
+            float4 px0 = shfl4(p0[q], src), px1 = shfl4(p1[q], src);
+            if (lane == 63 || x + 1 == NX) {
+                px0 = ph0[q];
+                px1 = ph1[q];
+            }
+            vx_s[ly][x] = edge_angle(p0[q], px0);
+            vx_s[HT + ly][x] = edge_angle(p1[q], px1);
+            vy_s[ly + 1][x] = edge_angle(p0[q], py0[q]);
+            vy_s[HT + 2 + ly][x] = edge_angle(p1[q], py1[q]);
+            const float ea = edge_angle(pm[q], p0[q]), eb = edge_angle(p0[q], p1[q]), ec = edge_angle(p1[q], pp[q]);
+            dz_s[j] = eb - ea;
+            dz_s[COLS + j] = ec - eb;
         }
     }
     __syncthreads();
     // f(c) = sum_a (v_a(c) - v_a(c - e_a)) / d_a^2
     const float ds0 = 1.0f / (g.dx * g.dx), ds1 = 1.0f / (g.dy * g.dy), ds2 = 1.0f / (g.dz * g.dz);
-    float fv[K];
+    constexpr int KC = CELLS / NT;
+    float fv[KC];
 #pragma unroll
-    for (int k = 0; k < K; k++) {
+    for (int k = 0; k < KC; k++) {
         const int i = tid + k * NT, l = i / NX, x = i % NX;
         const float vx = vx_s[l][x], vxm = vx_s[l][x > 0 ? x - 1 : NX - 1];
-        const float vy = vy_s[l + 1][x], vym = vy_s[l][x];
+        const int ry = l + l / HT + 1;
+        const float vy = vy_s[ry][x], vym = vy_s[ry - 1][x];
         fv[k] = (vx - vxm) * ds0 + (vy - vym) * ds1 + dz_s[i] * ds2;
     }
     __syncthreads();  // dz_s is read; lds now receives f
 #pragma unroll
-    for (int k = 0; k < K; k++) {
+    for (int k = 0; k < KC; k++) {
         const int i = tid + k * NT, l = i / NX, x = i % NX;
         reinterpret_cast<float*>(lds + l * LS + pad16(x >> 1))[x & 1] = fv[k];
     }
     __syncthreads();
-    // the R2C transform of k_r2c_x
+    // the R2C transform of k_r2c_x; line l is row y0 + l % HT of z plane z0 + l / HT
     fft_line<M, E, -1>(LayContig{lds + (tid / T) * LS}, tid % T, tid < TY * T, twm);
     for (int i = tid; i < (M + 1) * TY; i += NT) {
-        const int ly = i % TY, k = i / TY;
+        const int lh = i % HT, k = (i / HT) % (M + 1), h = i / (HT * (M + 1));
+        const int ly = h * HT + lh;
         const float2 zk = lds[ly * LS + pad16(k % M)];
         const float2 zm = cconj(lds[ly * LS + pad16((M - k) % M)]);
         const float2 e = cscale(cadd(zk, zm), 0.5f);
         const float2 dd = csub(zk, zm);
         const float2 o = make_float2(0.5f * dd.y, -0.5f * dd.x);  // (zk - zm) / (2i)
-        P[((size_t)k * nz + z) * ny + y0 + ly] = cadd(e, cmul(twn[k], o));
+        P[((size_t)k * nz + z0 + h) * ny + y0 + lh] = cadd(e, cmul(twn[k], o));
     }
 }
 
@@ -1344,7 +1372,7 @@ int launch_r2c_x(const GridInfo& g, const float* f, float2* P, const DevTables& 
 }
 template <int NX, int TY>
 static int div_r2c_t(const GridInfo& g, const float4* psi, float2* P, const DevTables& t, hipStream_t s, Batch b) {
-    dim3 grid(g.ny / TY, g.nz, members_of(b));
+    dim3 grid(2 * g.ny / TY, g.nz / 2, members_of(b));  // TY/2 rows in each of two z planes
     ISF_LAUNCH((k_div_r2c<NX, TY>), grid, dim3(DivR2cCfg<NX, TY>::NT), s, psi, P, g, t.tw, b);
 }
 // Fused while a z plane of psi (nx ny 16 bytes) is at most 1 MB: the z +- 1 rows a block
