@@ -212,15 +212,16 @@ def b_alg_fwd_adjoint(N, P, T, K):
     DAG of one simloop step; each array read or written once per fused kernel, the
     Poisson half spectrum counted as 4 B/cell each way; the set-up constraint iterations
     are not counted, their time is):
-      forward step   248 N + 24 P (the step) + 24 N tape (raw flow output, q1, q2)
+      forward step   248 N + 24 P (the step, whose velocity field is written to the tape)
+                     + 24 N tape (raw flow output, q1, q2)
       recompute      the same, for the T - K steps outside the last segment
       reverse step   psi chain 372 N: velocity adjoint 64 (psibar, psi, vbar -> psibar,
                      qbar), two Poisson solves 2 x 24, two one-form/div adjoints 2 x 56,
-                     restrict adjoint 52, flow adjoint 3 x 32; particle chain 52 N + 36 P:
-                     velocity pass 28, vbar written 12, velocity read 12, positions 12 P,
-                     pbar read and written 24 P."""
+                     restrict adjoint 52, flow adjoint 3 x 32; particle chain 24 N + 36 P:
+                     vbar written 12, taped velocity read 12, positions 12 P, pbar read and
+                     written 24 P."""
     fwd = 272 * N + 24 * P
-    rev = 424 * N + 36 * P
+    rev = 396 * N + 36 * P
     return fwd + fwd * (T - K) / T + rev
 
 

@@ -10,8 +10,9 @@
 // particle part (spawn, velocity, advection) runs on a second stream beside the next
 // step's psi passes.  Backward: segments are processed last to first; each is recomputed
 // from its checkpoint keeping the per-step tape (step input psi0, flow output psi1, first
-// gauge output psi3, the two gauge phases q1 q2, particles after the spawn) and then
-// reversed step by step.  The
+// gauge output psi3, the two gauge phases q1 q2, particles after the spawn, and with
+// particles the step's velocity field, which its advection and advection adjoint read) and
+// then reversed step by step.  The
 // recompute of segment s-1 runs on a second stream, with its own scratch and the other of
 // two tape sets, while segment s is reversed on the handle's stream.  The particle part
 // of the reverse (velocity, advection adjoint, spawn adjoint) only feeds each step's
@@ -32,16 +33,16 @@ using namespace isf;
 static constexpr uint32_t kRolloutMagic = 0x2011A0D7u;
 
 // Per-step tape of one segment: step outputs inside the segment (mid), flow output psi1,
-// first gauge output psi3, gauge phases q1 q2, particles after the spawn p0.
+// first gauge output psi3, gauge phases q1 q2, particles after the spawn p0, velocity
+// field of the step output vel (with particles only).
 struct Tape {
     std::vector<float4*> mid, psi1, psi3;
-    std::vector<float*> q1, q2, p0;
+    std::vector<float*> q1, q2, p0, vel;
 };
 // Scratch and stream of one chain of forward steps.
 struct Work {
     float2* S;     // spectrum (16N bytes); f at byte 0 and P at byte 4N in projections
     float* f;      // div scratch of the wide-plane projection path
-    float* vel;    // velocity field
     float* parts;  // particles being stepped
     hipStream_t st;
 };
@@ -68,7 +69,7 @@ struct isf_rollout {
     // segment tapes: set s % 2 holds segment s during the backward
     Tape tape[2];
     // recompute stream of the backward with its own scratch (spectrum / Poisson, f,
-    // velocity, particles); rec_done[i] / adj_done[i]: tape set i recomputed / reversed
+    // particles); rec_done[i] / adj_done[i]: tape set i recomputed / reversed
     Work rw;
     hipEvent_t rec_done[2], adj_done[2], start;
     // particle stream of the forward and the backward (the handle's side stream, idle
@@ -86,7 +87,7 @@ struct isf_rollout {
     hipEvent_t fstep, fseg[2];
     // work
     float4* psibar;
-    float *f, *fbar, *vel, *parts, *parts_final, *pbar, *phibar, *target, *resid;
+    float *f, *fbar, *parts, *parts_final, *pbar, *phibar, *target, *resid;
     // spawn batches of steps 1..T (n_spawn x 3 each, in slot order): they depend only on
     // the nozzle geometry, the seed and the step, so a segment forward copies them in
     // instead of drawing and sorting them again; built for spawn_key
@@ -168,7 +169,7 @@ static float2* P_in(isf_rollout_t r, float2* S) {
 }
 static float2* P_of(isf_rollout_t r) { return P_in(r, r->h->S); }
 // the handle's stream with the handle's scratch and the rollout's work buffers
-static Work main_work(isf_rollout_t r) { return Work{r->h->S, r->f, r->vel, r->parts, r->h->stream}; }
+static Work main_work(isf_rollout_t r) { return Work{r->h->S, r->f, r->parts, r->h->stream}; }
 static double poisson_scale(isf_rollout_t r) { return -0.25 * 2.0 / (double)r->N; }
 
 // fout = Poisson(fin) (src/parallel.jl:168-174)
@@ -291,11 +292,12 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
             int e2 = spawn_step(r, it, first0, nsp, tp.p0[k], pt);
             if (e2) return e2;
         }
-        LAUNCH_CHECK(launch_velocity(r->g, out, h->hbar, 1, w.vel, pt), "velocity_one_form");
-        if (r->P > 0)
-            LAUNCH_CHECK(launch_advect(r->g, k + 1 < ks ? tp.p0[k + 1] : w.parts, r->P, nullptr, w.vel, h->dt, pt,
-                                       Batch{}, 0, tp.p0[k], side != nullptr),
+        if (r->P > 0) {  // the velocity goes to the tape: the advection adjoint reads it again
+            LAUNCH_CHECK(launch_velocity(r->g, out, h->hbar, 1, tp.vel[k], pt), "velocity_one_form");
+            LAUNCH_CHECK(launch_advect(r->g, k + 1 < ks ? tp.p0[k + 1] : w.parts, r->P, nullptr, tp.vel[k], h->dt,
+                                       pt, Batch{}, 0, tp.p0[k], side != nullptr),
                          "staggered_advect");
+        }
     }
     if (checkpoint && s + 1 < r->nseg && r->P > 0)
         HIP_CHECK(hipMemcpyAsync(r->ck_parts[s + 1], w.parts, pb, hipMemcpyDeviceToDevice, pt), "particles checkpoint");
@@ -304,23 +306,21 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
 }
 
 // Particle part of the reverse of step k of segment s (tape tp), on the particle stream:
-// the step's velocity field, the advection adjoint (pbar in place, the velocity cotangent
-// into vb) and the spawn adjoint.
+// the advection adjoint at the step's taped velocity field (pbar in place, the velocity
+// cotangent into vb) and the spawn adjoint.
 static int step_adjoint_particles(isf_rollout_t r, const Tape& tp, int s, int k, float* vb) {
     isf_t h = r->h;
     hipStream_t st = r->pst;
     const GridInfo& g = r->g;
     const int it = s * r->K + k + 1;
-    const float4* psi5 = psi0_of(r, tp, s, k + 1);
     // :59-63 velocity and advection
     LAUNCH_CHECK(launch_fill_f32(vb, 0.0f, 3 * r->N, st), "fill");
     if (r->P > 0) {
-        LAUNCH_CHECK(launch_velocity(g, psi5, h->hbar, 1, r->vel, st), "velocity_one_form");
         // cell-sorted slot order: long same-cell runs for the scatter's wavefront reduction
         LAUNCH_CHECK(launch_cell_order(g, tp.p0[k], r->P, r->adj_order, r->adj_order_scratch,
                                                  r->adj_order_bytes, st),
                      "particle order");
-        LAUNCH_CHECK(launch_advect_adj(g, tp.p0[k], r->P, r->adj_order, r->vel, h->dt, r->pbar, vb, st),
+        LAUNCH_CHECK(launch_advect_adj(g, tp.p0[k], r->P, r->adj_order, tp.vel[k], h->dt, r->pbar, vb, st),
                      "advect adjoint");
         long long first0, nsp;
         spawn_slots(r, it, &first0, &nsp);
@@ -438,22 +438,22 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
         tp.q1.resize(r->K);
         tp.q2.resize(r->K);
         tp.p0.resize(r->K);
+        tp.vel.resize(r->P > 0 ? r->K : 0);
         for (int k = 0; k < r->K; k++) {
             RA(&tp.psi1[k], N);
             RA(&tp.psi3[k], N);
             RA(&tp.q1[k], N);
             RA(&tp.q2[k], N);
             RA(&tp.p0[k], P3);
+            if (r->P > 0) RA(&tp.vel[k], 3 * N);
         }
     }
     RA(&r->rw.S, 2 * N);
     RA(&r->rw.f, N);
-    RA(&r->rw.vel, 3 * N);
     RA(&r->rw.parts, P3);
     RA(&r->psibar, N);
     RA(&r->f, N);
     RA(&r->fbar, N);
-    RA(&r->vel, 3 * N);
     for (int i = 0; i < 2; i++) {
         r->vbar[i].resize(r->K);
         for (auto& p : r->vbar[i]) RA(&p, 3 * N);
