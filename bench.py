@@ -215,13 +215,14 @@ def b_alg_fwd_adjoint(N, P, T, K):
       forward step   248 N + 24 P (the step, whose velocity field is written to the tape)
                      + 24 N tape (raw flow output, q1, q2)
       recompute      the same, for the T - K steps outside the last segment
-      reverse step   psi chain 372 N: velocity adjoint 64 (psibar, psi, vbar -> psibar,
-                     qbar), two Poisson solves 2 x 24, two one-form/div adjoints 2 x 56,
-                     restrict adjoint 52, flow adjoint 3 x 32; particle chain 24 N + 36 P:
+      reverse step   psi chain 324 N: velocity adjoint 64 (psibar, psi, vbar -> psibar,
+                     qbar), two Poisson solves 2 x 24, two one-form/div adjoints 56 + 60 (the
+                     second with the restrict adjoint and its gauge cotangent fused), flow
+                     adjoint 3 x 32; particle chain 24 N + 36 P:
                      vbar written 12, taped velocity read 12, positions 12 P, pbar read and
                      written 24 P."""
     fwd = 272 * N + 24 * P
-    rev = 396 * N + 36 * P
+    rev = 348 * N + 36 * P
     return fwd + fwd * (T - K) / T + rev
 
 

@@ -229,6 +229,44 @@ __device__ __forceinline__ float4 psi_pre(const float4* __restrict__ psi, size_t
     return p;
 }
 
+// The restrict adjoint of one jet cell at its input p (psibar b of the output): the new
+// psibar, and phibar with its products by the cell position in s[0..3].
+__device__ __forceinline__ float4 restrict_adj_cell(float4 b, float4 p, int x, int y, int z, const GridInfo& g,
+                                                    const JetParams& j, double s[4]) {
+    const float4 o = jet_phase(p, x, y, z, g, j);
+    const float a1 = hypotf(p.x, p.y), a2 = hypotf(p.z, p.w);
+    // e^{i phi} recovered from the output (a_k > 0 in practice; guarded)
+    const float e1r = a1 > 0.0f ? o.x / a1 : 0.0f, e1i = a1 > 0.0f ? o.y / a1 : 0.0f;
+    const float e2r = a2 > 0.0f ? o.z / a2 : 0.0f, e2i = a2 > 0.0f ? o.w / a2 : 0.0f;
+    const float r1 = b.x * e1r + b.y * e1i, r2 = b.z * e2r + b.w * e2i;  // Re(conj(e) bar)
+    const float phib = a1 * (b.y * e1r - b.x * e1i) + a2 * (b.w * e2r - b.z * e2i);
+    const float i1 = a1 > 0.0f ? 1.0f / a1 : 0.0f, i2 = a2 > 0.0f ? 1.0f / a2 : 0.0f;
+    const float px = __fmul_rn((float)x, g.dx), py = __fmul_rn((float)y, g.dy), pz = __fmul_rn((float)z, g.dz);
+    s[0] = phib;
+    s[1] = (double)phib * px;
+    s[2] = (double)phib * py;
+    s[3] = (double)phib * pz;
+    return make_float4(r1 * p.x * i1, r1 * p.y * i1, r2 * p.z * i2, r2 * p.w * i2);
+}
+// the gauge cotangent of psibar b against the gauge output p (k_gauge_qbar)
+__device__ __forceinline__ float gauge_qbar(float4 b, float4 p) {
+    return (b.x * p.y - b.y * p.x) + (b.z * p.w - b.w * p.z);
+}
+// the four per-block sums of the restrict adjoint (every thread of the 256 calls this)
+__device__ __forceinline__ void restrict_adj_block_sums(double s[4], double* __restrict__ partials) {
+    s[0] = block_sum<256>(s[0]);
+    s[1] = block_sum<256>(s[1]);
+    s[2] = block_sum<256>(s[2]);
+    s[3] = block_sum<256>(s[3]);
+    if (threadIdx.x == 0) {
+        double* o = partials + 4 * (size_t)blockIdx.x;
+        o[0] = s[0];
+        o[1] = s[1];
+        o[2] = s[2];
+        o[3] = s[3];
+    }
+}
+
 // Adjoint of a one-form evaluated at psi (formed per PRE), fused with the adjoint of what
 // consumed it:
 //   OF_DIV: div then a Poisson solve produced q; w = fbar (Poisson-solved qbar),
@@ -240,15 +278,13 @@ __device__ __forceinline__ float4 psi_pre(const float4* __restrict__ psi, size_t
 // the cotangent of the stored psi (the normalize adjoint applied last).  With qbar_out, also the gauge
 // cotangent of the new psibar against psi (k_gauge_qbar with psi_out = psi), for a gauge
 // transform whose output psi is.
+// The psibar of cell c (x, y, z) before the normalize adjoint: the gauge adjoint and the
+// one-form contributions of the six edges at c; *pc = the psi the one-form saw at c.
 template <int MODE, int PRE>
-__global__ void __launch_bounds__(256)
-k_oneform_adj(float4* __restrict__ psibar, const float4* __restrict__ psi, const float* __restrict__ q,
-              const float* __restrict__ w, GridInfo g, float hbar, int sharp, JetParams j,
-              float* __restrict__ qbar_out) {
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells(g)) return;
-    int x, y, z;
-    cell_xyz(c, g, x, y, z);
+__device__ __forceinline__ float4 oneform_adj_cell(const float4* __restrict__ psibar, const float4* __restrict__ psi,
+                                                   const float* __restrict__ q, const float* __restrict__ w,
+                                                   const GridInfo& g, float hbar, int sharp, const JetParams& j,
+                                                   size_t c, int x, int y, int z, float4* pc_out) {
     const int n3[3] = {g.nx, g.ny, g.nz};
     const float d[3] = {g.dx, g.dy, g.dz};
     float4 acc = psibar[c];
@@ -283,9 +319,46 @@ k_oneform_adj(float4* __restrict__ psibar, const float4* __restrict__ psi, const
         edge_z(pm, pc, re, imz);
         add_mul(acc, angle_zbar(re, imz, wb), pm);       // c is the second end of (c-e_a, c)
     }
-    if (PRE == PRE_NORMALIZE) acc = normalize_adj_at(psi[c], acc);
-    psibar[c] = acc;
-    if (qbar_out) qbar_out[c] = (acc.x * pc.y - acc.y * pc.x) + (acc.z * pc.w - acc.w * pc.z);
+    *pc_out = pc;
+    return acc;
+}
+
+// RADJ (with PRE_RESTRICT): the restrict adjoint at psi follows in the same pass (the
+// arithmetic of k_restrict_adj, with its per-block sums in partials), and qbar_out then
+// takes the gauge cotangent against psi, the restrict input: one pass over psibar instead
+// of two.
+template <int MODE, int PRE, bool RADJ = false>
+__global__ void __launch_bounds__(256)
+k_oneform_adj(float4* __restrict__ psibar, const float4* __restrict__ psi, const float* __restrict__ q,
+              const float* __restrict__ w, GridInfo g, float hbar, int sharp, JetParams j,
+              float* __restrict__ qbar_out, double* __restrict__ partials = nullptr) {
+    static_assert(!RADJ || PRE == PRE_RESTRICT, "the fused restrict adjoint follows a one-form at restrict(psi)");
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if constexpr (RADJ) {
+        // every thread reaches the block sums
+        double sums[4] = {0.0, 0.0, 0.0, 0.0};
+        if (c < ncells(g)) {
+            int x, y, z;
+            cell_xyz(c, g, x, y, z);
+            float4 pc;
+            const float4 acc = oneform_adj_cell<MODE, PRE>(psibar, psi, q, w, g, hbar, sharp, j, c, x, y, z, &pc);
+            const float4 p = psi[c];
+            float4 nb = acc;
+            if (is_jet_cell(x, y, z, g, j)) nb = restrict_adj_cell(acc, p, x, y, z, g, j, sums);
+            psibar[c] = nb;
+            if (qbar_out) qbar_out[c] = gauge_qbar(nb, p);
+        }
+        restrict_adj_block_sums(sums, partials);
+    } else {
+        if (c >= ncells(g)) return;
+        int x, y, z;
+        cell_xyz(c, g, x, y, z);
+        float4 pc;
+        float4 acc = oneform_adj_cell<MODE, PRE>(psibar, psi, q, w, g, hbar, sharp, j, c, x, y, z, &pc);
+        if (PRE == PRE_NORMALIZE) acc = normalize_adj_at(psi[c], acc);
+        psibar[c] = acc;
+        if (qbar_out) qbar_out[c] = (acc.x * pc.y - acc.y * pc.x) + (acc.z * pc.w - acc.w * pc.z);
+    }
 }
 
 // Adjoint of restrict_velocity (jet cells) at its input psi_in, in place on psibar; per
@@ -297,46 +370,20 @@ __global__ void __launch_bounds__(256)
 k_restrict_adj(float4* __restrict__ psibar, const float4* __restrict__ psi_in, GridInfo g, JetParams j,
                double* __restrict__ partials, float* __restrict__ qbar_out) {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    double sums[4] = {0.0, 0.0, 0.0, 0.0};
     if (c < ncells(g)) {
         int x, y, z;
         cell_xyz(c, g, x, y, z);
         const bool jet = is_jet_cell(x, y, z, g, j);
-        if (qbar_out && !jet) {
-            const float4 b = psibar[c], p = psi_in[c];
-            qbar_out[c] = (b.x * p.y - b.y * p.x) + (b.z * p.w - b.w * p.z);
-        }
+        if (qbar_out && !jet) qbar_out[c] = gauge_qbar(psibar[c], psi_in[c]);
         if (jet) {
-            const float4 p = psi_in[c], b = psibar[c];
-            const float4 o = jet_phase(p, x, y, z, g, j);
-            const float a1 = hypotf(p.x, p.y), a2 = hypotf(p.z, p.w);
-            // e^{i phi} recovered from the output (a_k > 0 in practice; guarded)
-            const float e1r = a1 > 0.0f ? o.x / a1 : 0.0f, e1i = a1 > 0.0f ? o.y / a1 : 0.0f;
-            const float e2r = a2 > 0.0f ? o.z / a2 : 0.0f, e2i = a2 > 0.0f ? o.w / a2 : 0.0f;
-            const float r1 = b.x * e1r + b.y * e1i, r2 = b.z * e2r + b.w * e2i;  // Re(conj(e) bar)
-            const float phib = a1 * (b.y * e1r - b.x * e1i) + a2 * (b.w * e2r - b.z * e2i);
-            const float i1 = a1 > 0.0f ? 1.0f / a1 : 0.0f, i2 = a2 > 0.0f ? 1.0f / a2 : 0.0f;
-            const float4 nb = make_float4(r1 * p.x * i1, r1 * p.y * i1, r2 * p.z * i2, r2 * p.w * i2);
+            const float4 p = psi_in[c];
+            const float4 nb = restrict_adj_cell(psibar[c], p, x, y, z, g, j, sums);
             psibar[c] = nb;
-            if (qbar_out) qbar_out[c] = (nb.x * p.y - nb.y * p.x) + (nb.z * p.w - nb.w * p.z);
-            const float px = __fmul_rn((float)x, g.dx), py = __fmul_rn((float)y, g.dy), pz = __fmul_rn((float)z, g.dz);
-            s0 = phib;
-            s1 = (double)phib * px;
-            s2 = (double)phib * py;
-            s3 = (double)phib * pz;
+            if (qbar_out) qbar_out[c] = gauge_qbar(nb, p);
         }
     }
-    s0 = block_sum<256>(s0);
-    s1 = block_sum<256>(s1);
-    s2 = block_sum<256>(s2);
-    s3 = block_sum<256>(s3);
-    if (threadIdx.x == 0) {
-        double* o = partials + 4 * (size_t)blockIdx.x;
-        o[0] = s0;
-        o[1] = s1;
-        o[2] = s2;
-        o[3] = s3;
-    }
+    restrict_adj_block_sums(sums, partials);
 }
 
 // out[k] += sum over blocks of partials[4 b + k], k < 4 (one block, fixed order)
@@ -991,19 +1038,32 @@ size_t restrict_adj_blocks(const GridInfo& g) { return (ncells(g) + 255) / 256; 
 // the first reduction level's blocks (their sums follow the per-block rows in partials)
 constexpr size_t kReduce4Blocks = 128;
 size_t restrict_adj_partials(const GridInfo& g) { return 4 * restrict_adj_blocks(g) + 4 * kReduce4Blocks; }
-int launch_restrict_adj(const GridInfo& g, float4* psibar, const float4* psi_in, const JetParams& j, double* partials,
-                        double* out4, hipStream_t s, float* qbar_out) {
-    hipLaunchKernelGGL(k_restrict_adj, grid1d(ncells(g), 256), dim3(256), 0, s, psibar, psi_in, g, j, partials,
-                       qbar_out);
-    int e = (int)hipGetLastError();
-    if (e) return e;
+// out4 += the four sums of the per-block rows in partials (two levels, fixed order)
+static int restrict_adj_reduce(const GridInfo& g, double* partials, double* out4, hipStream_t s) {
     const size_t nb = restrict_adj_blocks(g);
     const size_t G = nb < kReduce4Blocks ? nb : kReduce4Blocks, chunk = (nb + G - 1) / G;
     double* part2 = partials + 4 * nb;
     hipLaunchKernelGGL(k_reduce4_part, dim3((unsigned)G), dim3(256), 0, s, partials, nb, chunk, part2);
-    e = (int)hipGetLastError();
+    const int e = (int)hipGetLastError();
     if (e) return e;
     ISF_LAUNCH(k_reduce4, dim3(1), dim3(256), s, part2, G, out4);
+}
+int launch_restrict_adj(const GridInfo& g, float4* psibar, const float4* psi_in, const JetParams& j, double* partials,
+                        double* out4, hipStream_t s, float* qbar_out) {
+    hipLaunchKernelGGL(k_restrict_adj, grid1d(ncells(g), 256), dim3(256), 0, s, psibar, psi_in, g, j, partials,
+                       qbar_out);
+    const int e = (int)hipGetLastError();
+    if (e) return e;
+    return restrict_adj_reduce(g, partials, out4, s);
+}
+int launch_oneform_adj_div_restrict(const GridInfo& g, float4* psibar, const float4* psi, const float* q,
+                                    const float* fbar, const JetParams& j, double* partials, double* out4,
+                                    float* qbar_out, hipStream_t s) {
+    hipLaunchKernelGGL((k_oneform_adj<OF_DIV, PRE_RESTRICT, true>), grid1d(ncells(g), 256), dim3(256), 0, s, psibar,
+                       psi, q, fbar, g, 1.0f, 0, j, qbar_out, partials);
+    const int e = (int)hipGetLastError();
+    if (e) return e;
+    return restrict_adj_reduce(g, partials, out4, s);
 }
 int launch_normalize_adj(const GridInfo& g, float4* psibar, const float4* psi_in, hipStream_t s) {
     ISF_LAUNCH(k_normalize_adj, grid1d(ncells(g), 256), dim3(256), s, psibar, psi_in, ncells(g));

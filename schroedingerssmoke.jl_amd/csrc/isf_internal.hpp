@@ -198,6 +198,11 @@ size_t restrict_adj_partials(const GridInfo& g);
 // qbar_out (optional): the gauge cotangent of the new psibar against psi_in
 int launch_restrict_adj(const GridInfo& g, float4* psibar, const float4* psi_in, const JetParams& j, double* partials,
                         double* out4, hipStream_t s, float* qbar_out = nullptr);
+// launch_oneform_adj_div with PRE_RESTRICT followed by launch_restrict_adj at the same psi,
+// in one pass over the cells (the same results)
+int launch_oneform_adj_div_restrict(const GridInfo& g, float4* psibar, const float4* psi, const float* q,
+                                    const float* fbar, const JetParams& j, double* partials, double* out4,
+                                    float* qbar_out, hipStream_t s);
 int launch_normalize_adj(const GridInfo& g, float4* psibar, const float4* psi_in, hipStream_t s);
 // order: slot processing order (null: slot order); cell-sorted slots make the velocity
 // scatter of a wavefront hit few cells

@@ -343,12 +343,10 @@ static int step_adjoint(isf_rollout_t r, const Tape& tp, int s, int k, const flo
     // :58 second projection (gauge output psi5, one-form at psi4 = restrict(psi3))
     int e = poisson(r, r->f, r->fbar);
     if (e) return e;
-    LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, tp.psi3[k], tp.q2[k], r->fbar, st, PRE_RESTRICT, &j),
-                 "projection adjoint");
-    // :57 jet constraint, with the first projection's gauge cotangent (output psi3)
-    LAUNCH_CHECK(launch_restrict_adj(g, r->psibar, tp.psi3[k], j, r->partials, r->vacc + 4 * (size_t)(it - 1), st,
-                                     r->f),
-                 "restrict adjoint");
+    // and :57 jet constraint, with the first projection's gauge cotangent (output psi3)
+    LAUNCH_CHECK(launch_oneform_adj_div_restrict(g, r->psibar, tp.psi3[k], tp.q2[k], r->fbar, j, r->partials,
+                                                 r->vacc + 4 * (size_t)(it - 1), r->f, st),
+                 "projection and restrict adjoint");
     // :50 first projection (gauge output psi3, one-form at psi2 = normalize(psi1)) and
     // :49 normalize
     if ((e = poisson(r, r->f, r->fbar))) return e;
@@ -655,11 +653,9 @@ int isf_rollout_backward(isf_rollout_t r, double grad_velocity[3], float* grad_t
     for (int j = r->C - 1; j >= 0; j--) {
         const JetParams j0 = jet_at(r, 0);
         if ((e = poisson(r, r->f, r->fbar))) return e;
-        LAUNCH_CHECK(launch_oneform_adj_div(g, r->psibar, r->cpsi[j], r->cq[j], r->fbar, st, PRE_RESTRICT, &j0),
-                     "projection adjoint");
-        LAUNCH_CHECK(launch_restrict_adj(g, r->psibar, r->cpsi[j], j0, r->partials,
-                                         r->vacc + 4 * (size_t)(r->T + j), st, j > 0 ? r->f : nullptr),
-                     "restrict adjoint");
+        LAUNCH_CHECK(launch_oneform_adj_div_restrict(g, r->psibar, r->cpsi[j], r->cq[j], r->fbar, j0, r->partials,
+                                                     r->vacc + 4 * (size_t)(r->T + j), j > 0 ? r->f : nullptr, st),
+                     "projection and restrict adjoint");
     }
     if (r->use_theta) {
         LAUNCH_CHECK(launch_apply_phase_adj(g, r->psibar, r->cpsi[0], r->theta, r->phibar, r->d.phase_res, st),
