@@ -733,7 +733,8 @@ static int simloop_impl(isf_t h, float* psi, float* parts, long long nslots, int
     }
     if (!h->ev_v) {
         hipError_t e;
-        if ((e = hipEventCreateWithFlags(&h->ev_v, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+        // completed by the fused forward pass's own launch (a stop event of hipExtLaunchKernel)
+        if ((e = hipEventCreate(&h->ev_v)) != hipSuccess) return hip_fail(e, "hipEventCreate");
         for (int b = 0; b < 2; b++)
             if ((e = hipEventCreateWithFlags(&h->ev_buf[b], hipEventDisableTiming)) != hipSuccess)
                 return hip_fail(e, "hipEventCreate");
@@ -763,8 +764,7 @@ static int simloop_impl(isf_t h, float* psi, float* parts, long long nslots, int
         // advection of step iter - 1 (velocity in bufs[side_b]) and this step's spawn
         int side_b = -1;
         auto side = [&]() -> int {
-            HIP_CHECK(hipEventRecord(h->ev_v, s), "hipEventRecord");
-            HIP_CHECK(hipStreamWaitEvent(ps, h->ev_v, 0), "hipStreamWaitEvent");
+            HIP_CHECK(hipStreamWaitEvent(ps, h->ev_v, 0), "hipStreamWaitEvent");  // the fused pass has ended
             // :63 staggered_advect! of step iter - 1, then :52-54 add_particles! of this step
             const int* order = nullptr;
             int r = ensure_order(h, parts, nslots, iter - 1, &order, ps);
@@ -791,7 +791,8 @@ static int simloop_impl(isf_t h, float* psi, float* parts, long long nslots, int
             }
             VelOut vo = vo0;
             vo.vel = bufs[b];
-            LAUNCH_CHECK(launch_flow_x_fwd(h->g, p, h->S, h->tb, s, Batch{}, &vo), "flow x forward + velocity_one_form");
+            LAUNCH_CHECK(launch_flow_x_fwd(h->g, p, h->S, h->tb, s, Batch{}, &vo, h->ev_v),
+                         "flow x forward + velocity_one_form");
             side_b = b;
         }
         // the advection right after the fused pass overlaps the flow plane pass: measured

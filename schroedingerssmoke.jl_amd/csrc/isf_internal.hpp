@@ -85,9 +85,10 @@ struct VelOut {
     int sharp;
 };
 // vo != null: the pass also writes the velocity field of psi (in batch mode each member's
-// hbar and 1/d, into the buffer of step *iter - 1; vo's fields are then not read)
+// hbar and 1/d, into the buffer of step *iter - 1; vo's fields are then not read).
+// done != null (with vo): the launch completes that event when the kernel ends.
 int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s,
-                      Batch b = Batch{}, const VelOut* vo = nullptr);
+                      Batch b = Batch{}, const VelOut* vo = nullptr, hipEvent_t done = nullptr);
 int launch_flow_plane(const GridInfo& g, float2* S, const DevTables& t, hipStream_t s, Batch b = Batch{});
 int launch_flow_x_inv(const GridInfo& g, const float2* S, float4* psi, bool normalize, const DevTables& t,
                       hipStream_t s, Batch b = Batch{});

@@ -12,6 +12,7 @@
 // One 3-D transform = an x pass (contiguous lines, written transposed into kx planes) and
 // a plane pass (y and z transforms of a whole kx plane held in LDS, with the spectral
 // multiplier applied between the forward and the inverse transforms).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1281,15 +1282,23 @@ static int x_tile(const GridInfo& g) {
 
 static inline unsigned members_of(const Batch& b) { return b.inst ? (unsigned)b.members : 1u; }
 
+// done != null: the launch itself completes that event when the kernel ends
+// (hipExtLaunchKernel's stop event), with no marker packet behind the kernel on the stream
 template <int NX, int TY>
 static int flow_x_fwd_t(const GridInfo& g, const float4* psi, const float2* sa, const float2* sb, float2* S,
-                        const DevTables& t, hipStream_t s, Batch b, const VelOut* vo) {
+                        const DevTables& t, hipStream_t s, Batch b, const VelOut* vo, hipEvent_t done = nullptr) {
     dim3 grid(g.ny / TY, g.nz, members_of(b));
     if (vo) {
         if (sa) return (int)hipErrorInvalidValue;
+        if (done) {
+            hipExtLaunchKernelGGL((k_flow_x_fwd<NX, TY, true>), grid, dim3(2 * TY * XCfg<NX>::T), 0, s, nullptr, done,
+                                  0, psi, sa, sb, S, g.ny, g.nz, t.tw, b, *vo);
+            return (int)hipGetLastError();
+        }
         ISF_LAUNCH((k_flow_x_fwd<NX, TY, true>), grid, dim3(2 * TY * XCfg<NX>::T), s, psi, sa, sb, S, g.ny, g.nz,
                    t.tw, b, *vo);
     }
+    if (done) return (int)hipErrorInvalidValue;
     ISF_LAUNCH((k_flow_x_fwd<NX, TY, false>), grid, dim3(2 * TY * XCfg<NX>::T), s, psi, sa, sb, S, g.ny, g.nz,
                t.tw, b, VelOut{});
 }
@@ -1348,8 +1357,8 @@ static int c2r_x_t(const GridInfo& g, const float2* P, int mode, float* q, float
     } while (0)
 
 int launch_flow_x_fwd(const GridInfo& g, const float4* psi, float2* S, const DevTables& t, hipStream_t s, Batch b,
-                      const VelOut* vo) {
-    ISF_X_DISPATCH(flow_x_fwd_t, g, psi, nullptr, nullptr, S, t, s, b, vo);
+                      const VelOut* vo, hipEvent_t done) {
+    ISF_X_DISPATCH(flow_x_fwd_t, g, psi, nullptr, nullptr, S, t, s, b, vo, done);
 }
 int launch_flow_x_inv(const GridInfo& g, const float2* S, float4* psi, bool normalize, const DevTables& t,
                       hipStream_t s, Batch b) {
