@@ -434,7 +434,9 @@ def test_full_steps_large_plane(isf_mod):
 
 @pytest.mark.parametrize("dims,box,n_slots,n_spawn", [((64, 32, 32), (4, 2, 2), 2400, 50),
                                                       ((256, 128, 128), (8, 4, 4), 1_000_000, 1000),
-                                                      ((64, 32, 32), (4, 2, 2), 0, 0)])
+                                                      ((64, 32, 32), (4, 2, 2), 0, 0),
+                                                      ((512, 32, 32), (16, 2, 2), 4000, 50),
+                                                      ((64, 256, 32), (2, 8, 2), 4000, 50)])
 def test_simloop_equals_isf_step(isf_mod, dims, box, n_slots, n_spawn):
     """isf_simloop (the driver's simloop, test/runtests.jl:40-66: the velocity pass of each
     step fused into the next step's forward x pass, two alternating velocity buffers)
