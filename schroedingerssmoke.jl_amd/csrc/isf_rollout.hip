@@ -253,9 +253,13 @@ static int spawn_step(isf_rollout_t r, int it, long long first0, long long nsp, 
 // Steps of segment s from its checkpoint into tape tp, on w's stream and scratch.  The
 // forward (checkpoint = true) also writes the next checkpoint; a recompute writes only
 // the tape (its last step output is the next psi checkpoint again, rewritten with the
-// same values).  With side, the particle part (spawn, velocity, advection, particle
-// checkpoint) runs on that stream after each step's psi (event fstep), beside the next
-// step's psi passes; the caller orders side after the set-up and waits for fseg at the end.
+// same values).  As in isf_simloop, the velocity field of step k - 1 (into its tape entry)
+// is formed by step k's forward x pass, which reads that step's output anyway, and step
+// k - 1's advection follows it, then step k's spawn; the segment's last velocity pass and
+// advection follow its last step.  With side, the particle part runs on that stream,
+// ordered after each forward x pass by the event its launch completes (fstep), beside the
+// step's remaining psi passes; the caller orders side after the set-up and waits for fseg
+// at the end.
 static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bool checkpoint,
                        hipStream_t side = nullptr) {
     isf_t h = r->h;
@@ -263,17 +267,43 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
     hipStream_t pt = side ? side : st;
     const size_t pb = sizeof(float) * 3 * (size_t)r->P;
     const int ks = seg_steps(r, s);
+    const bool parts = r->P > 0;
+    const VelOut vo0{nullptr, h->hbar, 1.0f / r->g.dx, 1.0f / r->g.dy, 1.0f / r->g.dz, 1};
     // the step outputs of tape set s % 2 are read by the velocities of segment s - 2
     if (side && s >= 2) HIP_CHECK(hipStreamWaitEvent(st, r->fseg[s % 2], 0), "hipStreamWaitEvent");
     // particles: tape entry p0[k] holds step k's state after its spawn; the advection of
     // step k writes p0[k + 1] (the segment's last step: w.parts), so no step copies them
-    if (r->P > 0)
+    if (parts)
         HIP_CHECK(hipMemcpyAsync(tp.p0[0], r->ck_parts[s], pb, hipMemcpyDeviceToDevice, pt), "particles checkpoint");
+    // :63 of step k (velocity in its tape entry) on the particle stream
+    auto advect = [&](int k) -> int {
+        LAUNCH_CHECK(launch_advect(r->g, k + 1 < ks ? tp.p0[k + 1] : w.parts, r->P, nullptr, tp.vel[k], h->dt, pt,
+                                   Batch{}, 0, tp.p0[k], side != nullptr),
+                     "staggered_advect");
+        return ISF_OK;
+    };
     for (int k = 0; k < ks; k++) {
         const int it = s * r->K + k + 1;
         const float4* in = psi0_of(r, tp, s, k);
         float4* out = psi0_of(r, tp, s, k + 1);
-        LAUNCH_CHECK(launch_flow_x_fwd(r->g, in, w.S, h->tb, st), "flow x forward");
+        if (parts && k > 0) {
+            // :59-62 of step k - 1 inside this step's forward x pass (in = its output)
+            VelOut vo = vo0;
+            vo.vel = tp.vel[k - 1];
+            LAUNCH_CHECK(launch_flow_x_fwd(r->g, in, w.S, h->tb, st, Batch{}, &vo, side ? r->fstep : nullptr),
+                         "flow x forward + velocity_one_form");
+            if (side) HIP_CHECK(hipStreamWaitEvent(side, r->fstep, 0), "hipStreamWaitEvent");
+            int e = advect(k - 1);
+            if (e) return e;
+        } else {
+            LAUNCH_CHECK(launch_flow_x_fwd(r->g, in, w.S, h->tb, st), "flow x forward");
+        }
+        long long first0, nsp;
+        spawn_slots(r, it, &first0, &nsp);
+        if (nsp > 0) {
+            int e2 = spawn_step(r, it, first0, nsp, tp.p0[k], pt);
+            if (e2) return e2;
+        }
         LAUNCH_CHECK(launch_flow_plane(r->g, w.S, h->tb, st), "flow plane");
         // psi1 for the tape, normalize_psi of it into psi3 (projected in place next)
         LAUNCH_CHECK(launch_flow_x_inv_raw_normalized(r->g, w.S, tp.psi1[k], tp.psi3[k], h->tb, st),
@@ -282,24 +312,18 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
         int e = project_q(r, tp.psi3[k], tp.q1[k], w, out, &j);  // out = restrict_velocity!(psi3)
         if (e) return e;
         if ((e = project_q(r, out, tp.q2[k], w))) return e;
+    }
+    if (parts) {
+        // the last step's velocity and advection
         if (side) {
             HIP_CHECK(hipEventRecord(r->fstep, st), "hipEventRecord");
             HIP_CHECK(hipStreamWaitEvent(side, r->fstep, 0), "hipStreamWaitEvent");
         }
-        long long first0, nsp;
-        spawn_slots(r, it, &first0, &nsp);
-        if (nsp > 0) {
-            int e2 = spawn_step(r, it, first0, nsp, tp.p0[k], pt);
-            if (e2) return e2;
-        }
-        if (r->P > 0) {  // the velocity goes to the tape: the advection adjoint reads it again
-            LAUNCH_CHECK(launch_velocity(r->g, out, h->hbar, 1, tp.vel[k], pt), "velocity_one_form");
-            LAUNCH_CHECK(launch_advect(r->g, k + 1 < ks ? tp.p0[k + 1] : w.parts, r->P, nullptr, tp.vel[k], h->dt,
-                                       pt, Batch{}, 0, tp.p0[k], side != nullptr),
-                         "staggered_advect");
-        }
+        LAUNCH_CHECK(launch_velocity(r->g, psi0_of(r, tp, s, ks), h->hbar, 1, tp.vel[ks - 1], pt), "velocity_one_form");
+        int e = advect(ks - 1);
+        if (e) return e;
     }
-    if (checkpoint && s + 1 < r->nseg && r->P > 0)
+    if (checkpoint && s + 1 < r->nseg && parts)
         HIP_CHECK(hipMemcpyAsync(r->ck_parts[s + 1], w.parts, pb, hipMemcpyDeviceToDevice, pt), "particles checkpoint");
     if (side) HIP_CHECK(hipEventRecord(r->fseg[s % 2], side), "hipEventRecord");
     return ISF_OK;
