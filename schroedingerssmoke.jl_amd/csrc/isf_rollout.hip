@@ -500,21 +500,24 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
     }
 #undef RA
     {
+        // every rollout event orders one of the rollout's streams after another on this
+        // device; none is waited for by the host, so none needs a system-scope release
+        constexpr unsigned kStreamEv = hipEventDisableTiming | hipEventDisableSystemFence;
         hipError_t he = hipStreamCreate(&r->rw.st);
         for (int i = 0; i < 2 && he == hipSuccess; i++) {
-            he = hipEventCreateWithFlags(&r->rec_done[i], hipEventDisableTiming);
-            if (he == hipSuccess) he = hipEventCreateWithFlags(&r->adj_done[i], hipEventDisableTiming);
+            he = hipEventCreateWithFlags(&r->rec_done[i], kStreamEv);
+            if (he == hipSuccess) he = hipEventCreateWithFlags(&r->adj_done[i], kStreamEv);
         }
-        if (he == hipSuccess) he = hipEventCreateWithFlags(&r->start, hipEventDisableTiming);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&r->start, kStreamEv);
         r->pst = h->pstream;
         for (int i = 0; i < 2 && he == hipSuccess; i++) {
             r->pev[i].assign(r->K, nullptr);
             for (int k = 0; k < r->K && he == hipSuccess; k++)
-                he = hipEventCreateWithFlags(&r->pev[i][k], hipEventDisableTiming);
-            if (he == hipSuccess) he = hipEventCreateWithFlags(&r->pdone[i], hipEventDisableTiming);
-            if (he == hipSuccess) he = hipEventCreateWithFlags(&r->fseg[i], hipEventDisableTiming);
+                he = hipEventCreateWithFlags(&r->pev[i][k], kStreamEv);
+            if (he == hipSuccess) he = hipEventCreateWithFlags(&r->pdone[i], kStreamEv);
+            if (he == hipSuccess) he = hipEventCreateWithFlags(&r->fseg[i], kStreamEv);
         }
-        if (he == hipSuccess) he = hipEventCreateWithFlags(&r->fstep, hipEventDisableTiming);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&r->fstep, kStreamEv);
         if (he != hipSuccess) return bail(hip_fail(he, "rollout streams"));
     }
     hipStream_t st = h->stream;
