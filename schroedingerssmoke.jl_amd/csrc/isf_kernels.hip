@@ -909,7 +909,6 @@ k_div_r2c(const float4* __restrict__ psi, float2* __restrict__ P, GridInfo g, co
     __shared__ float vy_s[TY + 2][NX];
     __shared__ float2 lds[TY * LS];     // v_z differences, then f as packed complex lines
     __shared__ float2 twm[FftTw<M, E>::SIZE];
-    __shared__ float2 twn[NX];
     float* dz_s = reinterpret_cast<float*>(lds);
     const int tid = threadIdx.x, lane = tid & 63;
     const int ny = g.ny, nz = g.nz;
@@ -918,7 +917,6 @@ k_div_r2c(const float4* __restrict__ psi, float2* __restrict__ P, GridInfo g, co
     const size_t zo0 = sz * z0, zo1 = sz * (z0 + 1);
     const size_t zom = sz * (z0 > 0 ? z0 - 1 : nz - 1), zop = sz * (z0 + 2 < nz ? z0 + 2 : 0);
     FftTw<M, E>::template fill<NT>(twm, tw, tid);
-    load_twiddles<NX, NT>(twn, tw);
     // v_y of the row below the tile, in both z planes
     {
         const int ym = y0 > 0 ? y0 - 1 : ny - 1;
@@ -929,8 +927,11 @@ k_div_r2c(const float4* __restrict__ psi, float2* __restrict__ P, GridInfo g, co
         }
     }
     // edge angles of the tile's columns, G columns per thread at a time with their loads
-    // issued together; a wavefront holds consecutive x of one row
-    constexpr int K = COLS / NT, G = K < 2 ? K : 2;
+    // issued together; a wavefront holds consecutive x of one row.  Rows of 256 and more
+    // take one column at a time: two in flight need 86 VGPRs, 4 wavefronts per SIMD
+    // (0.2626 against 0.2598 ms per step at 256x128x128), while the shorter rows gain
+    // from the second column (128x64x64 rollout forward 3.33 against 3.44 ms).
+    constexpr int K = COLS / NT, G = NX >= 256 ? 1 : (K < 2 ? K : 2);
 #pragma unroll 1
     for (int k0 = 0; k0 < K; k0 += G) {
         float4 p0[G], p1[G], pm[G], pp[G], py0[G], py1[G], ph0[G], ph1[G];
@@ -1000,7 +1001,7 @@ k_div_r2c(const float4* __restrict__ psi, float2* __restrict__ P, GridInfo g, co
         const float2 e = cscale(cadd(zk, zm), 0.5f);
         const float2 dd = csub(zk, zm);
         const float2 o = make_float2(0.5f * dd.y, -0.5f * dd.x);  // (zk - zm) / (2i)
-        P[((size_t)k * nz + z0 + h) * ny + y0 + lh] = cadd(e, cmul(twn[k], o));
+        P[((size_t)k * nz + z0 + h) * ny + y0 + lh] = cadd(e, cmul(tw[NX + k], o));
     }
 }
 
