@@ -904,7 +904,16 @@ k_div_r2c(const float4* __restrict__ psi, float2* __restrict__ P, GridInfo g, co
         g = m.g;
         tw = m.tb.tw;
     }
-    __shared__ float vx_s[TY][NX];
+    // STREAM (rows of 256 and more): f of a column's two cells right after its edge angles
+    // (one block barrier per column makes the row below visible), so neither v_x nor the
+    // z differences are staged and the block needs 38 instead of 54 KB of LDS at
+    // 256 x 16 lines (4 blocks per CU instead of 3)
+    constexpr bool STREAM = NX >= 256;
+    constexpr int NSEG = NX / 64;  // 64-cell row segments (STREAM)
+    __shared__ float vx_s[STREAM ? 1 : TY][NX];
+    // STREAM: v_x of each row segment's last cell, the x - 1 neighbour of the next
+    // segment's first cell
+    __shared__ float vxb[STREAM ? TY * NSEG : 1];
     // z plane h, row ly: v_y at y0 + ly in row h (HT + 1) + ly + 1; at y0 - 1 in row h (HT + 1)
     __shared__ float vy_s[TY + 2][NX];
     __shared__ float2 lds[TY * LS];     // v_z differences, then f as packed complex lines
@@ -926,69 +935,116 @@ k_div_r2c(const float4* __restrict__ psi, float2* __restrict__ P, GridInfo g, co
             vy_s[h * (HT + 1)][x] = edge_angle(psi[sy * ym + zo + x], psi[sy * y0 + zo + x]);
         }
     }
-    // edge angles of the tile's columns, G columns per thread at a time with their loads
-    // issued together; a wavefront holds consecutive x of one row.  Rows of 256 and more
-    // take one column at a time: two in flight need 86 VGPRs, 4 wavefronts per SIMD
-    // (0.2626 against 0.2598 ms per step at 256x128x128), while the shorter rows gain
-    // from the second column (128x64x64 rollout forward 3.33 against 3.44 ms).
-    constexpr int K = COLS / NT, G = NX >= 256 ? 1 : (K < 2 ? K : 2);
-#pragma unroll 1
-    for (int k0 = 0; k0 < K; k0 += G) {
-        float4 p0[G], p1[G], pm[G], pp[G], py0[G], py1[G], ph0[G], ph1[G];
-#pragma unroll
-        for (int q = 0; q < G; q++) {
-            const int j = tid + (k0 + q) * NT, ly = j / NX, x = j % NX, y = y0 + ly;
-            const size_t row = sy * y, rowy = sy * (y + 1 < ny ? y + 1 : 0);
-            p0[q] = psi[row + zo0 + x];
-            p1[q] = psi[row + zo1 + x];
-            pm[q] = psi[row + zom + x];
-            pp[q] = psi[row + zop + x];
-            py0[q] = psi[rowy + zo0 + x];
-            py1[q] = psi[rowy + zo1 + x];
-            ph0[q] = p0[q];
-            ph1[q] = p1[q];
-            if (lane == 63 || x + 1 == NX) {
-                const int xn = x + 1 < NX ? x + 1 : 0;
-                ph0[q] = psi[row + zo0 + xn];
-                ph1[q] = psi[row + zo1 + xn];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < G; q++) {
-            const int j = tid + (k0 + q) * NT, ly = j / NX, x = j % NX;
-            const int src = lane < 63 ? lane + 1 : lane;
-            float4 px0 = shfl4(p0[q], src), px1 = shfl4(p1[q], src);
-            if (lane == 63 || x + 1 == NX) {
-                px0 = ph0[q];
-                px1 = ph1[q];
-            }
-            vx_s[ly][x] = edge_angle(p0[q], px0);
-            vx_s[HT + ly][x] = edge_angle(p1[q], px1);
-            vy_s[ly + 1][x] = edge_angle(p0[q], py0[q]);
-            vy_s[HT + 2 + ly][x] = edge_angle(p1[q], py1[q]);
-            const float ea = edge_angle(pm[q], p0[q]), eb = edge_angle(p0[q], p1[q]), ec = edge_angle(p1[q], pp[q]);
-            dz_s[j] = eb - ea;
-            dz_s[COLS + j] = ec - eb;
-        }
-    }
-    __syncthreads();
-    // f(c) = sum_a (v_a(c) - v_a(c - e_a)) / d_a^2
+    // edge angles of the tile's columns; a wavefront holds consecutive x of one row.
+    // STREAM takes one column per thread at a time (57 VGPRs, 8 wavefronts per SIMD at
+    // 256 x 16 lines; two columns' loads in flight needed 86 VGPRs, 4 wavefronts per SIMD:
+    // 0.2626 -> 0.2598 ms per step at 256x128x128 for one column with v_x and the z
+    // differences staged, 0.2738 -> 0.2688 on a slower box for STREAM).  Shorter rows take
+    // G = 2 columns with their loads issued together (128x64x64 rollout forward 3.33
+    // against 3.44 ms with one).
+    constexpr int K = COLS / NT, G = K < 2 ? K : 2;
     const float ds0 = 1.0f / (g.dx * g.dx), ds1 = 1.0f / (g.dy * g.dy), ds2 = 1.0f / (g.dz * g.dz);
-    constexpr int KC = CELLS / NT;
-    float fv[KC];
+    if constexpr (STREAM) {
+#pragma unroll 1
+        for (int k = 0; k < K; k++) {
+            const int j = tid + k * NT, ly = j / NX, x = j % NX, y = y0 + ly;
+            const size_t row = sy * y, rowy = sy * (y + 1 < ny ? y + 1 : 0);
+            const float4 p0 = psi[row + zo0 + x], p1 = psi[row + zo1 + x];
+            const float4 pm = psi[row + zom + x], pp = psi[row + zop + x];
+            const float4 py0 = psi[rowy + zo0 + x], py1 = psi[rowy + zo1 + x];
+            float4 ph0 = p0, ph1 = p1;
+            if (lane == 63) {  // x + 1 is the next segment's first cell (or the row's first)
+                const int xn = x + 1 < NX ? x + 1 : 0;
+                ph0 = psi[row + zo0 + xn];
+                ph1 = psi[row + zo1 + xn];
+            }
+            const int src = lane < 63 ? lane + 1 : lane;
+            float4 px0 = shfl4(p0, src), px1 = shfl4(p1, src);
+            if (lane == 63) {
+                px0 = ph0;
+                px1 = ph1;
+            }
+            const float vx0 = edge_angle(p0, px0), vx1 = edge_angle(p1, px1);
+            vy_s[ly + 1][x] = edge_angle(p0, py0);
+            vy_s[HT + 2 + ly][x] = edge_angle(p1, py1);
+            const float ea = edge_angle(pm, p0), eb = edge_angle(p0, p1), ec = edge_angle(p1, pp);
+            if (lane == 63) {
+                vxb[ly * NSEG + x / 64] = vx0;
+                vxb[(HT + ly) * NSEG + x / 64] = vx1;
+            }
+            // v_x(c - e_x): the lane to the left, or for a segment's first cell the previous
+            // segment's last (the row's last cell for x = 0)
+            float vxm0 = __shfl(vx0, lane > 0 ? lane - 1 : 0, 64), vxm1 = __shfl(vx1, lane > 0 ? lane - 1 : 0, 64);
+            __syncthreads();  // the rows below and the segment ends of this column are written
+            if (lane == 0) {
+                const int sp = (x / 64 + NSEG - 1) % NSEG;
+                vxm0 = vxb[ly * NSEG + sp];
+                vxm1 = vxb[(HT + ly) * NSEG + sp];
+            }
+            const float f0 = (vx0 - vxm0) * ds0 + (vy_s[ly + 1][x] - vy_s[ly][x]) * ds1 + (eb - ea) * ds2;
+            const float f1 =
+                (vx1 - vxm1) * ds0 + (vy_s[HT + 2 + ly][x] - vy_s[HT + 1 + ly][x]) * ds1 + (ec - eb) * ds2;
+            reinterpret_cast<float*>(lds + ly * LS + pad16(x >> 1))[x & 1] = f0;
+            reinterpret_cast<float*>(lds + (HT + ly) * LS + pad16(x >> 1))[x & 1] = f1;
+        }
+    } else {
+#pragma unroll 1
+        for (int k0 = 0; k0 < K; k0 += G) {
+            float4 p0[G], p1[G], pm[G], pp[G], py0[G], py1[G], ph0[G], ph1[G];
 #pragma unroll
-    for (int k = 0; k < KC; k++) {
-        const int i = tid + k * NT, l = i / NX, x = i % NX;
-        const float vx = vx_s[l][x], vxm = vx_s[l][x > 0 ? x - 1 : NX - 1];
-        const int ry = l + l / HT + 1;
-        const float vy = vy_s[ry][x], vym = vy_s[ry - 1][x];
-        fv[k] = (vx - vxm) * ds0 + (vy - vym) * ds1 + dz_s[i] * ds2;
-    }
-    __syncthreads();  // dz_s is read; lds now receives f
+            for (int q = 0; q < G; q++) {
+                const int j = tid + (k0 + q) * NT, ly = j / NX, x = j % NX, y = y0 + ly;
+                const size_t row = sy * y, rowy = sy * (y + 1 < ny ? y + 1 : 0);
+                p0[q] = psi[row + zo0 + x];
+                p1[q] = psi[row + zo1 + x];
+                pm[q] = psi[row + zom + x];
+                pp[q] = psi[row + zop + x];
+                py0[q] = psi[rowy + zo0 + x];
+                py1[q] = psi[rowy + zo1 + x];
+                ph0[q] = p0[q];
+                ph1[q] = p1[q];
+                if (lane == 63 || x + 1 == NX) {
+                    const int xn = x + 1 < NX ? x + 1 : 0;
+                    ph0[q] = psi[row + zo0 + xn];
+                    ph1[q] = psi[row + zo1 + xn];
+                }
+            }
 #pragma unroll
-    for (int k = 0; k < KC; k++) {
-        const int i = tid + k * NT, l = i / NX, x = i % NX;
-        reinterpret_cast<float*>(lds + l * LS + pad16(x >> 1))[x & 1] = fv[k];
+            for (int q = 0; q < G; q++) {
+                const int j = tid + (k0 + q) * NT, ly = j / NX, x = j % NX;
+                const int src = lane < 63 ? lane + 1 : lane;
+                float4 px0 = shfl4(p0[q], src), px1 = shfl4(p1[q], src);
+                if (lane == 63 || x + 1 == NX) {
+                    px0 = ph0[q];
+                    px1 = ph1[q];
+                }
+                vx_s[ly][x] = edge_angle(p0[q], px0);
+                vx_s[HT + ly][x] = edge_angle(p1[q], px1);
+                vy_s[ly + 1][x] = edge_angle(p0[q], py0[q]);
+                vy_s[HT + 2 + ly][x] = edge_angle(p1[q], py1[q]);
+                const float ea = edge_angle(pm[q], p0[q]), eb = edge_angle(p0[q], p1[q]), ec = edge_angle(p1[q], pp[q]);
+                dz_s[j] = eb - ea;
+                dz_s[COLS + j] = ec - eb;
+            }
+        }
+        __syncthreads();
+        // f(c) = sum_a (v_a(c) - v_a(c - e_a)) / d_a^2
+        constexpr int KC = CELLS / NT;
+        float fv[KC];
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            const int i = tid + k * NT, l = i / NX, x = i % NX;
+            const float vx = vx_s[l][x], vxm = vx_s[l][x > 0 ? x - 1 : NX - 1];
+            const int ry = l + l / HT + 1;
+            const float vy = vy_s[ry][x], vym = vy_s[ry - 1][x];
+            fv[k] = (vx - vxm) * ds0 + (vy - vym) * ds1 + dz_s[i] * ds2;
+        }
+        __syncthreads();  // dz_s is read; lds now receives f
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            const int i = tid + k * NT, l = i / NX, x = i % NX;
+            reinterpret_cast<float*>(lds + l * LS + pad16(x >> 1))[x & 1] = fv[k];
+        }
     }
     __syncthreads();
     // the R2C transform of k_r2c_x; line l is row y0 + l % HT of z plane z0 + l / HT
