@@ -904,11 +904,12 @@ k_div_r2c(const float4* __restrict__ psi, float2* __restrict__ P, GridInfo g, co
         g = m.g;
         tw = m.tb.tw;
     }
-    // STREAM (rows of 256 and more): f of a column's two cells right after its edge angles
-    // (one block barrier per column makes the row below visible), so neither v_x nor the
-    // z differences are staged and the block needs 38 instead of 54 KB of LDS at
-    // 256 x 16 lines (4 blocks per CU instead of 3)
-    constexpr bool STREAM = NX >= 256;
+    // STREAM (rows of 64 and more, so that every wavefront starts a row segment): f of a
+    // column's two cells right after its edge angles (one block barrier per column makes
+    // the row below visible), so neither v_x nor the z differences are staged and the
+    // block needs 38 instead of 54 KB of LDS at 256 x 16 lines (4 blocks per CU instead
+    // of 3); the 48-jet ensemble step 0.585 -> 0.579 ms with it at 64 and 128 wide
+    constexpr bool STREAM = NX >= 64;
     constexpr int NSEG = NX / 64;  // 64-cell row segments (STREAM)
     __shared__ float vx_s[STREAM ? 1 : TY][NX];
     // STREAM: v_x of each row segment's last cell, the x - 1 neighbour of the next
@@ -939,9 +940,8 @@ k_div_r2c(const float4* __restrict__ psi, float2* __restrict__ P, GridInfo g, co
     // STREAM takes one column per thread at a time (57 VGPRs, 8 wavefronts per SIMD at
     // 256 x 16 lines; two columns' loads in flight needed 86 VGPRs, 4 wavefronts per SIMD:
     // 0.2626 -> 0.2598 ms per step at 256x128x128 for one column with v_x and the z
-    // differences staged, 0.2738 -> 0.2688 on a slower box for STREAM).  Shorter rows take
-    // G = 2 columns with their loads issued together (128x64x64 rollout forward 3.33
-    // against 3.44 ms with one).
+    // differences staged, 0.2738 -> 0.2688 on a slower box for STREAM).  Rows of 16 and 32
+    // take G = 2 columns with their loads issued together.
     constexpr int K = COLS / NT, G = K < 2 ? K : 2;
     const float ds0 = 1.0f / (g.dx * g.dx), ds1 = 1.0f / (g.dy * g.dy), ds2 = 1.0f / (g.dz * g.dz);
     if constexpr (STREAM) {
