@@ -259,22 +259,29 @@ static int spawn_step(isf_rollout_t r, int it, long long first0, long long nsp, 
 // advection follow its last step.  With side, the particle part runs on that stream,
 // ordered after each forward x pass by the event its launch completes (fstep), beside the
 // step's remaining psi passes; the caller orders side after the set-up and waits for fseg
-// at the end.
+// at the end.  Steps [k0, k1) only (k1 < 0: to the segment's end), so that a caller can
+// interleave a segment's enqueues with another stream's: the set-up comes with k0 = 0 and
+// the segment's tail with its last step.
 static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bool checkpoint,
-                       hipStream_t side = nullptr) {
+                       hipStream_t side = nullptr, int k0 = 0, int k1 = -1) {
     isf_t h = r->h;
     hipStream_t st = w.st;
     hipStream_t pt = side ? side : st;
     const size_t pb = sizeof(float) * 3 * (size_t)r->P;
     const int ks = seg_steps(r, s);
+    if (k1 < 0 || k1 > ks) k1 = ks;
     const bool parts = r->P > 0;
     const VelOut vo0{nullptr, h->hbar, 1.0f / r->g.dx, 1.0f / r->g.dy, 1.0f / r->g.dz, 1};
-    // the step outputs of tape set s % 2 are read by the velocities of segment s - 2
-    if (side && s >= 2) HIP_CHECK(hipStreamWaitEvent(st, r->fseg[s % 2], 0), "hipStreamWaitEvent");
-    // particles: tape entry p0[k] holds step k's state after its spawn; the advection of
-    // step k writes p0[k + 1] (the segment's last step: w.parts), so no step copies them
-    if (parts)
-        HIP_CHECK(hipMemcpyAsync(tp.p0[0], r->ck_parts[s], pb, hipMemcpyDeviceToDevice, pt), "particles checkpoint");
+    if (k0 == 0) {
+        // the step outputs of tape set s % 2 are read by the velocities of segment s - 2
+        if (side && s >= 2) HIP_CHECK(hipStreamWaitEvent(st, r->fseg[s % 2], 0), "hipStreamWaitEvent");
+        // particles: tape entry p0[k] holds step k's state after its spawn; the advection
+        // of step k writes p0[k + 1] (the segment's last step: w.parts), so no step copies
+        // them
+        if (parts)
+            HIP_CHECK(hipMemcpyAsync(tp.p0[0], r->ck_parts[s], pb, hipMemcpyDeviceToDevice, pt),
+                      "particles checkpoint");
+    }
     // :63 of step k (velocity in its tape entry) on the particle stream
     auto advect = [&](int k) -> int {
         LAUNCH_CHECK(launch_advect(r->g, k + 1 < ks ? tp.p0[k + 1] : w.parts, r->P, nullptr, tp.vel[k], h->dt, pt,
@@ -282,7 +289,7 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
                      "staggered_advect");
         return ISF_OK;
     };
-    for (int k = 0; k < ks; k++) {
+    for (int k = k0; k < k1; k++) {
         const int it = s * r->K + k + 1;
         const float4* in = psi0_of(r, tp, s, k);
         float4* out = psi0_of(r, tp, s, k + 1);
@@ -313,6 +320,7 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
         if (e) return e;
         if ((e = project_q(r, out, tp.q2[k], w))) return e;
     }
+    if (k1 < ks) return ISF_OK;
     if (parts) {
         // the last step's velocity and advection
         if (side) {
@@ -636,15 +644,16 @@ int isf_rollout_backward(isf_rollout_t r, double grad_velocity[3], float* grad_t
     HIP_CHECK(hipEventRecord(r->start, st), "hipEventRecord");
     HIP_CHECK(hipStreamWaitEvent(r->rw.st, r->start, 0), "hipStreamWaitEvent");
     HIP_CHECK(hipStreamWaitEvent(r->pst, r->start, 0), "hipStreamWaitEvent");
-    auto recompute = [&](int s) -> int {
+    // steps [k0, k1) of segment s's recompute (k1 < 0: to the end)
+    auto recompute = [&](int s, int k0 = 0, int k1 = -1) -> int {
         const int b = s % 2;
-        if (s + 2 < r->nseg) {
+        if (k0 == 0 && s + 2 < r->nseg) {
             HIP_CHECK(hipStreamWaitEvent(r->rw.st, r->adj_done[b], 0), "hipStreamWaitEvent");
             HIP_CHECK(hipStreamWaitEvent(r->rw.st, r->pdone[b], 0), "hipStreamWaitEvent");
         }
-        int e2 = seg_forward(r, s, r->tape[b], r->rw, false);
+        int e2 = seg_forward(r, s, r->tape[b], r->rw, false, nullptr, k0, k1);
         if (e2) return e2;
-        HIP_CHECK(hipEventRecord(r->rec_done[b], r->rw.st), "hipEventRecord");
+        if (k1 < 0 || k1 >= seg_steps(r, s)) HIP_CHECK(hipEventRecord(r->rec_done[b], r->rw.st), "hipEventRecord");
         return ISF_OK;
     };
     // the last segment's tape is still there after a forward (not after a backward, whose
@@ -655,20 +664,27 @@ int isf_rollout_backward(isf_rollout_t r, double grad_velocity[3], float* grad_t
     } else if ((e = recompute(r->nseg - 1))) {
         return e;
     }
+    // The host enqueues a segment's particle steps, its psi steps and the next recompute's
+    // steps interleaved, one step of each at a time: the streams then start their work
+    // together (enqueued one after another, the recompute's ~60 launches came before the
+    // segment's first psi step and the psi chain idled for them at every segment).
     for (int s = r->nseg - 1; s >= 0; s--) {
-        const int b = s % 2;
+        const int b = s % 2, ks = seg_steps(r, s), kr = s > 0 ? seg_steps(r, s - 1) : 0;
         HIP_CHECK(hipStreamWaitEvent(r->pst, r->rec_done[b], 0), "hipStreamWaitEvent");
         if (s + 2 < r->nseg) HIP_CHECK(hipStreamWaitEvent(r->pst, r->adj_done[b], 0), "hipStreamWaitEvent");
-        for (int k = seg_steps(r, s) - 1; k >= 0; k--) {
-            if ((e = step_adjoint_particles(r, r->tape[b], s, k, r->vbar[b][k]))) return e;
-            HIP_CHECK(hipEventRecord(r->pev[b][k], r->pst), "hipEventRecord");
-        }
-        HIP_CHECK(hipEventRecord(r->pdone[b], r->pst), "hipEventRecord");
         HIP_CHECK(hipStreamWaitEvent(st, r->rec_done[b], 0), "hipStreamWaitEvent");
-        if (s > 0 && (e = recompute(s - 1))) return e;
-        for (int k = seg_steps(r, s) - 1; k >= 0; k--) {
-            HIP_CHECK(hipStreamWaitEvent(st, r->pev[b][k], 0), "hipStreamWaitEvent");
-            if ((e = step_adjoint(r, r->tape[b], s, k, r->vbar[b][k]))) return e;
+        for (int i = 0; i < ks || i < kr; i++) {
+            const int k = ks - 1 - i;
+            if (k >= 0) {
+                if ((e = step_adjoint_particles(r, r->tape[b], s, k, r->vbar[b][k]))) return e;
+                HIP_CHECK(hipEventRecord(r->pev[b][k], r->pst), "hipEventRecord");
+                if (k == 0) HIP_CHECK(hipEventRecord(r->pdone[b], r->pst), "hipEventRecord");
+            }
+            if (i < kr && (e = recompute(s - 1, i, i + 1))) return e;
+            if (k >= 0) {
+                HIP_CHECK(hipStreamWaitEvent(st, r->pev[b][k], 0), "hipStreamWaitEvent");
+                if ((e = step_adjoint(r, r->tape[b], s, k, r->vbar[b][k]))) return e;
+            }
         }
         HIP_CHECK(hipEventRecord(r->adj_done[b], st), "hipEventRecord");
     }
