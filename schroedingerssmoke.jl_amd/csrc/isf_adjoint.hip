@@ -363,6 +363,168 @@ k_oneform_adj(float4* __restrict__ psibar, const float4* __restrict__ psi, const
     }
 }
 
+// The same adjoint, tiled for 64 <= nx <= 256 and computed per edge instead of per cell.
+// A block of kOfaThreads threads holds RY = kOfaThreads / nx whole x rows and walks tz z
+// planes; a thread owns one (x, y) column.  Each thread forms the one-form input
+// (psi_pre) of its own cell once, and evaluates the three forward edges (c, c + e_a) of
+// its cell once: it adds the edge's term for c itself and hands the edge's cotangent zbar
+// to the cell c + e_a, which adds zbar * psi_pre(c): through LDS for x and y (z-plane
+// parity double buffers), in a register for z (the same thread's next plane).  The two
+// edges whose first end lies outside the block are evaluated again where they are
+// needed: the backward y edge by the block's first row (from a halo row) and the
+// backward z edge at the slab's first plane.  So a cell costs about 3 + 1/RY + 1/tz edges
+// and 1 + 2/RY + 1/tz psi_pre instead of 6 and 7.  Same terms as k_oneform_adj, summed in
+// another order.
+constexpr int kOfaThreads = 512;
+template <int PRE>
+__device__ __forceinline__ float4 pre_of(float4 p, int x, int y, int z, const GridInfo& g, const JetParams& j) {
+    if (PRE == PRE_NORMALIZE) return normalized(p);
+    if (PRE == PRE_RESTRICT) return is_jet_cell(x, y, z, g, j) ? jet_phase(p, x, y, z, g, j) : p;
+    return p;
+}
+template <int MODE, int PRE, bool RADJ>
+__global__ void __launch_bounds__(kOfaThreads)
+k_oneform_adj_tile(float4* __restrict__ psibar, const float4* __restrict__ psi, const float* __restrict__ q,
+                   const float* __restrict__ w, GridInfo g, float hbar, int sharp, JetParams j, int tz,
+                   float* __restrict__ qbar_out, double* __restrict__ partials) {
+    static_assert(!RADJ || PRE == PRE_RESTRICT, "the fused restrict adjoint follows a one-form at restrict(psi)");
+    __shared__ float4 ps[2][kOfaThreads];  // psi_pre of the block's cells (z-plane parity)
+    __shared__ float wsh[2][MODE == OF_DIV ? kOfaThreads : 1];
+    __shared__ float2 zx_s[2][kOfaThreads], zy_s[2][kOfaThreads];  // zbar of the cell's backward x / y edge
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int RY = kOfaThreads / nx;
+    const int tid = threadIdx.x, x = tid % nx, ty = tid / nx;
+    const int y = blockIdx.x * RY + ty, z0 = blockIdx.y * tz;
+    const size_t sy = nx, sz = (size_t)nx * ny;
+    const int xp = x + 1 < nx ? x + 1 : 0, xm = x > 0 ? x - 1 : nx - 1;
+    const int yp = y + 1 < ny ? y + 1 : 0, ym = y > 0 ? y - 1 : ny - 1;
+    const bool top = ty == RY - 1, bottom = ty == 0;
+    const float id2[3] = {1.0f / (g.dx * g.dx), 1.0f / (g.dy * g.dy), 1.0f / (g.dz * g.dz)};
+    const float sc[3] = {sharp ? hbar / g.dx : hbar, sharp ? hbar / g.dy : hbar, sharp ? hbar / g.dz : hbar};
+    const size_t col = x + sy * y;
+    double sums[4] = {0.0, 0.0, 0.0, 0.0};
+    // this plane's cell (raw and pre), its w (OF_DIV), and the previous plane's pre and
+    // forward z cotangent (the backward z edge of this cell)
+    float4 praw = psi[col + sz * z0];
+    float4 pc = pre_of<PRE>(praw, x, y, z0, g, j);
+    float wc = MODE == OF_DIV ? w[col + sz * z0] : 0.0f;
+    float4 pprev;
+    float2 zzprev;
+    {
+        const int zm = z0 > 0 ? z0 - 1 : nz - 1;
+        const size_t cm = col + sz * zm;
+        pprev = pre_of<PRE>(psi[cm], x, y, zm, g, j);
+        const float wb = MODE == OF_DIV ? (w[cm] - wc) * id2[2] : w[3 * cm + 2] * sc[2];
+        float re, im;
+        edge_z(pprev, pc, re, im);
+        zzprev = angle_zbar(re, im, wb);
+    }
+#pragma unroll 1
+    for (int k = 0; k < tz; k++) {
+        const int z = z0 + k, par = k & 1;
+        const size_t c = col + sz * z;
+        const int zn = z + 1 < nz ? z + 1 : 0;
+        const size_t cn = col + sz * zn;
+        // loads of this step: the next plane's cell, the halo rows, this cell's cotangents
+        const float4 prawn = psi[cn];
+        const float wn = MODE == OF_DIV ? w[cn] : 0.0f;
+        float4 phr = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pbr = phr;
+        float wh = 0.0f, wbm = 0.0f;
+        if (top) {
+            phr = psi[x + sy * yp + sz * z];
+            if (MODE == OF_DIV) wh = w[x + sy * yp + sz * z];
+        }
+        if (bottom) {
+            pbr = psi[x + sy * ym + sz * z];
+            wbm = MODE == OF_DIV ? w[x + sy * ym + sz * z] : w[3 * (x + sy * ym + sz * z) + 1];
+        }
+        float4 acc = psibar[c];
+        if (q) {
+            float sn, co;
+            sincosf(q[c], &sn, &co);
+            acc = make_float4(acc.x * co - acc.y * sn, acc.x * sn + acc.y * co, acc.z * co - acc.w * sn,
+                              acc.z * sn + acc.w * co);
+        }
+        float wv[3] = {0.0f, 0.0f, 0.0f};
+        if (MODE == OF_VEL) {
+            wv[0] = w[3 * c + 0];
+            wv[1] = w[3 * c + 1];
+            wv[2] = w[3 * c + 2];
+        }
+        ps[par][tid] = pc;
+        if (MODE == OF_DIV) wsh[par][tid] = wc;
+        const float4 pn = pre_of<PRE>(prawn, x, y, zn, g, j);
+        const float4 ph = top ? pre_of<PRE>(phr, x, yp, z, g, j) : phr;
+        const float4 pb = bottom ? pre_of<PRE>(pbr, x, ym, z, g, j) : pbr;
+        __syncthreads();
+        // the three forward edges (c, c + e_a)
+        const int rx = ty * nx, ru = top ? rx + x : rx + nx + x;  // the row above (in the block)
+        const float4 pa[3] = {ps[par][rx + xp], top ? ph : ps[par][ru], pn};
+        float wf[3];
+        if (MODE == OF_DIV) {
+            wf[0] = (wc - wsh[par][rx + xp]) * id2[0];
+            wf[1] = (wc - (top ? wh : wsh[par][ru])) * id2[1];
+            wf[2] = (wc - wn) * id2[2];
+        } else {
+            wf[0] = wv[0] * sc[0];
+            wf[1] = wv[1] * sc[1];
+            wf[2] = wv[2] * sc[2];
+        }
+        float2 zf[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float re, im;
+            edge_z(pc, pa[a], re, im);
+            zf[a] = angle_zbar(re, im, wf[a]);
+            add_conj_mul(acc, zf[a], pa[a]);  // c is the first end of (c, c + e_a)
+        }
+        zx_s[par][rx + xp] = zf[0];
+        if (!top) zy_s[par][ru] = zf[1];
+        __syncthreads();
+        // the three backward edges (c - e_a, c): c is their second end
+        add_mul(acc, zx_s[par][tid], ps[par][rx + xm]);
+        if (!bottom) {
+            add_mul(acc, zy_s[par][tid], ps[par][rx - nx + x]);
+        } else {
+            const float wb = MODE == OF_DIV ? (wbm - wc) * id2[1] : wbm * sc[1];
+            float re, im;
+            edge_z(pb, pc, re, im);
+            add_mul(acc, angle_zbar(re, im, wb), pb);
+        }
+        add_mul(acc, zzprev, pprev);
+        // finish the cell as k_oneform_adj does
+        if constexpr (RADJ) {
+            float4 nb = acc;
+            if (is_jet_cell(x, y, z, g, j)) {
+                double cs[4];
+                nb = restrict_adj_cell(acc, praw, x, y, z, g, j, cs);
+#pragma unroll
+                for (int i = 0; i < 4; i++) sums[i] += cs[i];
+            }
+            psibar[c] = nb;
+            if (qbar_out) qbar_out[c] = gauge_qbar(nb, praw);
+        } else {
+            if (PRE == PRE_NORMALIZE) acc = normalize_adj_at(praw, acc);
+            psibar[c] = acc;
+            if (qbar_out) qbar_out[c] = (acc.x * pc.y - acc.y * pc.x) + (acc.z * pc.w - acc.w * pc.z);
+        }
+        pprev = pc;
+        zzprev = zf[2];
+        praw = prawn;
+        pc = pn;
+        wc = wn;
+    }
+    if constexpr (RADJ) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) sums[i] = block_sum<kOfaThreads>(sums[i]);
+        if (tid == 0) {
+            double* o = partials + 4 * ((size_t)blockIdx.y * gridDim.x + blockIdx.x);
+#pragma unroll
+            for (int i = 0; i < 4; i++) o[i] = sums[i];
+        }
+    }
+}
+
 // Adjoint of restrict_velocity (jet cells) at its input psi_in, in place on psibar; per
 // block FP64 partial sums of phibar and phibar * pos_a (4 per block) for the gradient with
 // respect to kvec and omega t.  With qbar_out, also the gauge cotangent of the new psibar
@@ -1017,9 +1179,32 @@ int launch_restrict_to(const GridInfo& g, const float4* in, float4* out, const J
 int launch_gauge_qbar(const GridInfo& g, const float4* psibar, const float4* psi_out, float* qbar, hipStream_t s) {
     ISF_LAUNCH(k_gauge_qbar, grid1d(ncells(g), 256), dim3(256), s, psibar, psi_out, qbar, ncells(g));
 }
+// the tiled one-form adjoint: rows of 64 to 256 cells; z planes per block up to 16, halved
+// (down to 2) while the launch would have fewer than 1024 blocks
+static bool ofa_tiled(const GridInfo& g) { return g.nx >= 64 && g.nx <= 256; }
+static int ofa_tz(const GridInfo& g) {
+    const long rows = g.ny / (kOfaThreads / g.nx);
+    int tz = 16;
+    while (tz > 2 && rows * (g.nz / tz) < 1024) tz >>= 1;
+    return tz;
+}
+static dim3 ofa_grid(const GridInfo& g, int tz) { return dim3(g.ny / (kOfaThreads / g.nx), g.nz / tz); }
 int launch_oneform_adj_div(const GridInfo& g, float4* psibar, const float4* psi, const float* q, const float* fbar,
                            hipStream_t s, int pre, const JetParams* j) {
     const JetParams jp = j ? *j : JetParams{};
+    if (pre == PRE_RESTRICT && !j) return (int)hipErrorInvalidValue;
+    if (ofa_tiled(g)) {
+        const int tz = ofa_tz(g);
+        const dim3 grid = ofa_grid(g, tz), block(kOfaThreads);
+        if (pre == PRE_NORMALIZE)
+            ISF_LAUNCH((k_oneform_adj_tile<OF_DIV, PRE_NORMALIZE, false>), grid, block, s, psibar, psi, q, fbar, g,
+                       1.0f, 0, jp, tz, nullptr, nullptr);
+        if (pre == PRE_RESTRICT)
+            ISF_LAUNCH((k_oneform_adj_tile<OF_DIV, PRE_RESTRICT, false>), grid, block, s, psibar, psi, q, fbar, g,
+                       1.0f, 0, jp, tz, nullptr, nullptr);
+        ISF_LAUNCH((k_oneform_adj_tile<OF_DIV, PRE_NONE, false>), grid, block, s, psibar, psi, q, fbar, g, 1.0f, 0,
+                   jp, tz, nullptr, nullptr);
+    }
     if (pre == PRE_NORMALIZE)
         ISF_LAUNCH((k_oneform_adj<OF_DIV, PRE_NORMALIZE>), grid1d(ncells(g), 256), dim3(256), s, psibar, psi, q, fbar,
                    g, 1.0f, 0, jp, nullptr);
@@ -1033,6 +1218,11 @@ int launch_oneform_adj_div(const GridInfo& g, float4* psibar, const float4* psi,
 }
 int launch_oneform_adj_vel(const GridInfo& g, float4* psibar, const float4* psi, const float* velbar, float hbar,
                            int sharp, hipStream_t s, float* qbar_out) {
+    if (ofa_tiled(g)) {
+        const int tz = ofa_tz(g);
+        ISF_LAUNCH((k_oneform_adj_tile<OF_VEL, PRE_NONE, false>), ofa_grid(g, tz), dim3(kOfaThreads), s, psibar, psi,
+                   nullptr, velbar, g, hbar, sharp, JetParams{}, tz, qbar_out, nullptr);
+    }
     ISF_LAUNCH((k_oneform_adj<OF_VEL, PRE_NONE>), grid1d(ncells(g), 256), dim3(256), s, psibar, psi, nullptr, velbar,
                g, hbar, sharp, JetParams{}, qbar_out);
 }
@@ -1040,9 +1230,8 @@ size_t restrict_adj_blocks(const GridInfo& g) { return (ncells(g) + 255) / 256; 
 // the first reduction level's blocks (their sums follow the per-block rows in partials)
 constexpr size_t kReduce4Blocks = 128;
 size_t restrict_adj_partials(const GridInfo& g) { return 4 * restrict_adj_blocks(g) + 4 * kReduce4Blocks; }
-// out4 += the four sums of the per-block rows in partials (two levels, fixed order)
-static int restrict_adj_reduce(const GridInfo& g, double* partials, double* out4, hipStream_t s) {
-    const size_t nb = restrict_adj_blocks(g);
+// out4 += the four sums of the nb per-block rows in partials (two levels, fixed order)
+static int restrict_adj_reduce(size_t nb, double* partials, double* out4, hipStream_t s) {
     const size_t G = nb < kReduce4Blocks ? nb : kReduce4Blocks, chunk = (nb + G - 1) / G;
     double* part2 = partials + 4 * nb;
     hipLaunchKernelGGL(k_reduce4_part, dim3((unsigned)G), dim3(256), 0, s, partials, nb, chunk, part2);
@@ -1056,16 +1245,26 @@ int launch_restrict_adj(const GridInfo& g, float4* psibar, const float4* psi_in,
                        qbar_out);
     const int e = (int)hipGetLastError();
     if (e) return e;
-    return restrict_adj_reduce(g, partials, out4, s);
+    return restrict_adj_reduce(restrict_adj_blocks(g), partials, out4, s);
 }
 int launch_oneform_adj_div_restrict(const GridInfo& g, float4* psibar, const float4* psi, const float* q,
                                     const float* fbar, const JetParams& j, double* partials, double* out4,
                                     float* qbar_out, hipStream_t s) {
-    hipLaunchKernelGGL((k_oneform_adj<OF_DIV, PRE_RESTRICT, true>), grid1d(ncells(g), 256), dim3(256), 0, s, psibar,
-                       psi, q, fbar, g, 1.0f, 0, j, qbar_out, partials);
+    size_t nb = restrict_adj_blocks(g);
+    if (ofa_tiled(g)) {
+        // at most restrict_adj_blocks(g) blocks (each holds 512 cells or more)
+        const int tz = ofa_tz(g);
+        const dim3 grid = ofa_grid(g, tz);
+        nb = (size_t)grid.x * grid.y;
+        hipLaunchKernelGGL((k_oneform_adj_tile<OF_DIV, PRE_RESTRICT, true>), grid, dim3(kOfaThreads), 0, s, psibar,
+                           psi, q, fbar, g, 1.0f, 0, j, tz, qbar_out, partials);
+    } else {
+        hipLaunchKernelGGL((k_oneform_adj<OF_DIV, PRE_RESTRICT, true>), grid1d(ncells(g), 256), dim3(256), 0, s,
+                           psibar, psi, q, fbar, g, 1.0f, 0, j, qbar_out, partials);
+    }
     const int e = (int)hipGetLastError();
     if (e) return e;
-    return restrict_adj_reduce(g, partials, out4, s);
+    return restrict_adj_reduce(nb, partials, out4, s);
 }
 int launch_normalize_adj(const GridInfo& g, float4* psibar, const float4* psi_in, hipStream_t s) {
     ISF_LAUNCH(k_normalize_adj, grid1d(ncells(g), 256), dim3(256), s, psibar, psi_in, ncells(g));

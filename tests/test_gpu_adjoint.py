@@ -152,6 +152,37 @@ def test_vjp_velocity_one_form(env):
     assert e < 2e-5
 
 
+def test_vjp_one_forms_tiled():
+    """The pressure projection's and the velocity one-form's VJPs on rows of 64 cells, the
+    tiled edge-wise one-form adjoint, against FP64 autograd (32-cell rows take the
+    per-cell kernel, tested above)."""
+    from isf_amd import ISF, DeviceArray as DA
+    dims, box = (64, 16, 16), (8, 2, 2)
+    tr = TorchRollout(dims, box, 0.1, 1 / 48, CEN, LEN, RAD, 0, 7, 0, 0, 0, 0, 1, 0, (8, 8, 8), SRES, 0.25, 4.0,
+                      np.zeros(SRES[::-1]))
+    isf = ISF(box, dims, 0.1, 1 / 48)
+    rng = np.random.default_rng(12)
+    x = smooth_psi(dims, 13)
+    ybar = rng.standard_normal(x.shape)
+    (ref,) = torch_vjp(lambda t: to_r(tr.project(to_c(t))), [x], ybar)
+    b = dev(DA, ybar)
+    isf.vjp_pressure_project_(dev(DA, x), b)
+    e = rel(b.to_host(), ref)
+    vbar = rng.standard_normal(isf.shape3 + (3,))
+
+    def fn(t):
+        v = tr.one_form(to_c(t), 0.1)
+        return torch.stack([v[a] / tr.d[a] for a in range(3)], -1)
+
+    (ref_v,) = torch_vjp(fn, [x], vbar)
+    bv = dev(DA, np.zeros_like(x))
+    isf.vjp_velocity_one_form_(dev(DA, x), dev(DA, vbar), bv, hbar=0.1, sharp=True)
+    ev = rel(bv.to_host(), ref_v)
+    print(f"  tiled: pressure_project vjp rel {e:.2e}, velocity one-form vjp rel {ev:.2e}")
+    assert e < 2e-5
+    assert ev < 2e-5
+
+
 def test_vjp_staggered_advect(env):
     tr, isf, DA = env
     rng = np.random.default_rng(8)
@@ -177,20 +208,20 @@ def test_vjp_staggered_advect(env):
 
 # ---------------------------------------------------------------- rollout
 
-def rollout_case(steps=4, K=4, nslots=1500, n_init=1000, n_spawn=50):
+def rollout_case(steps=4, K=4, nslots=1500, n_init=1000, n_spawn=50, dims=DIMS, box=BOX):
     from isf_amd import ISF
     from isf_amd.rollout import Rollout, gaussian_target
     hb = 0.1
     v = np.array([1.0, 0.05, -0.03], np.float32)
     kvec = v / np.float32(hb)
     omega = np.float32(np.sum(v ** np.float32(2)) / (np.float32(2) * np.float32(hb)))
-    isf = ISF(BOX, DIMS, hb, 1 / 48)
+    isf = ISF(box, dims, hb, 1 / 48)
     isf.jet_configure(kvec, omega, np.array(CEN, np.float32), LEN, RAD, n_spawn, 7, 0)
-    tgt = gaussian_target(BOX, SRES, 3, centre_lo=(0.2, 0.6, 0.6), centre_hi=(0.8, 1.4, 1.4))
+    tgt = gaussian_target(box, SRES, 3, centre_lo=(0.2, 0.6, 0.6), centre_hi=(0.8, 1.4, 1.4))
     r = Rollout(isf, steps, checkpoint_every=K, constraint_iters=4, phase_res=(8, 8, 8), splat_res=SRES,
                 splat_sigma=0.25, splat_cutoff=4.0, nslots=nslots, n_init=n_init, fill_seed=9)
     r.set_target(tgt)
-    tr = TorchRollout(DIMS, BOX, hb, 1 / 48, CEN, LEN, RAD, n_spawn, 7, 0, nslots, n_init, 9, steps, 4, (8, 8, 8),
+    tr = TorchRollout(dims, box, hb, 1 / 48, CEN, LEN, RAD, n_spawn, 7, 0, nslots, n_init, 9, steps, 4, (8, 8, 8),
                       SRES, 0.25, 4.0, tgt)
     return isf, r, tr, v
 
@@ -265,11 +296,14 @@ def test_backward_twice_after_one_forward():
     assert np.array_equal(g1["parts_init"], g2["parts_init"])
 
 
-def test_rollout_gradients_match_autograd():
-    isf, r, tr, v = rollout_case()
+# (64, 16, 16): rows of 64 cells take the tiled one-form adjoints (every variant the
+# reverse step uses), (32, 16, 16) the per-cell ones
+@pytest.mark.parametrize("dims,box", [(DIMS, BOX), ((64, 16, 16), (8, 2, 2))])
+def test_rollout_gradients_match_autograd(dims, box):
+    isf, r, tr, v = rollout_case(dims=dims, box=box)
     rng = np.random.default_rng(1)
     th = (0.3 * rng.standard_normal((8, 8, 8))).astype(np.float32)
-    pi = smooth_psi(DIMS, 2).astype(np.float32)
+    pi = smooth_psi(dims, 2).astype(np.float32)
     loss = r.forward(v, theta=th, psi_init=pi)
     g = r.backward()
     L, gt = tr.grad(v.astype(np.float64), th.astype(np.float64), psi_init=pi.astype(np.float64),
