@@ -33,11 +33,11 @@ using namespace isf;
 static constexpr uint32_t kRolloutMagic = 0x2011A0D7u;
 
 // Per-step tape of one segment: step outputs inside the segment (mid), flow output psi1,
-// first gauge output psi3, gauge phases q1 q2, particles after the spawn p0, velocity
-// field of the step output vel (with particles only).
+// first gauge output psi3, gauge phases q1 q2, velocity field of the step output vel
+// (with particles only).
 struct Tape {
     std::vector<float4*> mid, psi1, psi3;
-    std::vector<float*> q1, q2, p0, vel;
+    std::vector<float*> q1, q2, vel;
 };
 // Scratch and stream of one chain of forward steps.
 struct Work {
@@ -63,9 +63,11 @@ struct isf_rollout {
     float* theta;
     bool use_theta;
     float* parts_init;
-    // checkpoints at steps 0, K, 2K, ... (ck_psi[0] = cpsi[C], ck_parts[0] = parts_init)
+    // checkpoints at steps 0, K, 2K, ... (ck_psi[0] = cpsi[C])
     std::vector<float4*> ck_psi;
-    std::vector<float*> ck_parts;
+    // particle trajectory of the forward (with particles): traj[i] = the state of step
+    // i + 1 after its spawn (its advection's input), traj[T] = the final state (parts)
+    std::vector<float*> traj;
     // segment tapes: set s % 2 holds segment s during the backward
     Tape tape[2];
     // recompute stream of the backward with its own scratch (spectrum / Poisson, f,
@@ -270,22 +272,24 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
     const size_t pb = sizeof(float) * 3 * (size_t)r->P;
     const int ks = seg_steps(r, s);
     if (k1 < 0 || k1 > ks) k1 = ks;
-    const bool parts = r->P > 0;
+    // with particles every pass forms the velocity fields (the particle adjoint reads
+    // them); the positions are stepped by the forward alone, into the trajectory, which
+    // the backward reads instead of recomputing it
+    const bool parts = r->P > 0, ptraj = parts && checkpoint;
     const VelOut vo0{nullptr, h->hbar, 1.0f / r->g.dx, 1.0f / r->g.dy, 1.0f / r->g.dz, 1};
+    // the state of step s K + k + 1 after its spawn; the advection of step k reads it and
+    // writes the next entry (out of place), so no step copies particles
+    auto pos = [&](int k) { return r->traj[(size_t)s * r->K + k]; };
     if (k0 == 0) {
         // the step outputs of tape set s % 2 are read by the velocities of segment s - 2
         if (side && s >= 2) HIP_CHECK(hipStreamWaitEvent(st, r->fseg[s % 2], 0), "hipStreamWaitEvent");
-        // particles: tape entry p0[k] holds step k's state after its spawn; the advection
-        // of step k writes p0[k + 1] (the segment's last step: w.parts), so no step copies
-        // them
-        if (parts)
-            HIP_CHECK(hipMemcpyAsync(tp.p0[0], r->ck_parts[s], pb, hipMemcpyDeviceToDevice, pt),
-                      "particles checkpoint");
+        if (ptraj && s == 0)
+            HIP_CHECK(hipMemcpyAsync(pos(0), r->parts_init, pb, hipMemcpyDeviceToDevice, pt), "initial particles");
     }
     // :63 of step k (velocity in its tape entry) on the particle stream
     auto advect = [&](int k) -> int {
-        LAUNCH_CHECK(launch_advect(r->g, k + 1 < ks ? tp.p0[k + 1] : w.parts, r->P, nullptr, tp.vel[k], h->dt, pt,
-                                   Batch{}, 0, tp.p0[k], side != nullptr),
+        LAUNCH_CHECK(launch_advect(r->g, pos(k + 1), r->P, nullptr, tp.vel[k], h->dt, pt, Batch{}, 0, pos(k),
+                                   side != nullptr),
                      "staggered_advect");
         return ISF_OK;
     };
@@ -300,15 +304,17 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
             LAUNCH_CHECK(launch_flow_x_fwd(r->g, in, w.S, h->tb, st, Batch{}, &vo, side ? r->fstep : nullptr),
                          "flow x forward + velocity_one_form");
             if (side) HIP_CHECK(hipStreamWaitEvent(side, r->fstep, 0), "hipStreamWaitEvent");
-            int e = advect(k - 1);
-            if (e) return e;
+            if (ptraj) {
+                int e = advect(k - 1);
+                if (e) return e;
+            }
         } else {
             LAUNCH_CHECK(launch_flow_x_fwd(r->g, in, w.S, h->tb, st), "flow x forward");
         }
         long long first0, nsp;
         spawn_slots(r, it, &first0, &nsp);
-        if (nsp > 0) {
-            int e2 = spawn_step(r, it, first0, nsp, tp.p0[k], pt);
+        if (ptraj && nsp > 0) {
+            int e2 = spawn_step(r, it, first0, nsp, pos(k), pt);
             if (e2) return e2;
         }
         LAUNCH_CHECK(launch_flow_plane(r->g, w.S, h->tb, st), "flow plane");
@@ -328,11 +334,11 @@ static int seg_forward(isf_rollout_t r, int s, const Tape& tp, const Work& w, bo
             HIP_CHECK(hipStreamWaitEvent(side, r->fstep, 0), "hipStreamWaitEvent");
         }
         LAUNCH_CHECK(launch_velocity(r->g, psi0_of(r, tp, s, ks), h->hbar, 1, tp.vel[ks - 1], pt), "velocity_one_form");
-        int e = advect(ks - 1);
-        if (e) return e;
+        if (ptraj) {
+            int e = advect(ks - 1);
+            if (e) return e;
+        }
     }
-    if (checkpoint && s + 1 < r->nseg && parts)
-        HIP_CHECK(hipMemcpyAsync(r->ck_parts[s + 1], w.parts, pb, hipMemcpyDeviceToDevice, pt), "particles checkpoint");
     if (side) HIP_CHECK(hipEventRecord(r->fseg[s % 2], side), "hipEventRecord");
     return ISF_OK;
 }
@@ -349,10 +355,11 @@ static int step_adjoint_particles(isf_rollout_t r, const Tape& tp, int s, int k,
     LAUNCH_CHECK(launch_fill_f32(vb, 0.0f, 3 * r->N, st), "fill");
     if (r->P > 0) {
         // cell-sorted slot order: long same-cell runs for the scatter's wavefront reduction
-        LAUNCH_CHECK(launch_cell_order(g, tp.p0[k], r->P, r->adj_order, r->adj_order_scratch,
+        const float* pk = r->traj[(size_t)s * r->K + k];  // the step's advection input
+        LAUNCH_CHECK(launch_cell_order(g, pk, r->P, r->adj_order, r->adj_order_scratch,
                                                  r->adj_order_bytes, st),
                      "particle order");
-        LAUNCH_CHECK(launch_advect_adj(g, tp.p0[k], r->P, r->adj_order, tp.vel[k], h->dt, r->pbar, vb, st),
+        LAUNCH_CHECK(launch_advect_adj(g, pk, r->P, r->adj_order, tp.vel[k], h->dt, r->pbar, vb, st),
                      "advect adjoint");
         long long first0, nsp;
         spawn_slots(r, it, &first0, &nsp);
@@ -454,12 +461,12 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
     RA(&r->theta, no_phase ? 1 : (size_t)d.phase_res[0] * d.phase_res[1] * d.phase_res[2]);
     RA(&r->parts_init, P3);
     r->ck_psi.resize(r->nseg + 1);
-    r->ck_parts.resize(r->nseg + 1);
     r->ck_psi[0] = r->cpsi[r->C];
-    r->ck_parts[0] = r->parts_init;
     for (int s = 1; s <= r->nseg; s++) RA(&r->ck_psi[s], N);
-    for (int s = 1; s < r->nseg; s++) RA(&r->ck_parts[s], P3);
-    r->ck_parts[r->nseg] = nullptr;
+    if (r->P > 0) {
+        r->traj.resize((size_t)r->T + 1);
+        for (auto& p : r->traj) RA(&p, P3);
+    }
     for (Tape& tp : r->tape) {
         tp.mid.resize(r->K - 1);
         for (auto& p : tp.mid) RA(&p, N);
@@ -467,20 +474,18 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
         tp.psi3.resize(r->K);
         tp.q1.resize(r->K);
         tp.q2.resize(r->K);
-        tp.p0.resize(r->K);
         tp.vel.resize(r->P > 0 ? r->K : 0);
         for (int k = 0; k < r->K; k++) {
             RA(&tp.psi1[k], N);
             RA(&tp.psi3[k], N);
             RA(&tp.q1[k], N);
             RA(&tp.q2[k], N);
-            RA(&tp.p0[k], P3);
             if (r->P > 0) RA(&tp.vel[k], 3 * N);
         }
     }
     RA(&r->rw.S, 2 * N);
     RA(&r->rw.f, N);
-    RA(&r->rw.parts, P3);
+    r->rw.parts = nullptr;  // the recompute steps no particles
     RA(&r->psibar, N);
     RA(&r->f, N);
     RA(&r->fbar, N);
@@ -488,7 +493,10 @@ int isf_rollout_create(isf_rollout_t* out, isf_t h, const isf_rollout_desc* desc
         r->vbar[i].resize(r->K);
         for (auto& p : r->vbar[i]) RA(&p, 3 * N);
     }
-    RA(&r->parts, P3);
+    if (r->P > 0)
+        r->parts = r->traj[r->T];  // the forward's final state
+    else
+        RA(&r->parts, P3);
     RA(&r->parts_final, P3);
     RA(&r->pbar, P3);
     RA(&r->phibar, N);
