@@ -212,9 +212,11 @@ def b_alg_fwd_adjoint(N, P, T, K):
     DAG of one simloop step; each array read or written once per fused kernel, the
     Poisson half spectrum counted as 4 B/cell each way; the set-up constraint iterations
     are not counted, their time is):
-      forward step   248 N + 24 P (the step, whose velocity field is written to the tape)
+      forward step   248 N + 24 P (the step, whose velocity field is written to the tape;
+                     its advection writes the particle trajectory the backward reads)
                      + 24 N tape (raw flow output, q1, q2)
-      recompute      the same, for the T - K steps outside the last segment
+      recompute      the same without the particles (248 + 24) N, for the T - K steps
+                     outside the last segment
       reverse step   psi chain 324 N: velocity adjoint 64 (psibar, psi, vbar -> psibar,
                      qbar), two Poisson solves 2 x 24, two one-form/div adjoints 56 + 60 (the
                      second with the restrict adjoint and its gauge cotangent fused), flow
@@ -222,8 +224,9 @@ def b_alg_fwd_adjoint(N, P, T, K):
                      vbar written 12, taped velocity read 12, positions 12 P, pbar read and
                      written 24 P."""
     fwd = 272 * N + 24 * P
+    rec = 272 * N
     rev = 348 * N + 36 * P
-    return fwd + fwd * (T - K) / T + rev
+    return fwd + rec * (T - K) / T + rev
 
 
 def bench_fwd_adjoint(reps, dims=(128, 64, 64), box=(8, 4, 4), nslots=100_000, splat_res=(32, 16, 16)):
