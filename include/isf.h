@@ -318,7 +318,9 @@ int isf_vjp_staggered_advect(isf_t h, const float* parts0, long long nslots, con
  * (1 / nslots) sum over live slots of exp(-|y - p|^2 / (2 sigma^2)) for |y - p| <=
  * cutoff sigma, y the centres of splat_res cells over the box.
  * The backward pass runs the hand-written adjoint of every operator in reverse, with psi
- * and particles checkpointed every checkpoint_every steps and each segment recomputed. */
+ * checkpointed every checkpoint_every steps and each segment's psi chain recomputed; the
+ * forward keeps the whole particle trajectory ((steps + 1) x 12 nslots bytes), which the
+ * backward reads. */
 typedef struct {
     int steps;                 /* simloop bodies per rollout, >= 1 */
     int checkpoint_every;      /* K >= 1; K >= steps keeps the whole tape of one segment */
