@@ -146,8 +146,9 @@ def test_pointwise_and_stencils(isf_mod, n, L):
 
 
 # z planes of psi above 1 MB (nx ny > 65536): div(one_form) runs as its own z-marching
-# stencil into f before the R2C x pass instead of fused into it
-@pytest.mark.parametrize("n,L", [((512, 256, 16), (8, 4, 1)), ((256, 512, 16), (4, 8, 1))])
+# stencil into f before the R2C x pass instead of fused into it; (512, 64, 32): the fused
+# pass on the longest rows it takes (eight 64-cell segments per row)
+@pytest.mark.parametrize("n,L", [((512, 256, 16), (8, 4, 1)), ((256, 512, 16), (4, 8, 1)), ((512, 64, 32), (16, 2, 1))])
 def test_pressure_project_wide_planes(isf_mod, n, L):
     print(f"[project wide] n={n}", flush=True)
     isf = isf_mod.ISF(L, n, 0.1, 1 / 48)
